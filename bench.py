@@ -1,0 +1,215 @@
+"""Benchmark: batched elastic FWI gradient on one MI355X.
+
+Workload (the 500x1700 configuration): grid 500 x 1700 at 10 m plus a
+40-cell C-PML (608 x 1780 padded), nt = 4000, dt = 1 ms, 64 shots, 1700
+receivers, 12 Hz Ricker, seeded layered model with 3 elliptical anomalies.
+One "step" = one full gradient evaluation (calc_id 1: forward with boundary
+store, residual, time-reversed re-materialisation + adjoint + gradient) of all
+64 shots, with the model, source functions and observed data already resident
+in HBM.  value = shot-gradients / s.
+
+Prints ONE JSON line.  Extra fields: roofline (dominant kernel, HIP-event
+timed inside the timed region), job_roofline (whole gradient against the
+algorithmic bytes of a shot-gradient), cpu_baseline (the CPU oracle on a
+bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "fwiflow.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per cell and shot for each kernel (FP32): every array the
+# kernel must touch, read or written once; model parameters are shared by all
+# shots of a launch and amortised to zero; C-PML memory variables add 8 B per
+# read-modify-write where the kernel updates them.
+KERNEL_BYTES = {
+    "stress_fwd": 32.0,   # read vz vx szz sxx sxz, write szz sxx sxz
+    "vel_fwd": 28.0,      # read szz sxx sxz vz vx, write vz vx
+    "rev_vel": 36.0,      # read 3 stress, 2 velocity, 2 adjoint velocity, write 2 velocity
+    "rev_stress": 44.0,   # read 2 velocity, 3 stress, 3 adjoint stress, write 3 stress
+    "adj_vel": 28.0,      # read 3 adjoint stress, 2 adjoint velocity, write 2
+    "adj_stress": 32.0,   # read 2 adjoint velocity, 3 adjoint stress, write 3
+}
+# C-PML memory-variable traffic per PML cell-direction (2 variables RW = 16 B)
+KERNEL_PML = {"stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--nt", type=int, default=4000)
+    p.add_argument("--nshots", type=int, default=64)
+    p.add_argument("--cpu-sample-steps", type=int, default=300)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def geometry_counts(c):
+    """Active cells, PML cell-directions and per-kernel domain sizes."""
+    nzA = c.NZ - c.nPad
+    NX = c.NX
+    active = nzA * NX
+    pml_z = 2 * c.nPml * NX
+    pml_x = 2 * c.nPml * (nzA - 2 * c.nPml)
+    upd = (nzA - 4) * (NX - 4)  # stencil update region [2, n-3]^2
+    ring_rect = (nzA - 2 * c.nPml + 4) * (NX - 2 * c.nPml + 4)
+    return dict(active=active, pml_dirs=pml_z + pml_x, update=upd, ring_rect=ring_rect,
+                pml_z=pml_z, pml_x=pml_x)
+
+
+def algorithmic_bytes_per_shot_gradient(c, nt):
+    """168 B per active cell per step (forward 52 + re-materialised forward 52 +
+    adjoint with gradient 64) plus 96 B per C-PML cell-direction per step."""
+    g = geometry_counts(c)
+    return (168.0 * g["active"] + 96.0 * g["pml_dirs"]) * nt
+
+
+def kernel_launch_bytes(name, c, ns):
+    g = geometry_counts(c)
+    cells = g["ring_rect"] if name.startswith("rev_") else g["update"]
+    b = KERNEL_BYTES[name] * cells * ns
+    b += KERNEL_PML.get(name, 0.0) * g["pml_dirs"] * ns
+    return b
+
+
+def cpu_baseline(c, ws, sample_steps):
+    from fwiflow import models
+    from oracle import oracle
+
+    oracle.build()
+    cores = min(16, os.cpu_count() or 1)
+    cs = models.case3(nSteps=sample_steps, nshots=c.src_x.size)
+    para, stf = models.write_case(cs, os.path.join(ws, "cpu"))
+    lam, mu, den = models.moduli(cs, "true")
+    oracle.cufd(lam, mu, den, stf, 2, [0], para, precision=32, nthreads=cores)
+    lam0, mu0, den0 = models.moduli(cs, "start")
+    t0 = time.perf_counter()
+    oracle.cufd(lam0, mu0, den0, stf, 1, [0], para, precision=32, nthreads=cores)
+    t = time.perf_counter() - t0
+    per_grad = t * (c.nSteps / sample_steps)
+    return dict(value=1.0 / per_grad, unit="shot-gradients/s", cores=cores, kind="port",
+                sample=f"1 shot of the same grid, first {sample_steps} of {c.nSteps} steps "
+                       f"(calc_id 1, FP32 reference arithmetic, OpenMP), {t:.1f} s, "
+                       f"scaled linearly to {c.nSteps} steps")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(shots would shard one group per GPU, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import fwiflow
+    from fwiflow import models
+
+    c = models.case3(nSteps=a.nt, nshots=a.nshots)
+    ws = tempfile.mkdtemp(prefix="fwiflow_bench_")
+    para, stf = models.write_case(c, ws)
+    ids = list(range(a.nshots))
+    log(f"bench: grid {c.NZ}x{c.NX} nt {c.nSteps} shots {a.nshots}")
+    P = fwiflow.Plan(para, ids)
+    lam, mu, den = models.moduli(c, "true")
+    P.set_model(lam, mu, den)
+    P.set_stf(stf)
+    t0 = time.perf_counter()
+    P.run(2)
+    log(f"bench: observed data (forward, {a.nshots} shots) {time.perf_counter() - t0:.2f} s")
+    P.set_obs(P.data())
+    lam0, mu0, den0 = models.moduli(c, "start")
+    P.set_model(lam0, mu0, den0)
+    for i in range(a.warmup):
+        t0 = time.perf_counter()
+        P.run(1)
+        log(f"bench: warmup {i} {time.perf_counter() - t0:.2f} s")
+    P.set_timing(True)
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        P.run(1)  # synchronises the plan's stream before returning
+        log(f"bench: step {i} done at {time.perf_counter() - t0:.2f} s")
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / a.steps * 1e3
+    value = a.nshots * a.steps / elapsed
+
+    # per-kernel HIP-event timings (every 8th step is bracketed)
+    kt = {}
+    for name in KERNEL_BYTES:
+        ms, n = P.timing(name)
+        if n:
+            kt[name] = (ms / n, n)
+    launches = c.nSteps - 1
+    dominant = max(kt, key=lambda k: kt[k][0] * launches)
+    avg_ms = kt[dominant][0]
+    lb = kernel_launch_bytes(dominant, c, a.nshots)
+    achieved = lb / (avg_ms * 1e-3) / 1e9
+    grad_bytes = algorithmic_bytes_per_shot_gradient(c, c.nSteps)
+    job_gbs = grad_bytes * value / 1e9
+    g = geometry_counts(c)
+    out = {
+        "metric": "shot-gradients/s at 500x1700 nt=4000, % HBM peak; time-to-misfit vs plain",
+        "value": round(value, 4),
+        "unit": "shot-gradients/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"elastic FWI gradient, 500x1700 grid (+40 C-PML -> {c.NZ}x{c.NX}), "
+                        f"nt={c.nSteps}, dt=1 ms, {a.nshots} shots x 1700 receivers, "
+                        "calc_id 1 with observed data resident in HBM",
+            "grid": [c.NZ, c.NX],
+            "nt": c.nSteps,
+            "shots": a.nshots,
+        },
+        "gcell_updates_per_s": round(g["active"] * (c.nSteps - 1) * value / 1e9, 3),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dominant,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "avg_launch_ms": round(avg_ms, 4),
+            "bytes_per_launch": lb,
+        },
+        "job_roofline": {
+            "algorithmic_bytes_per_shot_gradient": grad_bytes,
+            "achieved": round(job_gbs, 1),
+            "frac": round(job_gbs / HBM_PEAK_GBS, 4),
+            "unit": "GB/s",
+        },
+        "kernel_avg_ms": {k: round(v[0], 4) for k, v in kt.items()},
+        "device_bytes": P.device_bytes(),
+    }
+    P.close()
+    if not a.no_cpu_baseline:
+        log("bench: CPU baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(c, ws, a.cpu_sample_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

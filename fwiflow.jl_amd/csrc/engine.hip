@@ -1,0 +1,1105 @@
+// gfx950 engine of the elastic FWI operator.
+//
+// Device data layout (all FP32, z fastest so a wave64 row is 64 consecutive
+// depth cells, one 256-byte coalesced access):
+//   model      lam, mu, ave_mu, byc_a, byc_b          [NX][NZ]
+//   wavefield  vz, vx, szz, sxx, sxz  per shot        [shot][NX][NZ]
+//   adjoint    same 5 components                      [shot][NX][NZ]
+//   C-PML memory variables (8), per shot              [shot][NX][NZ]
+//   boundary ring, 5 layers x 4 sides, 5 components   [step][component][shot][ring]
+//   seismograms syn/obs/res, time-major               [nSteps][all receivers]
+//   gradient accumulators (5, summed over shots)      [NX][NZ]
+// One thread owns one cell and loops over every shot of the group: the model
+// parameters and C-PML coefficients are read once per cell per step and the
+// gradient contributions of all shots are summed in registers before one
+// read-modify-write per step.
+//
+// Reference kernels restated here: el_stress.cu:24-131, el_velocity.cu:21-117,
+// el_velocity_adj.cu:22-108, el_stress_adj.cu:22-104, utilities.cu:125-205,
+// 361-424, 521-593, 653-747; driver sequence libCUFD.cu:157-463.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace fwiflow {
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return Status::Err(FWIFLOW_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TRY(expr)            \
+  do {                       \
+    Status s_ = (expr);      \
+    if (!s_.ok()) return s_; \
+  } while (0)
+
+namespace {
+
+constexpr int kBZ = 64;  // threads along z (one wave64 row)
+constexpr int kBX = 4;   // rows of x per workgroup -> 256 threads
+constexpr float kC1 = 9.0f / 8.0f;
+constexpr float kC2 = 1.0f / 24.0f;
+
+struct Geo {
+  int nz, nx, nzA, nPml, ns, nSteps;
+  long long N;  // cells per shot
+  float dt, rdz, rdx;
+  int ringW, ringH, ringlen;
+};
+
+struct Mdl {
+  const float *lam, *mu, *amu, *ba, *bb;
+};
+
+struct Pml {
+  const float *Kz, *az, *bz, *Kzh, *azh, *bzh;  // length nzA
+  const float *Kx, *ax, *bx, *Kxh, *axh, *bxh;  // length nx
+};
+
+struct Wave {
+  float *vz, *vx, *szz, *sxx, *sxz;
+};
+
+// C-PML memory variables, named after the reference's d_mem_* arrays
+struct Mem {
+  float* vzz;   // mem_dvz_dz
+  float* vzx;   // mem_dvz_dx
+  float* vxz;   // mem_dvx_dz
+  float* vxx;   // mem_dvx_dx
+  float* szz;   // mem_dszz_dz
+  float* sxx;   // mem_dsxx_dx
+  float* sxzz;  // mem_dsxz_dz
+  float* sxzx;  // mem_dsxz_dx
+};
+
+struct Src {
+  const int* zx;     // [ns][2] grid indices of the source
+  const float* inj;  // [ns][nSteps] stress increment 1500^2 * stf_w(t) * dt
+};
+
+// Boundary ring (Boundary.cu:17-41, utilities.cu:361-424): rows and columns
+// nPml-2 .. nPml+2 on each side of the interior, i.e. the band of width 5 of
+// the rectangle [nPml-2, nzA-nPml+1] x [nPml-2, nx-nPml+1].  Slot numbering is
+// this engine's own: top band [W][5], bottom band [W][5], left band [5][H-10],
+// right band [5][H-10], z fastest inside each so a wave writes contiguously.
+__host__ __device__ inline int ring_slot(int z, int x, const Geo& g) {
+  const int zr = z - (g.nPml - 2), xr = x - (g.nPml - 2);
+  const int H = g.ringH, W = g.ringW;
+  if (zr < 0 || zr >= H || xr < 0 || xr >= W) return -1;
+  if (zr < 5) return xr * 5 + zr;
+  if (zr >= H - 5) return 5 * W + xr * 5 + (zr - (H - 5));
+  const int hz = H - 10, zc = zr - 5;
+  if (xr < 5) return 10 * W + xr * hz + zc;
+  if (xr >= W - 5) return 10 * W + 5 * hz + (xr - (W - 5)) * hz + zc;
+  return -1;
+}
+
+__device__ inline float* ring_ptr(float* ring, const Geo& g, int it, int comp, int s, int slot) {
+  return ring + (((size_t)it * 5 + comp) * g.ns + s) * (size_t)g.ringlen + slot;
+}
+
+// ------------------------------------------------------------------ forward
+// el_stress.cu:50-88 + add_source (utilities.cu:521-555) + from_bnd
+// (utilities.cu:361-391): the ring cells save sigma^it and v^it before the
+// update of step it.
+__global__ __launch_bounds__(256) void k_stress_fwd(Geo g, Mdl m, Pml c, Wave w, Mem p, Src src,
+                                                    int it, float* __restrict__ ring) {
+  const int z = blockIdx.x * kBZ + threadIdx.x;
+  const int x = blockIdx.y * kBX + threadIdx.y;
+  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i];
+  const float l2m = lam + 2.f * mu;
+  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
+  const bool px = x < g.nPml || x > g.nx - g.nPml - 1;
+  float bz = 0.f, az = 0.f, rkz = 1.f, bzh = 0.f, azh = 0.f, rkzh = 1.f;
+  float bx = 0.f, ax = 0.f, rkx = 1.f, bxh = 0.f, axh = 0.f, rkxh = 1.f;
+  if (pz) {
+    bz = c.bz[z], az = c.az[z], rkz = 1.f / c.Kz[z];
+    bzh = c.bzh[z], azh = c.azh[z], rkzh = 1.f / c.Kzh[z];
+  }
+  if (px) {
+    bx = c.bx[x], ax = c.ax[x], rkx = 1.f / c.Kx[x];
+    bxh = c.bxh[x], axh = c.axh[x], rkxh = 1.f / c.Kxh[x];
+  }
+  const int slot = ring ? ring_slot(z, x, g) : -1;
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  for (int s = 0; s < g.ns; ++s) {
+    const long long o = s * g.N + i;
+    const float* vz = w.vz + o;
+    const float* vx = w.vx + o;
+    const float vz0 = vz[0], vx0 = vx[0];
+    float dvz_dz = (kC1 * (vz0 - vz[-1]) - kC2 * (vz[1] - vz[-2])) * rdz;
+    float dvx_dx = (kC1 * (vx0 - vx[-nz]) - kC2 * (vx[nz] - vx[-2 * nz])) * rdx;
+    float dvx_dz = (kC1 * (vx[1] - vx0) - kC2 * (vx[2] - vx[-1])) * rdz;
+    float dvz_dx = (kC1 * (vz[nz] - vz0) - kC2 * (vz[2 * nz] - vz[-nz])) * rdx;
+    if (pz) {
+      const float a = bz * p.vzz[o] + az * dvz_dz;
+      p.vzz[o] = a;
+      dvz_dz = dvz_dz * rkz + a;
+      const float b = bzh * p.vxz[o] + azh * dvx_dz;
+      p.vxz[o] = b;
+      dvx_dz = dvx_dz * rkzh + b;
+    }
+    if (px) {
+      const float a = bx * p.vxx[o] + ax * dvx_dx;
+      p.vxx[o] = a;
+      dvx_dx = dvx_dx * rkx + a;
+      const float b = bxh * p.vzx[o] + axh * dvz_dx;
+      p.vzx[o] = b;
+      dvz_dx = dvz_dx * rkxh + b;
+    }
+    float szz = w.szz[o], sxx = w.sxx[o], sxz = w.sxz[o];
+    if (slot >= 0) {
+      *ring_ptr(ring, g, it, 0, s, slot) = szz;
+      *ring_ptr(ring, g, it, 1, s, slot) = sxz;
+      *ring_ptr(ring, g, it, 2, s, slot) = sxx;
+      *ring_ptr(ring, g, it, 3, s, slot) = vz0;
+      *ring_ptr(ring, g, it, 4, s, slot) = vx0;
+    }
+    szz += (l2m * dvz_dz + lam * dvx_dx) * dt;
+    sxx += (lam * dvz_dz + l2m * dvx_dx) * dt;
+    sxz += amu * (dvx_dz + dvz_dx) * dt;
+    if (z == src.zx[2 * s] && x == src.zx[2 * s + 1]) {
+      const float inj = src.inj[(size_t)s * g.nSteps + it];
+      szz += inj;
+      sxx += 3.f * inj;
+    }
+    w.szz[o] = szz;
+    w.sxx[o] = sxx;
+    w.sxz[o] = sxz;
+  }
+}
+
+// el_velocity.cu:45-82.  x-PML test is `x > nx - nPml` (el_velocity.cu:56,71).
+__global__ __launch_bounds__(256) void k_vel_fwd(Geo g, Mdl m, Pml c, Wave w, Mem p) {
+  const int z = blockIdx.x * kBZ + threadIdx.x;
+  const int x = blockIdx.y * kBX + threadIdx.y;
+  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const float ba = m.ba[i], bb = m.bb[i];
+  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
+  const bool px = x < g.nPml || x > g.nx - g.nPml;
+  float bz = 0.f, az = 0.f, rkz = 1.f, bzh = 0.f, azh = 0.f, rkzh = 1.f;
+  float bx = 0.f, ax = 0.f, rkx = 1.f, bxh = 0.f, axh = 0.f, rkxh = 1.f;
+  if (pz) {
+    bz = c.bz[z], az = c.az[z], rkz = 1.f / c.Kz[z];
+    bzh = c.bzh[z], azh = c.azh[z], rkzh = 1.f / c.Kzh[z];
+  }
+  if (px) {
+    bx = c.bx[x], ax = c.ax[x], rkx = 1.f / c.Kx[x];
+    bxh = c.bxh[x], axh = c.axh[x], rkxh = 1.f / c.Kxh[x];
+  }
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  for (int s = 0; s < g.ns; ++s) {
+    const long long o = s * g.N + i;
+    const float* szz = w.szz + o;
+    const float* sxx = w.sxx + o;
+    const float* sxz = w.sxz + o;
+    const float sxz0 = sxz[0];
+    float dszz_dz = (kC1 * (szz[1] - szz[0]) - kC2 * (szz[2] - szz[-1])) * rdz;
+    float dsxz_dx = (kC1 * (sxz0 - sxz[-nz]) - kC2 * (sxz[nz] - sxz[-2 * nz])) * rdx;
+    float dsxz_dz = (kC1 * (sxz0 - sxz[-1]) - kC2 * (sxz[1] - sxz[-2])) * rdz;
+    float dsxx_dx = (kC1 * (sxx[nz] - sxx[0]) - kC2 * (sxx[2 * nz] - sxx[-nz])) * rdx;
+    if (pz) {
+      const float a = bzh * p.szz[o] + azh * dszz_dz;
+      p.szz[o] = a;
+      dszz_dz = dszz_dz * rkzh + a;
+      const float b = bz * p.sxzz[o] + az * dsxz_dz;
+      p.sxzz[o] = b;
+      dsxz_dz = dsxz_dz * rkz + b;
+    }
+    if (px) {
+      const float a = bx * p.sxzx[o] + ax * dsxz_dx;
+      p.sxzx[o] = a;
+      dsxz_dx = dsxz_dx * rkx + a;
+      const float b = bxh * p.sxx[o] + axh * dsxx_dx;
+      p.sxx[o] = b;
+      dsxx_dx = dsxx_dx * rkxh + b;
+    }
+    w.vz[o] += (dszz_dz + dsxz_dx) * ba * dt;
+    w.vx[o] += (dsxz_dz + dsxx_dx) * bb * dt;
+  }
+}
+
+// recording (utilities.cu:557-567): szz + 3 sxx at each receiver, written to
+// row `t` of the time-major seismogram array.
+__global__ void k_record(Geo g, const float* __restrict__ szz, const float* __restrict__ sxx,
+                         const int* __restrict__ rec, int R, float* __restrict__ row) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int z = rec[3 * r], x = rec[3 * r + 1], s = rec[3 * r + 2];
+  const long long o = s * g.N + (long long)x * g.nz + z;
+  row[r] = szz[o] + 3.f * sxx[o];
+}
+
+// Residual (libCUFD.cu:258-313 without the optional filters): window obs and
+// syn with w^2, res = obs_w - syn_w (first sample 0, gpuMinus utilities.cu:154-167),
+// sum res^2 (cuda_cal_objective, here in FP64), then window res again.
+__global__ __launch_bounds__(256) void k_residual(int nSteps, int R, const float* __restrict__ w2,
+                                                  const float* __restrict__ obs,
+                                                  const float* __restrict__ syn,
+                                                  float* __restrict__ res,
+                                                  double* __restrict__ partial) {
+  __shared__ double red[256];
+  const size_t total = (size_t)nSteps * R;
+  double acc = 0.0;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int t = (int)(k / (size_t)R);
+    const float ww = w2[t];
+    const float rv = t > 0 ? obs[k] * ww - syn[k] * ww : 0.f;
+    acc += (double)rv * (double)rv;
+    res[k] = rv * ww;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, int n, double* out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) acc += partial[k];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// ------------------------------------------------------------------ backward
+// Reverse-time re-materialisation (el_velocity.cu:84-117 with isFor = false)
+// fused with the ring restore of v (to_bnd, utilities.cu:394-424) and the
+// density-gradient accumulation.  The thread domain is the ring rectangle;
+// interior cells update, ring cells take the stored values.
+// acc_a += vz_adj (dszz/dz + dsxz/dx), acc_b += vx_adj (dsxz/dz + dsxx/dx).
+__global__ __launch_bounds__(256) void k_rev_vel(Geo g, Mdl m, Wave w, Wave a,
+                                                 float* __restrict__ acc_a,
+                                                 float* __restrict__ acc_b,
+                                                 const float* __restrict__ ring, int it) {
+  const int z = blockIdx.x * kBZ + threadIdx.x + (g.nPml - 2);
+  const int x = blockIdx.y * kBX + threadIdx.y + (g.nPml - 2);
+  if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
+  const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
+  const int slot = ring_slot(z, x, g);
+  if (!in && slot < 0) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const float ba = m.ba[i], bb = m.bb[i];
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  float ga = 0.f, gb = 0.f;
+  for (int s = 0; s < g.ns; ++s) {
+    const long long o = s * g.N + i;
+    if (in) {
+      const float* szz = w.szz + o;
+      const float* sxx = w.sxx + o;
+      const float* sxz = w.sxz + o;
+      const float sxz0 = sxz[0];
+      const float dszz_dz = (kC1 * (szz[1] - szz[0]) - kC2 * (szz[2] - szz[-1])) * rdz;
+      const float dsxz_dx = (kC1 * (sxz0 - sxz[-nz]) - kC2 * (sxz[nz] - sxz[-2 * nz])) * rdx;
+      const float dsxz_dz = (kC1 * (sxz0 - sxz[-1]) - kC2 * (sxz[1] - sxz[-2])) * rdz;
+      const float dsxx_dx = (kC1 * (sxx[nz] - sxx[0]) - kC2 * (sxx[2 * nz] - sxx[-nz])) * rdx;
+      const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
+      ga += a.vz[o] * fz;
+      gb += a.vx[o] * fx;
+      if (slot < 0) {
+        w.vz[o] -= fz * ba * dt;
+        w.vx[o] -= fx * bb * dt;
+      }
+    }
+    if (slot >= 0) {
+      w.vz[o] = *ring_ptr(const_cast<float*>(ring), g, it, 3, s, slot);
+      w.vx[o] = *ring_ptr(const_cast<float*>(ring), g, it, 4, s, slot);
+    }
+  }
+  if (in) {
+    acc_a[i] += ga;
+    acc_b[i] += gb;
+  }
+}
+
+// el_stress.cu:90-131 (isFor = false) fused with add_source(-) and the ring
+// restore of sigma.  acc_l += (szz_adj + sxx_adj)(dvz/dz + dvx/dx),
+// acc_m += szz_adj dvz/dz + sxx_adj dvx/dx, acc_s += sxz_adj (dvx/dz + dvz/dx).
+__global__ __launch_bounds__(256) void k_rev_stress(Geo g, Mdl m, Wave w, Wave a, Src src,
+                                                    float* __restrict__ acc_l,
+                                                    float* __restrict__ acc_m,
+                                                    float* __restrict__ acc_s,
+                                                    const float* __restrict__ ring, int it) {
+  const int z = blockIdx.x * kBZ + threadIdx.x + (g.nPml - 2);
+  const int x = blockIdx.y * kBX + threadIdx.y + (g.nPml - 2);
+  if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
+  const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
+  const int slot = ring_slot(z, x, g);
+  if (!in && slot < 0) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i];
+  const float l2m = lam + 2.f * mu;
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  float gl = 0.f, gm = 0.f, gs = 0.f;
+  for (int s = 0; s < g.ns; ++s) {
+    const long long o = s * g.N + i;
+    if (in) {
+      const float* vz = w.vz + o;
+      const float* vx = w.vx + o;
+      const float vz0 = vz[0], vx0 = vx[0];
+      const float dvz_dz = (kC1 * (vz0 - vz[-1]) - kC2 * (vz[1] - vz[-2])) * rdz;
+      const float dvx_dx = (kC1 * (vx0 - vx[-nz]) - kC2 * (vx[nz] - vx[-2 * nz])) * rdx;
+      const float dvx_dz = (kC1 * (vx[1] - vx0) - kC2 * (vx[2] - vx[-1])) * rdz;
+      const float dvz_dx = (kC1 * (vz[nz] - vz0) - kC2 * (vz[2 * nz] - vz[-nz])) * rdx;
+      const float szza = a.szz[o], sxxa = a.sxx[o];
+      gl += (szza + sxxa) * (dvz_dz + dvx_dx);
+      gm += szza * dvz_dz + sxxa * dvx_dx;
+      gs += a.sxz[o] * (dvx_dz + dvz_dx);
+      if (slot < 0) {
+        float szz = w.szz[o], sxx = w.sxx[o];
+        if (z == src.zx[2 * s] && x == src.zx[2 * s + 1]) {
+          const float inj = src.inj[(size_t)s * g.nSteps + it];
+          szz -= inj;
+          sxx -= 3.f * inj;
+        }
+        w.szz[o] = szz - (l2m * dvz_dz + lam * dvx_dx) * dt;
+        w.sxx[o] = sxx - (lam * dvz_dz + l2m * dvx_dx) * dt;
+        w.sxz[o] -= amu * (dvx_dz + dvz_dx) * dt;
+      }
+    }
+    if (slot >= 0) {
+      w.szz[o] = *ring_ptr(const_cast<float*>(ring), g, it, 0, s, slot);
+      w.sxz[o] = *ring_ptr(const_cast<float*>(ring), g, it, 1, s, slot);
+      w.sxx[o] = *ring_ptr(const_cast<float*>(ring), g, it, 2, s, slot);
+    }
+  }
+  if (in) {
+    acc_l[i] += gl;
+    acc_m[i] += gm;
+    acc_s[i] += gs;
+  }
+}
+
+// el_velocity_adj.cu:22-108, plus source_grad (utilities.cu:582-593) for step
+// `it` (the adjoint stress it reads is not touched by this kernel).  The
+// C-PML memory terms a*dpsi are evaluated only where a != 0; elsewhere they
+// are exactly zero in the reference as well.
+__global__ __launch_bounds__(256) void k_adj_vel(Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
+                                                 float* __restrict__ stf_grad, int it) {
+  const int z = blockIdx.x * kBZ + threadIdx.x;
+  const int x = blockIdx.y * kBX + threadIdx.y;
+  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
+  const float l2m = lam + 2.f * mu;
+  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
+  const bool px = x < g.nPml || x > g.nx - g.nPml - 1;
+  const float ax = c.ax[x], rkx = 1.f / c.Kx[x], bx = c.bx[x];
+  const float axh = c.axh[x], rkxh = 1.f / c.Kxh[x], bxh = c.bxh[x];
+  const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
+  const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  for (int s = 0; s < g.ns; ++s) {
+    const long long o = s * g.N + i;
+    const float* szz = a.szz + o;
+    const float* sxx = a.sxx + o;
+    const float* sxz = a.sxz + o;
+    if (it >= 0 && z == src.zx[2 * s] && x == src.zx[2 * s + 1])
+      stf_grad[(size_t)s * g.nSteps + it] = -(szz[0] + 3.f * sxx[0]) * dt;
+    // update vx
+    const float dszz_dx = (-kC1 * (szz[nz] - szz[0]) + kC2 * (szz[2 * nz] - szz[-nz])) * rdx;
+    const float dsxx_dx = (-kC1 * (sxx[nz] - sxx[0]) + kC2 * (sxx[2 * nz] - sxx[-nz])) * rdx;
+    const float dsxz_dz = (-kC1 * (sxz[0] - sxz[-1]) + kC2 * (sxz[1] - sxz[-2])) * rdz;
+    float vx = a.vx[o] + (lam * dszz_dx * rkx * dt + l2m * dsxx_dx * rkx * dt +
+                          amu * rkzh * dsxz_dz * dt);
+    if (ax != 0.f) {
+      const float* q = p.vxx + o;
+      vx += ax * ((-kC1 * (q[nz] - q[0]) + kC2 * (q[2 * nz] - q[-nz])) * rdx);
+    }
+    if (azh != 0.f) {
+      const float* q = p.vxz + o;
+      vx += azh * ((-kC1 * (q[0] - q[-1]) + kC2 * (q[1] - q[-2])) * rdz);
+    }
+    a.vx[o] = vx;
+    if (px) p.sxx[o] = bxh * p.sxx[o] + bb * vx * dt;
+    if (pz) p.sxzz[o] = bz * p.sxzz[o] + bb * vx * dt;
+    // update vz
+    const float dszz_dz = (-kC1 * (szz[1] - szz[0]) + kC2 * (szz[2] - szz[-1])) * rdz;
+    const float dsxx_dz = (-kC1 * (sxx[1] - sxx[0]) + kC2 * (sxx[2] - sxx[-1])) * rdz;
+    const float dsxz_dx = (-kC1 * (sxz[0] - sxz[-nz]) + kC2 * (sxz[nz] - sxz[-2 * nz])) * rdx;
+    float vz = a.vz[o] + (l2m * dszz_dz * rkz * dt + lam * dsxx_dz * rkz * dt +
+                          amu * rkxh * dsxz_dx * dt);
+    if (az != 0.f) {
+      const float* q = p.vzz + o;
+      vz += az * ((-kC1 * (q[1] - q[0]) + kC2 * (q[2] - q[-1])) * rdz);
+    }
+    if (axh != 0.f) {
+      const float* q = p.vzx + o;
+      vz += axh * ((-kC1 * (q[0] - q[-nz]) + kC2 * (q[nz] - q[-2 * nz])) * rdx);
+    }
+    a.vz[o] = vz;
+    if (px) p.sxzx[o] = bx * p.sxzx[o] + ba * vz * dt;
+    if (pz) p.szz[o] = bzh * p.szz[o] + ba * vz * dt;
+  }
+}
+
+// res_injection (utilities.cu:569-580): szz_adj += res, sxx_adj += 3 res.
+__global__ void k_res_inject(Geo g, float* __restrict__ szz, float* __restrict__ sxx,
+                             const int* __restrict__ rec, int R, const float* __restrict__ row) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int z = rec[3 * r], x = rec[3 * r + 1], s = rec[3 * r + 2];
+  const long long o = s * g.N + (long long)x * g.nz + z;
+  const float v = row[r];
+  atomicAdd(szz + o, v);
+  atomicAdd(sxx + o, 3.f * v);
+}
+
+// el_stress_adj.cu:22-104.  The 4 stress-side memory variables are updated
+// without PML gates in the reference; they are read only through a*dpsi terms,
+// i.e. where a != 0 (the C-PML layers) plus the 2-cell reach of the stencil,
+// so they are kept in that band (`zlive`/`xlive`) and nowhere else.
+__global__ __launch_bounds__(256) void k_adj_stress(Geo g, Mdl m, Pml c, Wave a, Mem p) {
+  const int z = blockIdx.x * kBZ + threadIdx.x;
+  const int x = blockIdx.y * kBX + threadIdx.y;
+  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
+  const float l2m = lam + 2.f * mu;
+  const bool zlive = z <= g.nPml + 1 || z >= g.nzA - g.nPml - 2;
+  const bool xlive = x <= g.nPml + 1 || x >= g.nx - g.nPml - 2;
+  const float ax = c.ax[x], rkx = 1.f / c.Kx[x], bx = c.bx[x];
+  const float axh = c.axh[x], rkxh = 1.f / c.Kxh[x], bxh = c.bxh[x];
+  const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
+  const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  for (int s = 0; s < g.ns; ++s) {
+    const long long o = s * g.N + i;
+    const float* vz = a.vz + o;
+    const float* vx = a.vx + o;
+    const float vz0 = vz[0], vx0 = vx[0];
+    const float dvz_dx = (-kC1 * (vz[nz] - vz0) + kC2 * (vz[2 * nz] - vz[-nz])) * rdx;
+    const float dvx_dz = (-kC1 * (vx[1] - vx0) + kC2 * (vx[2] - vx[-1])) * rdz;
+    float sxz = a.sxz[o] + (dvz_dx * rkx * ba * dt + dvx_dz * rkz * bb * dt);
+    if (ax != 0.f) {
+      const float* q = p.sxzx + o;
+      sxz += ax * ((-kC1 * (q[nz] - q[0]) + kC2 * (q[2 * nz] - q[-nz])) * rdx);
+    }
+    if (az != 0.f) {
+      const float* q = p.sxzz + o;
+      sxz += az * ((-kC1 * (q[1] - q[0]) + kC2 * (q[2] - q[-1])) * rdz);
+    }
+    a.sxz[o] = sxz;
+    if (xlive) p.vzx[o] = bxh * p.vzx[o] + sxz * amu * dt;
+    if (zlive) p.vxz[o] = bzh * p.vxz[o] + sxz * amu * dt;
+    const float dvx_dx = (-kC1 * (vx0 - vx[-nz]) + kC2 * (vx[nz] - vx[-2 * nz])) * rdx;
+    const float dvz_dz = (-kC1 * (vz0 - vz[-1]) + kC2 * (vz[1] - vz[-2])) * rdz;
+    float sxx = a.sxx[o] + bb * dvx_dx * rkxh * dt;
+    if (axh != 0.f) {
+      const float* q = p.sxx + o;
+      sxx += axh * ((-kC1 * (q[0] - q[-nz]) + kC2 * (q[nz] - q[-2 * nz])) * rdx);
+    }
+    float szz = a.szz[o] + ba * dvz_dz * rkzh * dt;
+    if (azh != 0.f) {
+      const float* q = p.szz + o;
+      szz += azh * ((-kC1 * (q[0] - q[-1]) + kC2 * (q[1] - q[-2])) * rdz);
+    }
+    a.sxx[o] = sxx;
+    a.szz[o] = szz;
+    if (xlive) p.vxx[o] = bx * p.vxx[o] + lam * szz * dt + l2m * sxx * dt;
+    if (zlive) p.vzz[o] = bz * p.vzz[o] + l2m * szz * dt + lam * sxx * dt;
+  }
+}
+
+// Chain rule from the accumulated derivatives to (lambda, mu, rho), once per
+// run.  Deterministic gather form of the reference's atomicAdd sprays
+// (el_stress.cu:113-124, el_velocity.cu:234-243), with the same target
+// conditions (including the always-true `gidx+1<=gidx<=nx-1-nPml`).
+__global__ __launch_bounds__(256) void k_finalize(Geo g, Mdl m, const float* __restrict__ acc,
+                                                  float* __restrict__ grad) {
+  const int z = blockIdx.x * kBZ + threadIdx.x;
+  const int x = blockIdx.y * kBX + threadIdx.y;
+  if (z >= g.nz || x >= g.nx) return;
+  const int nz = g.nz;
+  const long long N = g.N;
+  const float* acc_l = acc;
+  const float* acc_m = acc + N;
+  const float* acc_s = acc + 2 * N;
+  const float* acc_a = acc + 3 * N;
+  const float* acc_b = acc + 4 * N;
+  const int z0 = g.nPml, z1 = g.nzA - 1 - g.nPml, x0 = g.nPml, x1 = g.nx - 1 - g.nPml;
+  auto inI = [&](int zz, int xx) { return zz >= z0 && zz <= z1 && xx >= x0 && xx <= x1; };
+  const long long i = (long long)x * nz + z;
+  const float dt = g.dt;
+  float gl = 0.f, gm = 0.f, gd = 0.f;
+  if (inI(z, x)) {
+    gl = -acc_l[i] * dt * 1.0e6f;
+    gm = -2.f * acc_m[i] * dt * 1.0e6f;
+  }
+  auto shear = [&](int zz, int xx) -> float {
+    const long long j = (long long)xx * nz + zz;
+    const float am = m.amu[j];
+    if (am == 0.f) return 0.f;
+    const float S = 1.f / m.mu[j] + 1.f / m.mu[j + 1] + 1.f / m.mu[j + nz] + 1.f / m.mu[j + nz + 1];
+    return -acc_s[j] * dt * am / S * 1.0e6f;
+  };
+  float t = 0.f;
+  bool any = false;
+  if (inI(z, x) && m.amu[i] != 0.f) t += shear(z, x), any = true;
+  if (inI(z - 1, x) && z <= z1 && m.amu[i - 1] != 0.f) t += shear(z - 1, x), any = true;
+  if (inI(z, x - 1) && m.amu[i - nz] != 0.f) t += shear(z, x - 1), any = true;
+  if (inI(z - 1, x - 1) && z <= z1 && x <= x1 && m.amu[i - nz - 1] != 0.f)
+    t += shear(z - 1, x - 1), any = true;
+  if (any) {
+    const float mu = m.mu[i];
+    gm += t / (mu * mu);
+  }
+  auto ga = [&](long long j) { const float b = m.ba[j]; return acc_a[j] * dt * (0.5f * b * b); };
+  auto gb = [&](long long j) { const float b = m.bb[j]; return acc_b[j] * dt * (0.5f * b * b); };
+  if (inI(z, x)) gd += ga(i) + gb(i);
+  if (inI(z - 1, x) && z <= z1) gd += ga(i - 1);
+  if (inI(z, x - 1)) gd += gb(i - nz);
+  grad[i] = gl;
+  grad[N + i] = gm;
+  grad[2 * N + i] = gd;
+}
+
+// aveMuInit / aveBycInit (utilities.cu:125-152) over [2, nz-3] x [2, nx-3];
+// defaults 0 and 1/1000 elsewhere (Model.cu:67-73).
+__global__ void k_model_init(Geo g, const float* __restrict__ mu, const float* __restrict__ den,
+                             float* __restrict__ amu, float* __restrict__ ba,
+                             float* __restrict__ bb) {
+  const int z = blockIdx.x * kBZ + threadIdx.x;
+  const int x = blockIdx.y * kBX + threadIdx.y;
+  if (z >= g.nz || x >= g.nx) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  float am = 0.f, fa = 1.f / 1000.f, fb = 1.f / 1000.f;
+  if (z >= 2 && z <= g.nz - 3 && x >= 2 && x <= g.nx - 3) {
+    const float A = mu[i], B = mu[i + 1], C = mu[i + nz], D = mu[i + nz + 1];
+    if (A == 0.f || B == 0.f || C == 0.f || D == 0.f)
+      am = 0.f;
+    else
+      am = (float)(4.0 / (1.0 / A + 1.0 / B + 1.0 / C + 1.0 / D));
+    fa = (float)(2.0 / (double)(den[i + 1] + den[i]));
+    fb = (float)(2.0 / (double)(den[i + nz] + den[i]));
+  }
+  amu[i] = am;
+  ba[i] = fa;
+  bb[i] = fb;
+}
+
+// [rows][cols] -> [cols][rows] through LDS tiles.
+__global__ void k_transpose(const float* __restrict__ in, float* __restrict__ out, int rows,
+                            int cols) {
+  __shared__ float tile[32][33];
+  const int c = blockIdx.x * 32 + threadIdx.x;
+  const int r0 = blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    const int r = r0 + k;
+    if (r < rows && c < cols) tile[k][threadIdx.x] = in[(size_t)r * cols + c];
+  }
+  __syncthreads();
+  const int r = r0 + threadIdx.x;
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    const int cc = blockIdx.x * 32 + k;
+    if (r < rows && cc < cols) out[(size_t)cc * rows + r] = tile[threadIdx.x][k];
+  }
+}
+
+}  // namespace
+
+// ===================================================================== plan
+struct DeviceState {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  Geo g{};
+  int R = 0;
+  int ckpt_every = 0;
+  std::vector<void*> allocs;
+  long long bytes = 0;
+  // model + tables
+  float *lam = nullptr, *mu = nullptr, *amu = nullptr, *ba = nullptr, *bb = nullptr;
+  float* den = nullptr;
+  float* pml = nullptr;   // 6*nzA + 6*nx
+  float* win2 = nullptr;  // nSteps
+  int* src = nullptr;     // [ns][2]
+  float* inj = nullptr;   // [ns][nSteps]
+  int* rec = nullptr;     // [R][3]
+  // wavefields
+  float *fwd = nullptr, *adj = nullptr, *mem = nullptr;
+  float* ring = nullptr;
+  float *syn = nullptr, *obs = nullptr, *res = nullptr, *stage = nullptr;
+  float *acc = nullptr, *grad = nullptr, *stf_grad = nullptr;
+  double* partial = nullptr;
+  double* misfit = nullptr;
+  int npart = 0;
+  bool model_set = false, stf_set = false, obs_set = false;
+  double misfit_host = 0.0;
+  // timing
+  bool timing = false;
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long long>> totals;
+
+  ~DeviceState() {
+    if (stream) (void)hipSetDevice(dev);
+    for (auto& r : pending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+    for (void* p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  template <typename T>
+  Status alloc(T** p, size_t count) {
+    void* q = nullptr;
+    const size_t nb = count * sizeof(T);
+    hipError_t e = hipMalloc(&q, nb ? nb : 4);
+    if (e != hipSuccess)
+      return Status::Err(e == hipErrorOutOfMemory ? FWIFLOW_ERR_MEMORY : FWIFLOW_ERR_HIP,
+                         "hipMalloc(" + std::to_string(nb) + " bytes): " + hipGetErrorString(e));
+    allocs.push_back(q);
+    bytes += (long long)nb;
+    *p = (T*)q;
+    return Status::Ok();
+  }
+
+  hipEvent_t take_event() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+
+  // Launch helper: optionally brackets the launch with events.
+  template <typename F>
+  void launch(const char* name, bool sample, F&& f) {
+    if (timing && sample) {
+      Rec r{name, take_event(), take_event()};
+      (void)hipEventRecord(r.a, stream);
+      f();
+      (void)hipEventRecord(r.b, stream);
+      pending.push_back(r);
+    } else {
+      f();
+    }
+  }
+
+  void collect_timing() {
+    for (auto& r : pending) {
+      float ms = 0.f;
+      (void)hipEventSynchronize(r.b);
+      (void)hipEventElapsedTime(&ms, r.a, r.b);
+      auto& t = totals[r.name];
+      t.first += ms;
+      t.second += 1;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+
+  Pml pml_ptrs() const {
+    const int nzA = g.nzA, nx = g.nx;
+    const float* z = pml;
+    const float* x = pml + 6 * nzA;
+    return Pml{z,     z + nzA,     z + 2 * nzA, z + 3 * nzA, z + 4 * nzA, z + 5 * nzA,
+               x,     x + nx,      x + 2 * nx,  x + 3 * nx,  x + 4 * nx,  x + 5 * nx};
+  }
+  Mdl mdl() const { return Mdl{lam, mu, amu, ba, bb}; }
+  Wave wave(float* base) const {
+    const size_t F = (size_t)g.ns * g.N;
+    return Wave{base, base + F, base + 2 * F, base + 3 * F, base + 4 * F};
+  }
+  Mem memv() const {
+    const size_t F = (size_t)g.ns * g.N;
+    float* b = mem;
+    return Mem{b, b + F, b + 2 * F, b + 3 * F, b + 4 * F, b + 5 * F, b + 6 * F, b + 7 * F};
+  }
+};
+
+Plan::Plan() = default;
+Plan::~Plan() = default;
+
+Status Plan::create(const std::string& para_fname, int gpu, int group_size, const int* shot_ids) {
+  TRY(parse_para(para_fname, para));
+  TRY(parse_survey(para, group_size, shot_ids, survey));
+  gpu_id = gpu;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (gpu < 0 || gpu >= ndev)
+    return Status::Err(FWIFLOW_ERR_HIP, "gpu_id " + std::to_string(gpu) + " not present (" +
+                                            std::to_string(ndev) + " devices)");
+  d_.reset(new DeviceState());
+  DeviceState& d = *d_;
+  d.dev = gpu;
+  HIP_TRY(hipSetDevice(gpu));
+  HIP_TRY(hipStreamCreate(&d.stream));
+  Geo& g = d.g;
+  g.nz = para.nz;
+  g.nx = para.nx;
+  g.nzA = para.nzA();
+  g.nPml = para.nPml;
+  g.ns = group_size;
+  g.nSteps = para.nSteps;
+  g.N = (long long)para.nz * para.nx;
+  g.dt = para.dt;
+  g.rdz = 1.f / para.dz;
+  g.rdx = 1.f / para.dx;
+  g.ringH = g.nzA - 2 * g.nPml + 4;
+  g.ringW = g.nx - 2 * g.nPml + 4;
+  g.ringlen = 10 * g.ringW + 10 * (g.ringH - 10);
+  d.R = survey.total_rec();
+
+  const size_t N = (size_t)g.N;
+  TRY(d.alloc(&d.lam, N));
+  TRY(d.alloc(&d.mu, N));
+  TRY(d.alloc(&d.den, N));
+  TRY(d.alloc(&d.amu, N));
+  TRY(d.alloc(&d.ba, N));
+  TRY(d.alloc(&d.bb, N));
+  // host tables
+  std::vector<float> tab(6 * (size_t)g.nzA + 6 * (size_t)g.nx);
+  cpml_profiles(g.nzA, g.nPml, para.dz, para.f0, para.dt, tab.data());
+  cpml_profiles(g.nx, g.nPml, para.dx, para.f0, para.dt, tab.data() + 6 * g.nzA);
+  TRY(d.alloc(&d.pml, tab.size()));
+  HIP_TRY(hipMemcpy(d.pml, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<float> w2(g.nSteps);
+  window_squared(g.nSteps, para.dt, 0.005f, w2.data());
+  TRY(d.alloc(&d.win2, w2.size()));
+  HIP_TRY(hipMemcpy(d.win2, w2.data(), w2.size() * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<int> src(2 * g.ns), rec(3 * (size_t)d.R);
+  size_t r = 0;
+  for (int s = 0; s < g.ns; s++) {
+    const Shot& sh = survey.shots[s];
+    src[2 * s] = sh.z_src;
+    src[2 * s + 1] = sh.x_src;
+    for (size_t k = 0; k < sh.z_rec.size(); k++, r++) {
+      rec[3 * r] = sh.z_rec[k];
+      rec[3 * r + 1] = sh.x_rec[k];
+      rec[3 * r + 2] = s;
+    }
+  }
+  TRY(d.alloc(&d.src, src.size()));
+  HIP_TRY(hipMemcpy(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
+  TRY(d.alloc(&d.rec, rec.size()));
+  HIP_TRY(hipMemcpy(d.rec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
+  TRY(d.alloc(&d.inj, (size_t)g.ns * g.nSteps));
+  const size_t F = (size_t)g.ns * N;
+  TRY(d.alloc(&d.fwd, 5 * F));
+  TRY(d.alloc(&d.mem, 8 * F));
+  const size_t T = (size_t)g.nSteps * d.R;
+  TRY(d.alloc(&d.syn, T));
+  d.npart = 1024;
+  TRY(d.alloc(&d.partial, (size_t)d.npart));
+  TRY(d.alloc(&d.misfit, 1));
+  return Status::Ok();
+}
+
+Status Plan::set_model(const double* Lambda, const double* Mu, const double* Den, bool zfast) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (!Lambda || !Mu || !Den) return Status::Err(FWIFLOW_ERR_ARG, "null model pointer");
+  std::vector<float> lam, mu, den;
+  model_to_zfast(para, Lambda, Mu, Den, zfast, lam, mu, den, cp_max);
+  TRY(check_courant(para, cp_max));
+  DeviceState& d = *d_;
+  HIP_TRY(hipSetDevice(d.dev));
+  const size_t nb = (size_t)d.g.N * sizeof(float);
+  HIP_TRY(hipMemcpyAsync(d.lam, lam.data(), nb, hipMemcpyHostToDevice, d.stream));
+  HIP_TRY(hipMemcpyAsync(d.mu, mu.data(), nb, hipMemcpyHostToDevice, d.stream));
+  HIP_TRY(hipMemcpyAsync(d.den, den.data(), nb, hipMemcpyHostToDevice, d.stream));
+  dim3 blk(kBZ, kBX), grd((d.g.nz + kBZ - 1) / kBZ, (d.g.nx + kBX - 1) / kBX);
+  hipLaunchKernelGGL(k_model_init, grd, blk, 0, d.stream, d.g, d.mu, d.den, d.amu, d.ba, d.bb);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  d.model_set = true;
+  return Status::Ok();
+}
+
+Status Plan::set_stf(const double* stf) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (!stf) return Status::Err(FWIFLOW_ERR_ARG, "null stf pointer");
+  DeviceState& d = *d_;
+  const int nSteps = d.g.nSteps;
+  // Src_Rec.cu:132-143: row shot_id -> float -> window ratio 0.001; the
+  // increment added to szz is 1500^2 * stf * dt (add_source, utilities.cu:528-538)
+  std::vector<float> w2(nSteps), inj((size_t)d.g.ns * nSteps);
+  window_squared(nSteps, para.dt, 0.001f, w2.data());
+  for (int s = 0; s < d.g.ns; s++) {
+    const double* row = stf + (size_t)survey.shots[s].id * nSteps;
+    for (int it = 0; it < nSteps; it++) {
+      const float amp = (float)row[it] * w2[it];
+      inj[(size_t)s * nSteps + it] = (float)(2250000.0 * (double)amp * (double)para.dt);
+    }
+  }
+  HIP_TRY(hipSetDevice(d.dev));
+  HIP_TRY(hipMemcpy(d.inj, inj.data(), inj.size() * sizeof(float), hipMemcpyHostToDevice));
+  d.stf_set = true;
+  return Status::Ok();
+}
+
+static Status ensure_obs_buffers(DeviceState& d) {
+  const size_t T = (size_t)d.g.nSteps * d.R;
+  if (!d.obs) TRY(d.alloc(&d.obs, T));
+  if (!d.res) TRY(d.alloc(&d.res, T));
+  if (!d.stage) TRY(d.alloc(&d.stage, T));
+  return Status::Ok();
+}
+
+Status Plan::set_obs(const float* obs) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (!obs) return Status::Err(FWIFLOW_ERR_ARG, "null obs pointer");
+  DeviceState& d = *d_;
+  HIP_TRY(hipSetDevice(d.dev));
+  TRY(ensure_obs_buffers(d));
+  const size_t T = (size_t)d.g.nSteps * d.R;
+  // host [R][nSteps] (concatenated Shot files) -> device time-major [nSteps][R]
+  HIP_TRY(hipMemcpyAsync(d.stage, obs, T * sizeof(float), hipMemcpyHostToDevice, d.stream));
+  dim3 blk(32, 8), grd((d.g.nSteps + 31) / 32, (d.R + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, grd, blk, 0, d.stream, d.stage, d.obs, d.R, d.g.nSteps);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  d.obs_set = true;
+  return Status::Ok();
+}
+
+Status Plan::load_obs() {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  const int nSteps = para.nSteps;
+  std::vector<float> host((size_t)nSteps * d_->R);
+  size_t off = 0;
+  for (const Shot& sh : survey.shots) {
+    const std::string fn = para.data_dir + "/Shot" + std::to_string(sh.id) + ".bin";
+    FILE* fp = std::fopen(fn.c_str(), "rb");
+    if (!fp) return Status::Err(FWIFLOW_ERR_IO, "cannot open \"" + fn + "\"");
+    const size_t n = sh.z_rec.size() * (size_t)nSteps;
+    const size_t got = std::fread(host.data() + off, sizeof(float), n, fp);
+    std::fclose(fp);
+    if (got != n)
+      return Status::Err(FWIFLOW_ERR_IO, "\"" + fn + "\" holds " + std::to_string(got) +
+                                             " floats, expected " + std::to_string(n));
+    off += n;
+  }
+  return set_obs(host.data());
+}
+
+Status Plan::set_checkpoint(int k) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (k < 0) return Status::Err(FWIFLOW_ERR_ARG, "checkpoint interval must be >= 0");
+  if (k > 0)
+    return Status::Err(FWIFLOW_ERR_UNSUPPORTED, "interior checkpoint resets are not built yet");
+  d_->ckpt_every = k;
+  return Status::Ok();
+}
+
+Status Plan::set_timing(bool on) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  d_->timing = on;
+  d_->totals.clear();
+  return Status::Ok();
+}
+
+Status Plan::timing(const std::string& name, double* ms, long long* count) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  auto it = d_->totals.find(name);
+  if (ms) *ms = it == d_->totals.end() ? 0.0 : it->second.first;
+  if (count) *count = it == d_->totals.end() ? 0 : it->second.second;
+  return Status::Ok();
+}
+
+long long Plan::device_bytes() const { return d_ ? d_->bytes : 0; }
+
+Status Plan::run(int calc_id) {
+  TRY(check_calc_id(calc_id));
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  DeviceState& d = *d_;
+  if (!d.model_set) return Status::Err(FWIFLOW_ERR_ARG, "model not set");
+  if (!d.stf_set) return Status::Err(FWIFLOW_ERR_ARG, "source time functions not set");
+  const bool if_res = calc_id != 2, withAdj = calc_id == 1;
+  if (if_res && !d.obs_set) return Status::Err(FWIFLOW_ERR_ARG, "observed data not set");
+  HIP_TRY(hipSetDevice(d.dev));
+  const Geo& g = d.g;
+  const size_t N = (size_t)g.N, F = (size_t)g.ns * N;
+  const size_t T = (size_t)g.nSteps * d.R;
+  if (withAdj) {
+    if (!d.adj) TRY(d.alloc(&d.adj, 5 * F));
+    if (!d.ring) TRY(d.alloc(&d.ring, (size_t)g.nSteps * 5 * g.ns * (size_t)g.ringlen));
+    if (!d.acc) TRY(d.alloc(&d.acc, 5 * N));
+    if (!d.grad) TRY(d.alloc(&d.grad, 3 * N));
+    if (!d.stf_grad) TRY(d.alloc(&d.stf_grad, (size_t)g.ns * g.nSteps));
+  }
+  hipStream_t st = d.stream;
+  HIP_TRY(hipMemsetAsync(d.fwd, 0, 5 * F * sizeof(float), st));
+  HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
+  HIP_TRY(hipMemsetAsync(d.syn, 0, T * sizeof(float), st));
+
+  const Mdl m = d.mdl();
+  const Pml c = d.pml_ptrs();
+  const Wave w = d.wave(d.fwd);
+  const Mem p = d.memv();
+  const Src src{d.src, d.inj};
+  const dim3 blk(kBZ, kBX);
+  const dim3 grdU((g.nzA + kBZ - 1) / kBZ, (g.nx + kBX - 1) / kBX);
+  const int recBlocks = (d.R + 255) / 256;
+  const int sample_every = 8;
+
+  // ---------------------------------------------------------- forward
+  for (int it = 0; it <= g.nSteps - 2; it++) {
+    const bool smp = (it % sample_every) == 0;
+    float* ring = withAdj ? d.ring : nullptr;
+    d.launch("stress_fwd", smp, [&] {
+      hipLaunchKernelGGL(k_stress_fwd, grdU, blk, 0, st, g, m, c, w, p, src, it, ring);
+    });
+    d.launch("vel_fwd", smp, [&] { hipLaunchKernelGGL(k_vel_fwd, grdU, blk, 0, st, g, m, c, w, p); });
+    hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, w.szz, w.sxx, d.rec, d.R,
+                       d.syn + (size_t)(it + 1) * d.R);
+  }
+  HIP_TRY(hipGetLastError());
+
+  if (if_res) {
+    hipLaunchKernelGGL(k_residual, dim3(d.npart), dim3(256), 0, st, g.nSteps, d.R, d.win2, d.obs,
+                       d.syn, d.res, d.partial);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d.partial, d.npart, d.misfit);
+    HIP_TRY(hipGetLastError());
+  }
+
+  if (withAdj) {
+    // ------------------------------------------------------- backward
+    const Wave a = d.wave(d.adj);
+    HIP_TRY(hipMemsetAsync(d.adj, 0, 5 * F * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(d.acc, 0, 5 * N * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(d.stf_grad, 0, (size_t)g.ns * g.nSteps * sizeof(float), st));
+    const dim3 grdR((g.ringH + kBZ - 1) / kBZ, (g.ringW + kBX - 1) / kBX);
+    float* acc = d.acc;
+    // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
+    hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, g, m, c, a, p, src, d.stf_grad, -1);
+    hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx, d.rec,
+                       d.R, d.res + (size_t)(g.nSteps - 1) * d.R);
+    hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, g, m, c, a, p);
+    for (int it = g.nSteps - 2; it >= 0; it--) {
+      const bool smp = (it % sample_every) == 0;
+      d.launch("rev_vel", smp, [&] {
+        hipLaunchKernelGGL(k_rev_vel, grdR, blk, 0, st, g, m, w, a, acc + 3 * N, acc + 4 * N,
+                           d.ring, it);
+      });
+      d.launch("rev_stress", smp, [&] {
+        hipLaunchKernelGGL(k_rev_stress, grdR, blk, 0, st, g, m, w, a, src, acc, acc + N,
+                           acc + 2 * N, d.ring, it);
+      });
+      d.launch("adj_vel", smp, [&] {
+        hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, g, m, c, a, p, src, d.stf_grad, it);
+      });
+      hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx, d.rec,
+                         d.R, d.res + (size_t)it * d.R);
+      d.launch("adj_stress", smp, [&] {
+        hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, g, m, c, a, p);
+      });
+    }
+    const dim3 grdA((g.nz + kBZ - 1) / kBZ, (g.nx + kBX - 1) / kBX);
+    hipLaunchKernelGGL(k_finalize, grdA, blk, 0, st, g, m, d.acc, d.grad);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  if (d.timing) d.collect_timing();
+  if (if_res) HIP_TRY(hipMemcpy(&d.misfit_host, d.misfit, sizeof(double), hipMemcpyDeviceToHost));
+  last_calc = calc_id;
+  return Status::Ok();
+}
+
+Status Plan::misfit(double* out) {
+  if (!d_ || last_calc != 0) return Status::Err(FWIFLOW_ERR_ARG, "no calc_id 0 result");
+  if (out) *out = 0.5 * d_->misfit_host;
+  return Status::Ok();
+}
+
+Status Plan::gradients(double* gL, double* gM, double* gD, double* gS) {
+  if (!d_ || last_calc != 1) return Status::Err(FWIFLOW_ERR_ARG, "no calc_id 1 result");
+  DeviceState& d = *d_;
+  HIP_TRY(hipSetDevice(d.dev));
+  const int nz = d.g.nz, nx = d.g.nx;
+  const size_t N = (size_t)d.g.N;
+  std::vector<float> h(3 * N);
+  HIP_TRY(hipMemcpy(h.data(), d.grad, 3 * N * sizeof(float), hipMemcpyDeviceToHost));
+  double* outs[3] = {gL, gM, gD};
+  for (int k = 0; k < 3; k++) {
+    if (!outs[k]) continue;
+    const float* src = h.data() + k * N;
+    // z-fastest -> row-major [NZ][NX] (libCUFD.cu:480-486)
+    for (int i = 0; i < nz; i++)
+      for (int j = 0; j < nx; j++) outs[k][(size_t)i * nx + j] = src[(size_t)j * nz + i];
+  }
+  if (gS) {
+    std::vector<float> s((size_t)d.g.ns * d.g.nSteps);
+    HIP_TRY(hipMemcpy(s.data(), d.stf_grad, s.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < s.size(); k++) gS[k] = s[k];
+  }
+  return Status::Ok();
+}
+
+Status Plan::data(float* syn) {
+  if (!d_ || last_calc < 0) return Status::Err(FWIFLOW_ERR_ARG, "no run result");
+  if (!syn) return Status::Err(FWIFLOW_ERR_ARG, "null output pointer");
+  DeviceState& d = *d_;
+  HIP_TRY(hipSetDevice(d.dev));
+  TRY(ensure_obs_buffers(d));
+  const size_t T = (size_t)d.g.nSteps * d.R;
+  // time-major [nSteps][R] -> [R][nSteps] (Shot file layout, utilities.cu:557-567)
+  dim3 blk(32, 8), grd((d.R + 31) / 32, (d.g.nSteps + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, grd, blk, 0, d.stream, d.syn, d.stage, d.g.nSteps, d.R);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(syn, d.stage, T * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  return Status::Ok();
+}
+
+Status Plan::write_data() {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  std::vector<float> h((size_t)para.nSteps * d_->R);
+  TRY(data(h.data()));
+  size_t off = 0;
+  for (const Shot& sh : survey.shots) {
+    const std::string fn = para.data_dir + "/Shot" + std::to_string(sh.id) + ".bin";
+    FILE* fp = std::fopen(fn.c_str(), "wb");
+    if (!fp) return Status::Err(FWIFLOW_ERR_IO, "cannot write \"" + fn + "\"");
+    const size_t n = sh.z_rec.size() * (size_t)para.nSteps;
+    const size_t put = std::fwrite(h.data() + off, sizeof(float), n, fp);
+    std::fclose(fp);
+    if (put != n) return Status::Err(FWIFLOW_ERR_IO, "short write to \"" + fn + "\"");
+    off += n;
+  }
+  return Status::Ok();
+}
+
+Status Plan::write_scratch() { return Status::Ok(); }
+
+}  // namespace fwiflow

@@ -1,0 +1,140 @@
+/*
+ * fwiflow.h -- C ABI of the MI355X (gfx950) elastic FWI operator.
+ *
+ * Drop-in for FwiFlow.jl's `cufd` driver and the three TensorFlow ops built
+ * on it (`FwiOp`, `FwiOpGrad`, `FwiObsOp`).  Plain pointers and sizes only;
+ * every entry point returns an `int` status (FWIFLOW_OK = 0) and never exits
+ * the process.  The text of the last error of the calling thread is returned
+ * by fwiflow_last_error().
+ *
+ * Conventions shared with the reference (deps/CustomOps/FWI/FwiOp.cpp and
+ * deps/CustomOps/FWI/Src/libCUFD.cu):
+ *   - Lambda and Mu in MPa, Den in kg/m^3, each a row-major [NZ][NX] array of
+ *     double (NZ, NX = "nz", "nx" of the parameter file: the padded grid);
+ *   - stf is row-major [nShotsTotal][nSteps]; row = shot id (Src_Rec.cu:135);
+ *   - shot_ids are 0-based int32, group_size of them;
+ *   - grad_Lambda/grad_Mu/grad_Den are [NZ][NX] (dJ/dLambda and dJ/dMu per MPa);
+ *   - grad_stf rows 0..group_size-1 are written by group position
+ *     (libCUFD.cu:454-456); other rows are not touched;
+ *   - the upstream gradient dy of the misfit is not applied (FwiOp.cpp:253-255
+ *     never passes it): callers that need dy scale the outputs themselves.
+ */
+#ifndef FWIFLOW_H_
+#define FWIFLOW_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Each replaces an exit(1)/printf path of the reference. */
+enum fwiflow_status {
+  FWIFLOW_OK = 0,
+  FWIFLOW_ERR_CALC_ID = 1,     /* calc_id not 0/1/2 (libCUFD.cu:45-48, Parameter.cpp:141-143) */
+  FWIFLOW_ERR_IO = 2,          /* file missing/unreadable (Parameter.cpp:22-25, Src_Rec.cu:26-29,
+                                  utilities.cu:12-16) */
+  FWIFLOW_ERR_JSON = 3,        /* malformed JSON or missing/ill-typed key (Parameter.cpp:41-113) */
+  FWIFLOW_ERR_COURANT = 4,     /* Courant number > 1 (utilities.cu:225-240) */
+  FWIFLOW_ERR_SHAPE = 5,       /* shot id / geometry / source or receiver outside the grid */
+  FWIFLOW_ERR_UNSUPPORTED = 6, /* "filter", "if_src_update" or "if_win" requested */
+  FWIFLOW_ERR_HIP = 7,         /* HIP runtime error, including "no GPU" */
+  FWIFLOW_ERR_ARG = 8,         /* null pointer or invalid argument */
+  FWIFLOW_ERR_MEMORY = 9       /* device or pinned-host allocation failed */
+};
+
+/*
+ * Replaces `void cufd(double* misfit, double* grad_Lambda, double* grad_Mu,
+ * double* grad_Den, double* grad_stf, const double* Lambda, const double* Mu,
+ * const double* Den, const double* stf, int calc_id, const int gpu_id,
+ * int group_size, const int* shot_ids, const std::string para_fname)`
+ * (deps/CustomOps/FWI/Src/libCUFD.cu:34-38, declared in FwiOp.h:9-12; bound
+ * by ccall in deps/CustomOps/FWI/Obcalc.jl:16-20).
+ *   calc_id 0: misfit 0.5*sum(res^2) over the group (libCUFD.cu:528-536);
+ *              reads {data_dir}/Shot{id}.bin as observed data.
+ *   calc_id 1: gradients (misfit not written); reads the Shot files.
+ *   calc_id 2: observed data: writes {data_dir}/Shot{id}.bin, f32 [nrec][nSteps].
+ * Output pointers that the calc_id does not use may be NULL.
+ */
+int fwiflow_cufd(double* misfit, double* grad_Lambda, double* grad_Mu, double* grad_Den,
+                 double* grad_stf, const double* Lambda, const double* Mu, const double* Den,
+                 const double* stf, int calc_id, int gpu_id, int group_size,
+                 const int* shot_ids, const char* para_fname);
+
+/* Host-only validation of a cufd call: parses the JSON files, checks calc_id,
+ * shot ids, source/receiver positions and the Courant number exactly as
+ * fwiflow_cufd does before touching a GPU.  Needs no GPU.  No counterpart in
+ * the reference (its checks exit the process, see the status codes above). */
+int fwiflow_check(int calc_id, int group_size, const int* shot_ids, const char* para_fname,
+                  const double* Lambda, const double* Mu, const double* Den);
+
+/* Host tables the GPU path uses, for inspection (no GPU needed):
+ * cpml_z: 6 x (NZ-nPad) floats K,a,b,K_half,a_half,b_half (Cpml.cu:7-117),
+ * cpml_x: 6 x NX floats, win_data: nSteps floats of w(t)^2 for data
+ * (ratio 0.005, libCUFD.cu:63), win_src: nSteps floats of w(t)^2 for the source
+ * (ratio 0.001, Src_Rec.cu:140).  Any pointer may be NULL. */
+int fwiflow_host_tables(const char* para_fname, float* cpml_z, float* cpml_x, float* win_data,
+                        float* win_src);
+
+/* Last error text of the calling thread ("" if none). */
+const char* fwiflow_last_error(void);
+
+/* Library version string. */
+const char* fwiflow_version(void);
+
+/* ------------------------------------------------------------------------
+ * Persistent plan: parse the JSON files once, keep the model, observed data,
+ * wavefields and boundary store resident in HBM, run all shots of the group
+ * batched (one launch per kernel per time step covers every shot).  No
+ * counterpart in the reference, which rebuilds everything on each call
+ * (libCUFD.cu:51-145); fwiflow_cufd is implemented on top of it.
+ * ---------------------------------------------------------------------- */
+typedef struct fwiflow_plan fwiflow_plan;
+
+int fwiflow_plan_create(fwiflow_plan** plan, const char* para_fname, int gpu_id,
+                        int group_size, const int* shot_ids);
+int fwiflow_plan_destroy(fwiflow_plan* plan);
+/* Model in the reference layout (row-major [NZ][NX] double; MPa, MPa, kg/m^3). */
+int fwiflow_plan_set_model(fwiflow_plan* plan, const double* Lambda, const double* Mu,
+                           const double* Den);
+/* Model in the Julia-native layout: column-major (NZ,NX) == z fastest, no
+ * transpose; same units. */
+int fwiflow_plan_set_model_zfast(fwiflow_plan* plan, const double* Lambda, const double* Mu,
+                                 const double* Den);
+/* Source time functions [nShotsTotal][nSteps]; row shot_ids[i] feeds shot i. */
+int fwiflow_plan_set_stf(fwiflow_plan* plan, const double* stf);
+/* Observed data from host memory, concatenated over the group in group order,
+ * each shot f32 [nrec][nSteps] (the Shot{id}.bin layout). */
+int fwiflow_plan_set_obs(fwiflow_plan* plan, const float* obs);
+/* Observed data from {data_dir}/Shot{id}.bin. */
+int fwiflow_plan_load_obs(fwiflow_plan* plan);
+/* Run calc_id 0, 1 or 2 on the device; results stay resident. */
+int fwiflow_plan_run(fwiflow_plan* plan, int calc_id);
+int fwiflow_plan_misfit(fwiflow_plan* plan, double* misfit);
+/* Gradients of the last calc_id 1 run; layouts as in fwiflow_cufd
+ * (grad_stf: group_size rows).  Any pointer may be NULL. */
+int fwiflow_plan_gradients(fwiflow_plan* plan, double* grad_Lambda, double* grad_Mu,
+                           double* grad_Den, double* grad_stf);
+/* Synthetic data of the last run, concatenated [nrec][nSteps] per shot. */
+int fwiflow_plan_data(fwiflow_plan* plan, float* syn);
+/* Write the synthetic data to {data_dir}/Shot{id}.bin (calc_id 2's side effect). */
+int fwiflow_plan_write_data(fwiflow_plan* plan);
+/* Geometry of the plan: dims[0..5] = NZ, NX, nSteps, group_size, total receivers,
+ * nPml. */
+int fwiflow_plan_dims(fwiflow_plan* plan, int* dims);
+/* Backward scheme: ckpt_every = 0 reproduces the reference (boundary ring
+ * every step + time-reversed re-materialisation); k > 0 additionally stores
+ * the interior wavefield every k steps and resets the reverse sweep to it
+ * there, which bounds the FP32 drift of the reversal. */
+int fwiflow_plan_set_checkpoint(fwiflow_plan* plan, int ckpt_every);
+/* Kernel timing with HIP events on the plan's stream (off by default).
+ * fwiflow_plan_timing(name) returns the summed milliseconds and launch count
+ * of kernel `name` since timing was enabled. */
+int fwiflow_plan_set_timing(fwiflow_plan* plan, int enable);
+int fwiflow_plan_timing(fwiflow_plan* plan, const char* name, double* total_ms, long long* count);
+/* Device bytes held by the plan. */
+int fwiflow_plan_device_bytes(fwiflow_plan* plan, long long* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FWIFLOW_H_ */

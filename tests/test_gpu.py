@@ -1,0 +1,223 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerances (stated by the operator's contract): seismograms within 1e-4
+relative L2 of the reference-arithmetic FP32 oracle, gradients within 1e-3;
+each also stated against the FP64 oracle run of the same scheme.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import fwiflow
+from fwiflow import models
+from fwiflow.models import Case
+from fwiflow.utils import npad, padding, velocity_to_moduli
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-4
+GRAD_TOL = 1e-3
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _obs_files(para_dir, ids, nrec, nt):
+    return [np.fromfile(os.path.join(para_dir, "Data", f"Shot{i}.bin"),
+                        dtype=np.float32).reshape(nrec, nt) for i in ids]
+
+
+def _parity(oracle_mod, case, ws, ids=None, check_fp64=True):
+    """obs from the true model on the GPU, then misfit and gradient at the
+    starting model on the GPU and the oracle; returns the relative errors."""
+    para, stf = models.write_case(case, ws)
+    ids = list(range(len(case.src_x))) if ids is None else ids
+    lam, mu, den = models.moduli(case, "true")
+    say("case", case.name, "grid", case.NZ, case.NX, "nt", case.nSteps, "shots", ids)
+    fwiflow.fwi_obs_op(lam, mu, den, stf, 0, ids, para)
+    nrec = len(case.rec_x)
+    gpu_obs = _obs_files(ws, ids, nrec, case.nSteps)
+    o32 = oracle_mod.cufd(lam, mu, den, stf, 2, ids, para, precision=32)[5]
+    out = {"seis32": max(rel(g, o) for g, o in zip(gpu_obs, o32["syn"]))}
+    if check_fp64:
+        o64 = oracle_mod.cufd(lam, mu, den, stf, 2, ids, para, precision=64)[5]
+        out["seis64"] = max(rel(g, o) for g, o in zip(gpu_obs, o64["syn"]))
+    # oracle's obs == GPU obs files (written above by the GPU): both sides use them
+    lam0, mu0, den0 = models.moduli(case, "start")
+    J = fwiflow.fwi_op(lam0, mu0, den0, stf, 0, ids, para)
+    gL, gM, gD, gS = fwiflow.fwi_op_grad(lam0, mu0, den0, stf, 0, ids, para)
+    Jo = oracle_mod.cufd(lam0, mu0, den0, stf, 0, ids, para, precision=32)[0]
+    r = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=32)
+    out["misfit32"] = abs(J - Jo) / abs(Jo)
+    out["lam32"], out["mu32"], out["den32"] = rel(gL, r[1]), rel(gM, r[2]), rel(gD, r[3])
+    out["stf32"] = rel(gS[:len(ids)], r[4][:len(ids)])
+    if check_fp64:
+        r64 = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=64)
+        out["lam64"], out["mu64"], out["den64"] = rel(gL, r64[1]), rel(gM, r64[2]), rel(gD, r64[3])
+    say(case.name, {k: f"{v:.2e}" for k, v in out.items()})
+    return out
+
+
+def _assert(out):
+    for k, v in out.items():
+        tol = SEIS_TOL if k.startswith(("seis", "misfit")) else GRAD_TOL
+        assert v <= tol, (k, v)
+
+
+def test_small_case_parity(oracle_mod, workspace):
+    _assert(_parity(oracle_mod, models.case_small(nSteps=300), workspace))
+
+
+def test_config1_parity(oracle_mod, workspace):
+    """100x200 + 32-cell C-PML, nt 1000, dt 2 ms, 1 shot, 200 receivers, 10 Hz."""
+    _assert(_parity(oracle_mod, models.case1(), workspace))
+
+
+def test_docs_marmousi_parity(oracle_mod, workspace):
+    """The reference's tutorial setup (docs/src/tutorials/fwi.md:12-43): the
+    Marmousi Vp of docs/data (interior of the 224x448 file, re-padded),
+    acoustic (cs = 0), rho = 2500, FWIExample acquisition written as the
+    survey file stores it (ind_src_z = 2, ind_src_x = 4:8:384, receivers
+    3:381; FWI.jl:85-95), its 4.5 Hz source file; 3 of its 48 shots and the
+    first 800 of 2000 steps."""
+    nz, nx, nPml = 134, 384, 32
+    NZ, NX = 224, 448
+    cp = np.fromfile(os.path.join(GOLDEN, "Model_Cp_true.bin"), dtype=np.float32)
+    cp = cp.astype(np.float64).reshape(NX, NZ).T  # z fastest on disk
+    cp0 = np.fromfile(os.path.join(GOLDEN, "Model_Cp_init_1D.bin"), dtype=np.float32)
+    cp0 = cp0.astype(np.float64).reshape(NX, NZ).T
+    stf = np.fromfile(os.path.join(GOLDEN, "sourceF_4p5_2_high.bin"), dtype=np.float32)
+    nt = 800
+    inner = (slice(nPml, nPml + nz), slice(nPml, nPml + nx))
+    src_x = np.arange(4, 385, 8)
+    case = Case("marmousi", nz, nx, 24.0, 24.0, nt, 0.0025, 4.5, nPml, cp[inner],
+                np.zeros((nz, nx)), np.full((nz, nx), 2500.0), cp0[inner], np.zeros((nz, nx)),
+                np.full((nz, nx), 2500.0), 2 * np.ones(len(src_x), int), src_x,
+                2 * np.ones(379, int), np.arange(3, 382),
+                stf[:nt].astype(np.float64).reshape(1, nt))
+    assert case.NZ == NZ and case.NX == NX
+    _assert(_parity(oracle_mod, case, workspace, ids=[0, 9, 40], check_fp64=False))
+
+
+def test_batched_shots_equal_separate_runs(workspace):
+    c = models.case_small(nSteps=250, nshots=3)
+    para, stf = models.write_case(c, workspace)
+    lam, mu, den = models.moduli(c, "true")
+    fwiflow.fwi_obs_op(lam * 1.01, mu, den, stf, 0, [0, 1, 2], para)
+    with fwiflow.Plan(para, [0, 1, 2]) as P:
+        P.set_model(lam, mu, den)
+        P.set_stf(stf)
+        P.load_obs()
+        P.run(2)
+        batched = P.data()
+        P.run(1)
+        gb = P.gradients()
+    sep, gs = [], None
+    for i in range(3):
+        with fwiflow.Plan(para, [i]) as P:
+            P.set_model(lam, mu, den)
+            P.set_stf(stf)
+            P.load_obs()
+            P.run(2)
+            sep.append(P.data())
+            P.run(1)
+            g = P.gradients()
+            gs = list(g[:3]) if gs is None else [a + b for a, b in zip(gs, g[:3])]
+    assert np.array_equal(batched, np.concatenate(sep))  # shots are independent
+    for a, b in zip(gb[:3], gs):
+        assert rel(a, b) < 1e-5
+
+
+def test_plan_rerun_is_deterministic(workspace):
+    c = models.case_small(nSteps=200)
+    para, stf = models.write_case(c, workspace)
+    lam, mu, den = models.moduli(c, "true")
+    fwiflow.fwi_obs_op(lam, mu, den, stf, 0, [0, 1], para)
+    lam0, mu0, den0 = models.moduli(c, "start")
+    with fwiflow.Plan(para, [0, 1]) as P:
+        P.set_model(lam0, mu0, den0)
+        P.set_stf(stf)
+        P.load_obs()
+        P.run(1)
+        a = P.gradients()
+        P.run(1)
+        b = P.gradients()
+        P.run(0)
+        J = P.misfit()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert J == fwiflow.fwi_op(lam0, mu0, den0, stf, 0, [0, 1], para)
+
+
+def test_misfit_zero_at_truth_and_zfast_layout(workspace):
+    c = models.case_small(nSteps=200)
+    para, stf = models.write_case(c, workspace)
+    lam, mu, den = models.moduli(c, "true")
+    fwiflow.fwi_obs_op(lam, mu, den, stf, 0, [0, 1], para)
+    assert fwiflow.fwi_op(lam, mu, den, stf, 0, [0, 1], para) == 0.0
+    with fwiflow.Plan(para, [0, 1]) as P:  # Julia-native column-major input
+        P.set_model(lam.ravel(order="F"), mu.ravel(order="F"), den.ravel(order="F"),
+                    zfast=True)
+        P.set_stf(stf)
+        P.load_obs()
+        P.run(0)
+        assert P.misfit() == 0.0
+
+
+def test_full_size_grid_properties(oracle_mod, workspace):
+    """The 500x1700 (+40 C-PML) grid at full size, 2 shots, first 300 steps:
+    seismograms against the oracle, linearity in the source, misfit 0 at truth."""
+    c = models.case3(nSteps=300, nshots=64)
+    para, stf = models.write_case(c, workspace)
+    lam, mu, den = models.moduli(c, "true")
+    ids = [0, 40]
+    say("case c3-grid", c.NZ, c.NX, "nt", c.nSteps)
+    with fwiflow.Plan(para, ids) as P:
+        P.set_model(lam, mu, den)
+        P.set_stf(stf)
+        P.run(2)
+        d1 = P.data()
+        P.set_stf(-3.0 * stf)
+        P.run(2)
+        d3 = P.data()
+    assert rel(d3, -3.0 * d1) < 1e-6
+    o = oracle_mod.cufd(lam, mu, den, stf, 2, ids, para, precision=32)[5]
+    ref = np.concatenate([s.ravel() for s in o["syn"]])
+    assert rel(d1, ref) < SEIS_TOL
+    fwiflow.fwi_obs_op(lam, mu, den, stf, 0, ids, para)
+    assert fwiflow.fwi_op(lam, mu, den, stf, 0, ids, para) == 0.0
+
+
+def test_velocity_api_and_autograd(workspace):
+    """compute_misfit_and_gradient (FWI.jl's chain: pad, mask, moduli) and the
+    torch autograd op: backward = gradient x dy."""
+    torch = pytest.importorskip("torch")
+    c = models.case_small(nSteps=200)
+    fwi = fwiflow.make_fwi(c.nz, c.nx, c.dz, c.dx, c.nSteps, c.dt, c.src_z, c.src_x, c.rec_z,
+                           c.rec_x, nPml=c.nPml, f0=c.f0, WORKSPACE=workspace)
+    obs = fwiflow.compute_observation(fwi, c.cp, c.cs, c.rho, c.stf)
+    assert obs.shape == (2, c.nSteps, c.nx)
+    J, gcp, gcs, grho = fwiflow.compute_misfit_and_gradient(
+        fwi, c.cp0, c.cs0, c.rho0, c.stf, cp_ref=c.cp0, cs_ref=c.cs0, rho_ref=c.rho0)
+    assert gcp.shape == (c.nz, c.nx) and J > 0
+    assert J == fwiflow.compute_misfit(fwi, c.cp0, c.cs0, c.rho0, c.stf, cp_ref=c.cp0,
+                                       cs_ref=c.cs0, rho_ref=c.rho0)
+    # top 10 rows of the interior are masked (FWI.jl:49)
+    assert np.all(gcp[:10] == 0) and np.any(gcp[10:] != 0)
+    lam0, mu0, den0 = (torch.tensor(a, requires_grad=True) for a in models.moduli(c, "start"))
+    stf = torch.tensor(np.repeat(c.stf, 2, axis=0), requires_grad=True)
+    loss = 2.5 * fwiflow.ops.FwiOp.apply(lam0, mu0, den0, stf, 0, [0, 1], fwi.para_path)
+    loss.backward()
+    gL, gM, gD, gS = fwiflow.fwi_op_grad(lam0.detach().numpy(), mu0.detach().numpy(),
+                                         den0.detach().numpy(), stf.detach().numpy(), 0, [0, 1],
+                                         fwi.para_path)
+    assert np.allclose(lam0.grad.numpy(), 2.5 * gL) and np.allclose(stf.grad.numpy(), 2.5 * gS)

@@ -1,0 +1,172 @@
+"""C-ABI library: loads, exports every declared symbol, and turns each of the
+reference's exit(1) paths into a status code before any GPU work.  CPU only:
+no compute entry point is exercised on a device here."""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fwiflow
+from fwiflow import _lib, models
+from fwiflow.utils import paraGen, surveyGen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "fwiflow.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fwiflow_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    decl = _declared_symbols()
+    assert len(decl) >= 20
+    assert sorted(_lib.EXPORTS) == decl
+    for name in decl:
+        assert hasattr(L, name), name
+    assert "gfx950" in fwiflow.version()
+
+
+def _small(ws, **kw):
+    c = models.case_small(nSteps=100)
+    para, stf = models.write_case(c, ws)
+    lam, mu, den = models.moduli(c, "true")
+    return c, para, stf, lam, mu, den
+
+
+def _code(exc):
+    return exc.value.status
+
+
+def test_bad_calc_id(workspace):
+    c, para, stf, lam, mu, den = _small(workspace)
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(3, [0], para, lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_CALC_ID"
+
+
+def test_missing_parameter_file(workspace):
+    c, para, stf, lam, mu, den = _small(workspace)
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(2, [0], os.path.join(workspace, "nope.json"), lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_IO"
+
+
+def test_garbled_and_incomplete_json(workspace):
+    c, para, stf, lam, mu, den = _small(workspace)
+    good = open(para).read()
+    open(para, "w").write(good[:-5])
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(2, [0], para, lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_JSON"
+    d = json.loads(good)
+    del d["nSteps"]
+    open(para, "w").write(json.dumps(d))
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(2, [0], para, lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_JSON" and "nSteps" in str(e.value)
+
+
+def test_unsupported_options(workspace):
+    c, para, stf, lam, mu, den = _small(workspace)
+    d = json.loads(open(para).read())
+    for key, val in (("filter", [0, 0.1, 100, 200]), ("if_src_update", True), ("if_win", True)):
+        dd = dict(d)
+        dd[key] = val
+        open(para, "w").write(json.dumps(dd))
+        with pytest.raises(fwiflow.FwiflowError) as e:
+            fwiflow.check_call(2, [0], para, lam, mu, den)
+        assert _code(e) == "FWIFLOW_ERR_UNSUPPORTED"
+    # explicit false flags are accepted
+    dd = dict(d)
+    dd["if_win"] = False
+    dd["if_src_update"] = False
+    open(para, "w").write(json.dumps(dd))
+    fwiflow.check_call(2, [0], para, lam, mu, den)
+
+
+def test_shot_id_out_of_range(workspace):
+    c, para, stf, lam, mu, den = _small(workspace)
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(2, [0, 7], para, lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_SHAPE"
+
+
+def test_source_outside_grid(workspace):
+    c, para, stf, lam, mu, den = _small(workspace)
+    surveyGen([2], [c.nx + 40], c.rec_z, c.rec_x, json.loads(open(para).read())["survey_fname"])
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(2, [0], para, lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_SHAPE"
+
+
+def test_courant_violation(workspace):
+    c, para, stf, lam, mu, den = _small(workspace)
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(2, [0], para, lam * 10, mu * 10, den)
+    assert _code(e) == "FWIFLOW_ERR_COURANT"
+
+
+def test_missing_observed_data(workspace):
+    c, para, stf, lam, mu, den = _small(workspace)
+    fwiflow.check_call(2, [0, 1], para, lam, mu, den)  # calc 2 writes, needs none
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(0, [0], para, lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_IO"
+    # truncated Shot file
+    os.makedirs(os.path.join(workspace, "Data"), exist_ok=True)
+    np.zeros(10, np.float32).tofile(os.path.join(workspace, "Data", "Shot0.bin"))
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(1, [0], para, lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_IO"
+
+
+def test_host_tables_match_oracle(workspace, oracle_mod):
+    """C-PML profiles and window tables of the GPU path equal the oracle's
+    reference-arithmetic FP32 values (up to 2 ulp: host pow/exp overloads)."""
+    c, para, stf, lam, mu, den = _small(workspace)
+    nzA = c.NZ - c.nPad
+    cz, cx, wd, ws = fwiflow.host_tables(para, nzA, c.NX, c.nSteps)
+    oz = oracle_mod.cpml(nzA, c.nPml, c.dz, c.f0, c.dt, precision=32)
+    ox = oracle_mod.cpml(c.NX, c.nPml, c.dx, c.f0, c.dt, precision=32)
+    np.testing.assert_allclose(cz, oz, rtol=3e-7, atol=1e-12)
+    np.testing.assert_allclose(cx, ox, rtol=3e-7, atol=1e-12)
+    w = oracle_mod.window(np.ones((1, c.nSteps)), c.dt, 0.005, precision=32)[0]
+    np.testing.assert_allclose(wd, w, rtol=3e-7, atol=1e-12)
+    w = oracle_mod.window(np.ones((1, c.nSteps)), c.dt, 0.001, precision=32)[0]
+    np.testing.assert_allclose(ws, w, rtol=3e-7, atol=1e-12)
+
+
+def test_compute_without_gpu_fails_loudly(workspace):
+    """No CPU fallback: with no device the op returns FWIFLOW_ERR_HIP."""
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    c, para, stf, lam, mu, den = _small(workspace)
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.fwi_obs_op(lam, mu, den, stf, 0, [0], para)
+    assert _code(e) == "FWIFLOW_ERR_HIP"
+
+
+def test_paragen_surveygen_format(workspace):
+    """The files are one-line JSON with the keys Parameter.cpp / Src_Rec.cu read."""
+    p = os.path.join(workspace, "p.json")
+    s = os.path.join(workspace, "s.json")
+    paraGen(100, 200, 20, 20, 1000, 0.002, 10, 32, 28, p, s, os.path.join(workspace, "D"))
+    surveyGen([2], [100], [1, 1], [3, 4], s)
+    lines = open(p).read().splitlines()
+    assert len(lines) == 1
+    d = json.loads(lines[0])
+    assert list(d)[:9] == ["nz", "nx", "dz", "dx", "nSteps", "dt", "f0", "nPoints_pml", "nPad"]
+    assert isinstance(d["nz"], int) and isinstance(d["dt"], float)
+    sv = json.loads(open(s).read())
+    assert sv["nShots"] == 1 and sv["shot0"]["nrec"] == 2 and sv["shot0"]["x_rec"] == [3, 4]
+    assert os.path.isdir(os.path.join(workspace, "D"))

@@ -259,8 +259,18 @@ __global__ __launch_bounds__(256) void k_residual(int nSteps, int R, const float
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
        k += (size_t)gridDim.x * blockDim.x) {
     const int t = (int)(k / (size_t)R);
+    float rv;
+    {
+      // obs and syn are windowed separately (two roundings, as the reference's
+      // separate cuda_window launches do); an FMA contraction of
+      // obs*ww - syn*ww would leave a rounding residue when obs == syn.
+#pragma clang fp contract(off)
+      const float ww = w2[t];
+      const float ow = obs[k] * ww;
+      const float sw = syn[k] * ww;
+      rv = t > 0 ? ow - sw : 0.f;
+    }
     const float ww = w2[t];
-    const float rv = t > 0 ? obs[k] * ww - syn[k] * ww : 0.f;
     acc += (double)rv * (double)rv;
     res[k] = rv * ww;
   }

@@ -51,9 +51,31 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--nt", type=int, default=4000)
     p.add_argument("--nshots", type=int, default=64)
-    p.add_argument("--cpu-sample-steps", type=int, default=300)
+    p.add_argument("--cpu-sample-steps", type=int, default=1500)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a plain 1 GiB device-to-device copy, 4 times (the "
+                        "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
+
+
+def copy_calibration():
+    import torch
+
+    n = 1 << 28  # 2^28 float32 = 1 GiB
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(4):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 4
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4, "copies": 5,
+                      "copy_ms": ms, "copy_rw_GBs": 2 * n * 4 / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def log(*a):
@@ -111,6 +133,9 @@ def cpu_baseline(c, ws, sample_steps):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
               "(shots would shard one group per GPU, see DESIGN.md)", file=sys.stderr)

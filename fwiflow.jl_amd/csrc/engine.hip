@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -46,7 +47,6 @@ namespace fwiflow {
 namespace {
 
 constexpr int kBZ = 64;  // threads along z (one wave64 row)
-constexpr int kBX = 4;   // rows of x per workgroup -> 256 threads
 constexpr float kC1 = 9.0f / 8.0f;
 constexpr float kC2 = 1.0f / 24.0f;
 
@@ -108,14 +108,34 @@ __device__ inline float* ring_ptr(float* ring, const Geo& g, int it, int comp, i
   return ring + (((size_t)it * 5 + comp) * g.ns + s) * (size_t)g.ringlen + slot;
 }
 
+// 1-D launch over a 2-D tile grid.  Workgroups are dealt to the 8 XCDs
+// round-robin by linear id; with `xcd` set, the ids are permuted so that each
+// XCD walks one contiguous band of tiles (z fastest, then x), and tiles that
+// share x-halo columns share that XCD's L2.
+struct Tiling {
+  int gz, gx, n, xcd;
+};
+
+__device__ inline void tile_of(const Tiling& t, int& bz, int& bx) {
+  int b = blockIdx.x;
+  if (t.xcd) {
+    const int per = t.n / 8;
+    if (b < per * 8) b = (b % 8) * per + b / 8;
+  }
+  bz = b % t.gz;
+  bx = b / t.gz;
+}
+
 // ------------------------------------------------------------------ forward
 // el_stress.cu:50-88 + add_source (utilities.cu:521-555) + from_bnd
 // (utilities.cu:361-391): the ring cells save sigma^it and v^it before the
 // update of step it.
-__global__ __launch_bounds__(256) void k_stress_fwd(Geo g, Mdl m, Pml c, Wave w, Mem p, Src src,
+__global__ __launch_bounds__(1024) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p, Src src,
                                                     int it, float* __restrict__ ring) {
-  const int z = blockIdx.x * kBZ + threadIdx.x;
-  const int x = blockIdx.y * kBX + threadIdx.y;
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z = tz * kBZ + threadIdx.x;
+  const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
@@ -183,9 +203,11 @@ __global__ __launch_bounds__(256) void k_stress_fwd(Geo g, Mdl m, Pml c, Wave w,
 }
 
 // el_velocity.cu:45-82.  x-PML test is `x > nx - nPml` (el_velocity.cu:56,71).
-__global__ __launch_bounds__(256) void k_vel_fwd(Geo g, Mdl m, Pml c, Wave w, Mem p) {
-  const int z = blockIdx.x * kBZ + threadIdx.x;
-  const int x = blockIdx.y * kBX + threadIdx.y;
+__global__ __launch_bounds__(1024) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p) {
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z = tz * kBZ + threadIdx.x;
+  const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
@@ -302,12 +324,14 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // density-gradient accumulation.  The thread domain is the ring rectangle;
 // interior cells update, ring cells take the stored values.
 // acc_a += vz_adj (dszz/dz + dsxz/dx), acc_b += vx_adj (dsxz/dz + dsxx/dx).
-__global__ __launch_bounds__(256) void k_rev_vel(Geo g, Mdl m, Wave w, Wave a,
+__global__ __launch_bounds__(1024) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave w, Wave a,
                                                  float* __restrict__ acc_a,
                                                  float* __restrict__ acc_b,
                                                  const float* __restrict__ ring, int it) {
-  const int z = blockIdx.x * kBZ + threadIdx.x + (g.nPml - 2);
-  const int x = blockIdx.y * kBX + threadIdx.y + (g.nPml - 2);
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z = tz * kBZ + threadIdx.x + (g.nPml - 2);
+  const int x = tx * (int)blockDim.y + threadIdx.y + (g.nPml - 2);
   if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
   const int slot = ring_slot(z, x, g);
@@ -350,13 +374,15 @@ __global__ __launch_bounds__(256) void k_rev_vel(Geo g, Mdl m, Wave w, Wave a,
 // el_stress.cu:90-131 (isFor = false) fused with add_source(-) and the ring
 // restore of sigma.  acc_l += (szz_adj + sxx_adj)(dvz/dz + dvx/dx),
 // acc_m += szz_adj dvz/dz + sxx_adj dvx/dx, acc_s += sxz_adj (dvx/dz + dvz/dx).
-__global__ __launch_bounds__(256) void k_rev_stress(Geo g, Mdl m, Wave w, Wave a, Src src,
+__global__ __launch_bounds__(1024) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wave w, Wave a, Src src,
                                                     float* __restrict__ acc_l,
                                                     float* __restrict__ acc_m,
                                                     float* __restrict__ acc_s,
                                                     const float* __restrict__ ring, int it) {
-  const int z = blockIdx.x * kBZ + threadIdx.x + (g.nPml - 2);
-  const int x = blockIdx.y * kBX + threadIdx.y + (g.nPml - 2);
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z = tz * kBZ + threadIdx.x + (g.nPml - 2);
+  const int x = tx * (int)blockDim.y + threadIdx.y + (g.nPml - 2);
   if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
   const int slot = ring_slot(z, x, g);
@@ -410,10 +436,12 @@ __global__ __launch_bounds__(256) void k_rev_stress(Geo g, Mdl m, Wave w, Wave a
 // `it` (the adjoint stress it reads is not touched by this kernel).  The
 // C-PML memory terms a*dpsi are evaluated only where a != 0; elsewhere they
 // are exactly zero in the reference as well.
-__global__ __launch_bounds__(256) void k_adj_vel(Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
+__global__ __launch_bounds__(1024) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
                                                  float* __restrict__ stf_grad, int it) {
-  const int z = blockIdx.x * kBZ + threadIdx.x;
-  const int x = blockIdx.y * kBX + threadIdx.y;
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z = tz * kBZ + threadIdx.x;
+  const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
@@ -486,9 +514,11 @@ __global__ void k_res_inject(Geo g, float* __restrict__ szz, float* __restrict__
 // without PML gates in the reference; they are read only through a*dpsi terms,
 // i.e. where a != 0 (the C-PML layers) plus the 2-cell reach of the stencil,
 // so they are kept in that band (`zlive`/`xlive`) and nowhere else.
-__global__ __launch_bounds__(256) void k_adj_stress(Geo g, Mdl m, Pml c, Wave a, Mem p) {
-  const int z = blockIdx.x * kBZ + threadIdx.x;
-  const int x = blockIdx.y * kBX + threadIdx.y;
+__global__ __launch_bounds__(1024) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p) {
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z = tz * kBZ + threadIdx.x;
+  const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
@@ -543,10 +573,12 @@ __global__ __launch_bounds__(256) void k_adj_stress(Geo g, Mdl m, Pml c, Wave a,
 // run.  Deterministic gather form of the reference's atomicAdd sprays
 // (el_stress.cu:113-124, el_velocity.cu:234-243), with the same target
 // conditions (including the always-true `gidx+1<=gidx<=nx-1-nPml`).
-__global__ __launch_bounds__(256) void k_finalize(Geo g, Mdl m, const float* __restrict__ acc,
+__global__ __launch_bounds__(1024) void k_finalize(Tiling tl, Geo g, Mdl m, const float* __restrict__ acc,
                                                   float* __restrict__ grad) {
-  const int z = blockIdx.x * kBZ + threadIdx.x;
-  const int x = blockIdx.y * kBX + threadIdx.y;
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z = tz * kBZ + threadIdx.x;
+  const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z >= g.nz || x >= g.nx) return;
   const int nz = g.nz;
   const long long N = g.N;
@@ -594,11 +626,13 @@ __global__ __launch_bounds__(256) void k_finalize(Geo g, Mdl m, const float* __r
 
 // aveMuInit / aveBycInit (utilities.cu:125-152) over [2, nz-3] x [2, nx-3];
 // defaults 0 and 1/1000 elsewhere (Model.cu:67-73).
-__global__ void k_model_init(Geo g, const float* __restrict__ mu, const float* __restrict__ den,
+__global__ void k_model_init(Tiling tl, Geo g, const float* __restrict__ mu, const float* __restrict__ den,
                              float* __restrict__ amu, float* __restrict__ ba,
                              float* __restrict__ bb) {
-  const int z = blockIdx.x * kBZ + threadIdx.x;
-  const int x = blockIdx.y * kBX + threadIdx.y;
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z = tz * kBZ + threadIdx.x;
+  const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z >= g.nz || x >= g.nx) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
@@ -638,12 +672,28 @@ __global__ void k_transpose(const float* __restrict__ in, float* __restrict__ ou
 }  // namespace
 
 // ===================================================================== plan
+static Tiling make_tiling(int nz_dom, int nx_dom, int bx, bool xcd) {
+  Tiling t;
+  t.gz = (nz_dom + kBZ - 1) / kBZ;
+  t.gx = (nx_dom + bx - 1) / bx;
+  t.n = t.gz * t.gx;
+  t.xcd = xcd ? 1 : 0;
+  return t;
+}
+
+static int env_int(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return v && *v ? std::atoi(v) : dflt;
+}
+
 struct DeviceState {
   int dev = 0;
   hipStream_t stream = nullptr;
   Geo g{};
   int R = 0;
   int ckpt_every = 0;
+  int tile_x = 4;    // x extent of a workgroup tile (blockDim.y)
+  bool xcd = true;   // XCD-banded workgroup order
   std::vector<void*> allocs;
   long long bytes = 0;
   // model + tables
@@ -772,6 +822,8 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d_.reset(new DeviceState());
   DeviceState& d = *d_;
   d.dev = gpu;
+  d.tile_x = std::max(1, std::min(16, env_int("FWIFLOW_TILE_X", 4)));
+  d.xcd = env_int("FWIFLOW_XCD_REMAP", 1) != 0;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
   Geo& g = d.g;
@@ -847,8 +899,9 @@ Status Plan::set_model(const double* Lambda, const double* Mu, const double* Den
   HIP_TRY(hipMemcpyAsync(d.lam, lam.data(), nb, hipMemcpyHostToDevice, d.stream));
   HIP_TRY(hipMemcpyAsync(d.mu, mu.data(), nb, hipMemcpyHostToDevice, d.stream));
   HIP_TRY(hipMemcpyAsync(d.den, den.data(), nb, hipMemcpyHostToDevice, d.stream));
-  dim3 blk(kBZ, kBX), grd((d.g.nz + kBZ - 1) / kBZ, (d.g.nx + kBX - 1) / kBX);
-  hipLaunchKernelGGL(k_model_init, grd, blk, 0, d.stream, d.g, d.mu, d.den, d.amu, d.ba, d.bb);
+  const Tiling tA = make_tiling(d.g.nz, d.g.nx, d.tile_x, d.xcd);
+  hipLaunchKernelGGL(k_model_init, dim3(tA.n), dim3(kBZ, d.tile_x), 0, d.stream, tA, d.g, d.mu,
+                     d.den, d.amu, d.ba, d.bb);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(d.stream));
   d.model_set = true;
@@ -977,8 +1030,9 @@ Status Plan::run(int calc_id) {
   const Wave w = d.wave(d.fwd);
   const Mem p = d.memv();
   const Src src{d.src, d.inj};
-  const dim3 blk(kBZ, kBX);
-  const dim3 grdU((g.nzA + kBZ - 1) / kBZ, (g.nx + kBX - 1) / kBX);
+  const dim3 blk(kBZ, d.tile_x);
+  const Tiling tU = make_tiling(g.nzA, g.nx, d.tile_x, d.xcd);
+  const dim3 grdU(tU.n);
   const int recBlocks = (d.R + 255) / 256;
   const int sample_every = 8;
 
@@ -987,9 +1041,9 @@ Status Plan::run(int calc_id) {
     const bool smp = (it % sample_every) == 0;
     float* ring = withAdj ? d.ring : nullptr;
     d.launch("stress_fwd", smp, [&] {
-      hipLaunchKernelGGL(k_stress_fwd, grdU, blk, 0, st, g, m, c, w, p, src, it, ring);
+      hipLaunchKernelGGL(k_stress_fwd, grdU, blk, 0, st, tU, g, m, c, w, p, src, it, ring);
     });
-    d.launch("vel_fwd", smp, [&] { hipLaunchKernelGGL(k_vel_fwd, grdU, blk, 0, st, g, m, c, w, p); });
+    d.launch("vel_fwd", smp, [&] { hipLaunchKernelGGL(k_vel_fwd, grdU, blk, 0, st, tU, g, m, c, w, p); });
     hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, w.szz, w.sxx, d.rec, d.R,
                        d.syn + (size_t)(it + 1) * d.R);
   }
@@ -1009,34 +1063,35 @@ Status Plan::run(int calc_id) {
     HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(d.acc, 0, 5 * N * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(d.stf_grad, 0, (size_t)g.ns * g.nSteps * sizeof(float), st));
-    const dim3 grdR((g.ringH + kBZ - 1) / kBZ, (g.ringW + kBX - 1) / kBX);
+    const Tiling tR = make_tiling(g.ringH, g.ringW, d.tile_x, d.xcd);
+    const dim3 grdR(tR.n);
     float* acc = d.acc;
     // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
-    hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, g, m, c, a, p, src, d.stf_grad, -1);
+    hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, tU, g, m, c, a, p, src, d.stf_grad, -1);
     hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx, d.rec,
                        d.R, d.res + (size_t)(g.nSteps - 1) * d.R);
-    hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, g, m, c, a, p);
+    hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, tU, g, m, c, a, p);
     for (int it = g.nSteps - 2; it >= 0; it--) {
       const bool smp = (it % sample_every) == 0;
       d.launch("rev_vel", smp, [&] {
-        hipLaunchKernelGGL(k_rev_vel, grdR, blk, 0, st, g, m, w, a, acc + 3 * N, acc + 4 * N,
+        hipLaunchKernelGGL(k_rev_vel, grdR, blk, 0, st, tR, g, m, w, a, acc + 3 * N, acc + 4 * N,
                            d.ring, it);
       });
       d.launch("rev_stress", smp, [&] {
-        hipLaunchKernelGGL(k_rev_stress, grdR, blk, 0, st, g, m, w, a, src, acc, acc + N,
+        hipLaunchKernelGGL(k_rev_stress, grdR, blk, 0, st, tR, g, m, w, a, src, acc, acc + N,
                            acc + 2 * N, d.ring, it);
       });
       d.launch("adj_vel", smp, [&] {
-        hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, g, m, c, a, p, src, d.stf_grad, it);
+        hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, tU, g, m, c, a, p, src, d.stf_grad, it);
       });
       hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx, d.rec,
                          d.R, d.res + (size_t)it * d.R);
       d.launch("adj_stress", smp, [&] {
-        hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, g, m, c, a, p);
+        hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, tU, g, m, c, a, p);
       });
     }
-    const dim3 grdA((g.nz + kBZ - 1) / kBZ, (g.nx + kBX - 1) / kBX);
-    hipLaunchKernelGGL(k_finalize, grdA, blk, 0, st, g, m, d.acc, d.grad);
+    const Tiling tA = make_tiling(g.nz, g.nx, d.tile_x, d.xcd);
+    hipLaunchKernelGGL(k_finalize, dim3(tA.n), blk, 0, st, tA, g, m, d.acc, d.grad);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipStreamSynchronize(st));

@@ -33,6 +33,8 @@ HBM_PEAK_GBS = 8000.0
 # shots of a launch and amortised to zero; C-PML memory variables add 8 B per
 # read-modify-write where the kernel updates them.
 KERNEL_BYTES = {
+    "fwd_step": 40.0,     # fused: read vz vx szz sxx sxz, write all five (ping-pong)
+    "adj_step": 40.0,     # fused adjoint: read and write the five adjoint components
     "stress_fwd": 32.0,   # read vz vx szz sxx sxz, write szz sxx sxz
     "vel_fwd": 28.0,      # read szz sxx sxz vz vx, write vz vx
     "rev_vel": 36.0,      # read 3 stress, 2 velocity, 2 adjoint velocity, write 2 velocity
@@ -41,7 +43,7 @@ KERNEL_BYTES = {
     "adj_stress": 32.0,   # read 2 adjoint velocity, 3 adjoint stress, write 3
 }
 # C-PML memory-variable traffic per PML cell-direction (2 variables RW = 16 B)
-KERNEL_PML = {"stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
+KERNEL_PML = {"fwd_step": 32.0, "adj_step": 32.0, "stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
 
 
 def parse():

@@ -256,6 +256,577 @@ __global__ __launch_bounds__(1024) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c
   }
 }
 
+// ------------------------------------------------------------ fused forward
+// One time step (el_stress.cu:50-88 + add_source + el_velocity.cu:45-82 and
+// the ring save of from_bnd) in one pass.  A workgroup owns every row of one
+// shot for the x-segment [x0, x0+tx) and marches along x:
+//   column c  : sigma^{n+1}(., c) from v^n(., c-2..c+2)
+//   column c-2: v^{n+1}(., c-2) from sigma^{n+1}(., c-4..c)
+// x-neighbours are carried in per-thread register rings, z-neighbours are read
+// from LDS.  sigma^{n+1} is recomputed on 2 halo columns each side (same code,
+// same bits as the owner's value).  State and memory variables are read from
+// buffer A and written to buffer B, so halo reads never see a neighbour's new
+// values.
+struct Seg {
+  int tx, nseg, n, xcd;
+};
+
+__device__ inline void seg_of(const Seg& q, int ns, int& seg, int& s) {
+  int b = blockIdx.x;
+  if (q.xcd) {
+    const int per = q.n / 8;
+    if (b < per * 8) b = (b % 8) * per + b / 8;
+  }
+  seg = b / ns;
+  s = b % ns;
+}
+
+__device__ inline int mod3(int c) { return (c + 3) % 3; }
+
+__global__ __launch_bounds__(1024) void k_fwd_step(Seg q, Geo g, Mdl m, Pml c, Wave A, Wave B,
+                                                   Mem pA, Mem pB, Src src, int it,
+                                                   float* __restrict__ ring) {
+  extern __shared__ float lds[];
+  const int NT = blockDim.x;
+  float* lvz = lds;            // [2][NT] v^n at the current column
+  float* lvx = lds + 2 * NT;
+  float* lszz = lds + 4 * NT;  // [3][NT] sigma^{n+1} of the last 3 columns
+  float* lsxz = lds + 7 * NT;
+  const int z = threadIdx.x;
+  int seg, s;
+  seg_of(q, g.ns, seg, s);
+  const int x0 = seg * q.tx;
+  const int x1 = min(x0 + q.tx, g.nx);
+  const int nz = g.nz;
+  const long long so = (long long)s * g.N;
+  const bool zval = z < nz;
+  const bool zin = z >= 2 && z <= g.nzA - 3;
+  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
+  float bz = 0.f, az = 0.f, rkz = 1.f, bzh = 0.f, azh = 0.f, rkzh = 1.f;
+  if (zin && pz) {
+    bz = c.bz[z], az = c.az[z], rkz = 1.f / c.Kz[z];
+    bzh = c.bzh[z], azh = c.azh[z], rkzh = 1.f / c.Kzh[z];
+  }
+  const int zs = src.zx[2 * s], xs = src.zx[2 * s + 1];
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  auto ld = [&](const float* f, int col) -> float {
+    return (zval && col >= 0 && col < g.nx) ? f[so + (long long)col * nz + z] : 0.f;
+  };
+  // v rings: r*[k] holds column c-2+k
+  float vz0 = ld(A.vz, x0 - 4), vz1 = ld(A.vz, x0 - 3), vz2 = ld(A.vz, x0 - 2),
+        vz3 = ld(A.vz, x0 - 1), vz4;
+  float vx0 = ld(A.vx, x0 - 4), vx1 = ld(A.vx, x0 - 3), vx2 = ld(A.vx, x0 - 2),
+        vx3 = ld(A.vx, x0 - 1), vx4;
+  // sigma^{n+1} rings: sxz at columns c-4..c, sxx at columns c-3..c
+  float sxz0 = 0.f, sxz1 = 0.f, sxz2 = 0.f, sxz3 = 0.f, sxz4 = 0.f;
+  float sxx0 = 0.f, sxx1 = 0.f, sxx2 = 0.f, sxx3 = 0.f;
+  for (int col = x0 - 2; col <= x1 + 1; ++col) {
+    // ---- phase 1: bring v^n(., col+2) in, publish v^n(., col) for z-stencils
+    vz4 = ld(A.vz, col + 2);
+    vx4 = ld(A.vx, col + 2);
+    const int sv = (col & 1) * NT;
+    lvz[sv + z] = vz2;
+    lvx[sv + z] = vx2;
+    __syncthreads();
+    // ---- phase 2: sigma^{n+1}(z, col)
+    float nszz = 0.f, nsxx = 0.f, nsxz = 0.f;
+    if (zval && col >= 0 && col < g.nx) {
+      const long long i = (long long)col * nz + z;
+      const long long o = so + i;
+      float szz = A.szz[o], sxx = A.sxx[o], sxz = A.sxz[o];
+      const bool own = col >= x0 && col < x1;
+      if (zin && col >= 2 && col <= g.nx - 3) {
+        const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i];
+        const float l2m = lam + 2.f * mu;
+        const float* Vz = lvz + sv + z;
+        const float* Vx = lvx + sv + z;
+        float dvz_dz = (kC1 * (vz2 - Vz[-1]) - kC2 * (Vz[1] - Vz[-2])) * rdz;
+        float dvx_dx = (kC1 * (vx2 - vx1) - kC2 * (vx3 - vx0)) * rdx;
+        float dvx_dz = (kC1 * (Vx[1] - vx2) - kC2 * (Vx[2] - Vx[-1])) * rdz;
+        float dvz_dx = (kC1 * (vz3 - vz2) - kC2 * (vz4 - vz1)) * rdx;
+        if (pz) {
+          const float a = bz * pA.vzz[o] + az * dvz_dz;
+          dvz_dz = dvz_dz * rkz + a;
+          const float b = bzh * pA.vxz[o] + azh * dvx_dz;
+          dvx_dz = dvx_dz * rkzh + b;
+          if (own) {
+            pB.vzz[o] = a;
+            pB.vxz[o] = b;
+          }
+        }
+        if (col < g.nPml || col > g.nx - g.nPml - 1) {
+          const float a = c.bx[col] * pA.vxx[o] + c.ax[col] * dvx_dx;
+          dvx_dx = dvx_dx * (1.f / c.Kx[col]) + a;
+          const float b = c.bxh[col] * pA.vzx[o] + c.axh[col] * dvz_dx;
+          dvz_dx = dvz_dx * (1.f / c.Kxh[col]) + b;
+          if (own) {
+            pB.vxx[o] = a;
+            pB.vzx[o] = b;
+          }
+        }
+        if (own && ring) {
+          const int slot = ring_slot(z, col, g);
+          if (slot >= 0) {
+            *ring_ptr(ring, g, it, 0, s, slot) = szz;
+            *ring_ptr(ring, g, it, 1, s, slot) = sxz;
+            *ring_ptr(ring, g, it, 2, s, slot) = sxx;
+            *ring_ptr(ring, g, it, 3, s, slot) = vz2;
+            *ring_ptr(ring, g, it, 4, s, slot) = vx2;
+          }
+        }
+        szz += (l2m * dvz_dz + lam * dvx_dx) * dt;
+        sxx += (lam * dvz_dz + l2m * dvx_dx) * dt;
+        sxz += amu * (dvx_dz + dvz_dx) * dt;
+        if (z == zs && col == xs) {
+          const float inj = src.inj[(size_t)s * g.nSteps + it];
+          szz += inj;
+          sxx += 3.f * inj;
+        }
+        if (own) {
+          B.szz[o] = szz;
+          B.sxx[o] = sxx;
+          B.sxz[o] = sxz;
+        }
+      }
+      nszz = szz;
+      nsxx = sxx;
+      nsxz = sxz;
+    }
+    const int ss = mod3(col) * NT;
+    lszz[ss + z] = nszz;
+    lsxz[ss + z] = nsxz;
+    sxz0 = sxz1, sxz1 = sxz2, sxz2 = sxz3, sxz3 = sxz4, sxz4 = nsxz;
+    sxx0 = sxx1, sxx1 = sxx2, sxx2 = sxx3, sxx3 = nsxx;
+    __syncthreads();
+    // ---- phase 3: v^{n+1}(z, col-2)
+    const int cp = col - 2;
+    if (zin && cp >= x0 && cp < x1 && cp >= 2 && cp <= g.nx - 3) {
+      const long long i = (long long)cp * nz + z;
+      const long long o = so + i;
+      const float ba = m.ba[i], bb = m.bb[i];
+      const int sp = mod3(cp) * NT + z;
+      const float* Szz = lszz + sp;
+      const float* Sxz = lsxz + sp;
+      float dszz_dz = (kC1 * (Szz[1] - Szz[0]) - kC2 * (Szz[2] - Szz[-1])) * rdz;
+      float dsxz_dx = (kC1 * (sxz2 - sxz1) - kC2 * (sxz3 - sxz0)) * rdx;
+      float dsxz_dz = (kC1 * (Sxz[0] - Sxz[-1]) - kC2 * (Sxz[1] - Sxz[-2])) * rdz;
+      float dsxx_dx = (kC1 * (sxx2 - sxx1) - kC2 * (sxx3 - sxx0)) * rdx;
+      if (pz) {
+        const float a = bzh * pA.szz[o] + azh * dszz_dz;
+        pB.szz[o] = a;
+        dszz_dz = dszz_dz * rkzh + a;
+        const float b = bz * pA.sxzz[o] + az * dsxz_dz;
+        pB.sxzz[o] = b;
+        dsxz_dz = dsxz_dz * rkz + b;
+      }
+      if (cp < g.nPml || cp > g.nx - g.nPml) {
+        const float a = c.bx[cp] * pA.sxzx[o] + c.ax[cp] * dsxz_dx;
+        pB.sxzx[o] = a;
+        dsxz_dx = dsxz_dx * (1.f / c.Kx[cp]) + a;
+        const float b = c.bxh[cp] * pA.sxx[o] + c.axh[cp] * dsxx_dx;
+        pB.sxx[o] = b;
+        dsxx_dx = dsxx_dx * (1.f / c.Kxh[cp]) + b;
+      }
+      B.vz[o] = vz0 + (dszz_dz + dsxz_dx) * ba * dt;
+      B.vx[o] = vx0 + (dsxz_dz + dsxx_dx) * bb * dt;
+    }
+    vz0 = vz1, vz1 = vz2, vz2 = vz3, vz3 = vz4;
+    vx0 = vx1, vx1 = vx2, vx2 = vx3, vx3 = vx4;
+  }
+}
+
+// ------------------------------------------------------------ fused adjoint
+// One adjoint step: el_velocity_adj.cu:22-108, res_injection
+// (utilities.cu:569-580), el_stress_adj.cu:22-104 and source_grad
+// (utilities.cu:582-593) in one column-marching pass, same scheme as
+// k_fwd_step: adjoint velocity at column c, adjoint stress at column c-2,
+// state and memory variables read from A and written to B.  The residual is
+// injected into the owner's adjoint stress before the stress update, through
+// a per-shot, per-column receiver index.
+struct RecIdx {
+  const int* ptr;  // [ns][nx+1] start of each column's receivers
+  const int* zr;   // receiver row
+  const int* rg;   // receiver index in the time-major residual row
+};
+
+__global__ __launch_bounds__(1024) void k_adj_step(Seg q, Geo g, Mdl m, Pml c, Wave A, Wave B,
+                                                   Mem pA, Mem pB, Src src, RecIdx rx,
+                                                   const float* __restrict__ res_row, int R,
+                                                   float* __restrict__ stf_grad, int stf_it) {
+  extern __shared__ float lds[];
+  const int NT = blockDim.x;
+  float* lszz = lds;             // [2][NT] adjoint stress (old) at column c
+  float* lsxx = lds + 2 * NT;
+  float* lsxz = lds + 4 * NT;
+  float* lvz = lds + 6 * NT;     // [3][NT] adjoint velocity (new), last 3 columns
+  float* lvx = lds + 9 * NT;
+  float* lpzz = lds + 12 * NT;   // [3][NT] new mem_dsxz_dz
+  float* lpsz = lds + 15 * NT;   // [3][NT] new mem_dszz_dz
+  const int z = threadIdx.x;
+  int seg, s;
+  seg_of(q, g.ns, seg, s);
+  const int x0 = seg * q.tx;
+  const int x1 = min(x0 + q.tx, g.nx);
+  const int nz = g.nz;
+  const long long so = (long long)s * g.N;
+  const bool zval = z < nz;
+  const bool zin = z >= 2 && z <= g.nzA - 3;
+  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
+  const bool zlive = z <= g.nPml + 1 || z >= g.nzA - g.nPml - 2;
+  float az = 0.f, rkz = 1.f, bz = 0.f, azh = 0.f, rkzh = 1.f, bzh = 0.f;
+  if (zin) {
+    az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
+    azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
+  }
+  const int zs = src.zx[2 * s], xs = src.zx[2 * s + 1];
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  auto ld = [&](const float* f, int col) -> float {
+    return (zval && col >= 0 && col < g.nx) ? f[so + (long long)col * nz + z] : 0.f;
+  };
+  // adjoint stress (old) rings, slot k = column c-2+k
+  float zz0 = ld(A.szz, x0 - 4), zz1 = ld(A.szz, x0 - 3), zz2 = ld(A.szz, x0 - 2),
+        zz3 = ld(A.szz, x0 - 1), zz4;
+  float xx0 = ld(A.sxx, x0 - 4), xx1 = ld(A.sxx, x0 - 3), xx2 = ld(A.sxx, x0 - 2),
+        xx3 = ld(A.sxx, x0 - 1), xx4;
+  float xz0 = ld(A.sxz, x0 - 4), xz1 = ld(A.sxz, x0 - 3), xz2 = ld(A.sxz, x0 - 2),
+        xz3 = ld(A.sxz, x0 - 1), xz4;
+  // new adjoint velocity rings: vz at columns c-3..c, vx at c-4..c
+  float nvz0 = 0.f, nvz1 = 0.f, nvz2 = 0.f, nvz3 = 0.f;
+  float nvx0 = 0.f, nvx1 = 0.f, nvx2 = 0.f, nvx3 = 0.f, nvx4 = 0.f;
+  // new velocity-side memory variables: mem_dsxz_dx at c-3..c, mem_dsxx_dx at c-4..c
+  float pxz0 = 0.f, pxz1 = 0.f, pxz2 = 0.f, pxz3 = 0.f;
+  float pxx0 = 0.f, pxx1 = 0.f, pxx2 = 0.f, pxx3 = 0.f, pxx4 = 0.f;
+  for (int col = x0 - 2; col <= x1 + 1; ++col) {
+    // ---- phase 1
+    zz4 = ld(A.szz, col + 2);
+    xx4 = ld(A.sxx, col + 2);
+    xz4 = ld(A.sxz, col + 2);
+    const int sv = (col & 1) * NT;
+    lszz[sv + z] = zz2;
+    lsxx[sv + z] = xx2;
+    lsxz[sv + z] = xz2;
+    __syncthreads();
+    // ---- phase 2: adjoint velocity at (z, col)
+    float vzn = 0.f, vxn = 0.f, psxz = 0.f, psxx = 0.f, psxzz = 0.f, pszz = 0.f;
+    const bool own = col >= x0 && col < x1;
+    if (zval && col >= 0 && col < g.nx) {
+      const long long i = (long long)col * nz + z;
+      const long long o = so + i;
+      vzn = A.vz[o];
+      vxn = A.vx[o];
+      psxz = pA.sxzx[o];
+      psxx = pA.sxx[o];
+      psxzz = pA.sxzz[o];
+      pszz = pA.szz[o];
+      if (own && stf_it >= 0 && z == zs && col == xs)
+        stf_grad[(size_t)s * g.nSteps + stf_it] = -(zz2 + 3.f * xx2) * dt;
+      if (zin && col >= 2 && col <= g.nx - 3) {
+        const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
+        const float l2m = lam + 2.f * mu;
+        const float ax = c.ax[col], rkx = 1.f / c.Kx[col], bx = c.bx[col];
+        const float axh = c.axh[col], rkxh = 1.f / c.Kxh[col], bxh = c.bxh[col];
+        const bool px = col < g.nPml || col > g.nx - g.nPml - 1;
+        const float* Szz = lszz + sv + z;
+        const float* Sxx = lsxx + sv + z;
+        const float* Sxz = lsxz + sv + z;
+        const float dszz_dx = (-kC1 * (zz3 - zz2) + kC2 * (zz4 - zz1)) * rdx;
+        const float dsxx_dx = (-kC1 * (xx3 - xx2) + kC2 * (xx4 - xx1)) * rdx;
+        const float dsxz_dz = (-kC1 * (Sxz[0] - Sxz[-1]) + kC2 * (Sxz[1] - Sxz[-2])) * rdz;
+        vxn += (lam * dszz_dx * rkx * dt + l2m * dsxx_dx * rkx * dt + amu * rkzh * dsxz_dz * dt);
+        if (ax != 0.f) {
+          const float* qv = pA.vxx + o;
+          vxn += ax * ((-kC1 * (qv[nz] - qv[0]) + kC2 * (qv[2 * nz] - qv[-nz])) * rdx);
+        }
+        if (azh != 0.f) {
+          const float* qv = pA.vxz + o;
+          vxn += azh * ((-kC1 * (qv[0] - qv[-1]) + kC2 * (qv[1] - qv[-2])) * rdz);
+        }
+        if (px) psxx = bxh * psxx + bb * vxn * dt;
+        if (pz) psxzz = bz * psxzz + bb * vxn * dt;
+        const float dszz_dz = (-kC1 * (Szz[1] - Szz[0]) + kC2 * (Szz[2] - Szz[-1])) * rdz;
+        const float dsxx_dz = (-kC1 * (Sxx[1] - Sxx[0]) + kC2 * (Sxx[2] - Sxx[-1])) * rdz;
+        const float dsxz_dx = (-kC1 * (xz2 - xz1) + kC2 * (xz3 - xz0)) * rdx;
+        vzn += (l2m * dszz_dz * rkz * dt + lam * dsxx_dz * rkz * dt + amu * rkxh * dsxz_dx * dt);
+        if (az != 0.f) {
+          const float* qv = pA.vzz + o;
+          vzn += az * ((-kC1 * (qv[1] - qv[0]) + kC2 * (qv[2] - qv[-1])) * rdz);
+        }
+        if (axh != 0.f) {
+          const float* qv = pA.vzx + o;
+          vzn += axh * ((-kC1 * (qv[0] - qv[-nz]) + kC2 * (qv[nz] - qv[-2 * nz])) * rdx);
+        }
+        if (px) psxz = bx * psxz + ba * vzn * dt;
+        if (pz) pszz = bzh * pszz + ba * vzn * dt;
+        if (own) {
+          B.vx[o] = vxn;
+          B.vz[o] = vzn;
+          if (px) {
+            pB.sxx[o] = psxx;
+            pB.sxzx[o] = psxz;
+          }
+          if (pz) {
+            pB.sxzz[o] = psxzz;
+            pB.szz[o] = pszz;
+          }
+        }
+      }
+    }
+    const int sn = mod3(col) * NT;
+    lvz[sn + z] = vzn;
+    lvx[sn + z] = vxn;
+    lpzz[sn + z] = psxzz;
+    lpsz[sn + z] = pszz;
+    nvz0 = nvz1, nvz1 = nvz2, nvz2 = nvz3, nvz3 = vzn;
+    nvx0 = nvx1, nvx1 = nvx2, nvx2 = nvx3, nvx3 = nvx4, nvx4 = vxn;
+    pxz0 = pxz1, pxz1 = pxz2, pxz2 = pxz3, pxz3 = psxz;
+    pxx0 = pxx1, pxx1 = pxx2, pxx2 = pxx3, pxx3 = pxx4, pxx4 = psxx;
+    __syncthreads();
+    // ---- phase 3: residual injection + adjoint stress at (z, col-2)
+    const int cp = col - 2;
+    if (cp >= x0 && cp < x1 && zval) {
+      const long long i = (long long)cp * nz + z;
+      const long long o = so + i;
+      float szz = zz0, sxx = xx0, sxz = xz0;
+      const int* rp = rx.ptr + (size_t)s * (g.nx + 1);
+      for (int k = rp[cp]; k < rp[cp + 1]; ++k)
+        if (rx.zr[k] == z) {
+          const float r = res_row[rx.rg[k]];
+          szz += r;
+          sxx += 3.f * r;
+        }
+      if (zin && cp >= 2 && cp <= g.nx - 3) {
+        const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
+        const float l2m = lam + 2.f * mu;
+        const float ax = c.ax[cp], rkx = 1.f / c.Kx[cp], bx = c.bx[cp];
+        const float axh = c.axh[cp], rkxh = 1.f / c.Kxh[cp], bxh = c.bxh[cp];
+        const bool xlive = cp <= g.nPml + 1 || cp >= g.nx - g.nPml - 2;
+        const int sp = mod3(cp) * NT + z;
+        const float* Vz = lvz + sp;
+        const float* Vx = lvx + sp;
+        // nvz slot k = column col-3+k (cp = slot 1); nvx slot k = column col-4+k (cp = slot 2)
+        const float dvz_dx = (-kC1 * (nvz2 - nvz1) + kC2 * (nvz3 - nvz0)) * rdx;
+        const float dvx_dz = (-kC1 * (Vx[1] - Vx[0]) + kC2 * (Vx[2] - Vx[-1])) * rdz;
+        sxz += (dvz_dx * rkx * ba * dt + dvx_dz * rkz * bb * dt);
+        if (ax != 0.f)  // mem_dsxz_dx at cp-1..cp+2: pxz slot k = column col-3+k
+          sxz += ax * ((-kC1 * (pxz2 - pxz1) + kC2 * (pxz3 - pxz0)) * rdx);
+        if (az != 0.f) {
+          const float* qv = lpzz + sp;
+          sxz += az * ((-kC1 * (qv[1] - qv[0]) + kC2 * (qv[2] - qv[-1])) * rdz);
+        }
+        const float dvx_dx = (-kC1 * (nvx2 - nvx1) + kC2 * (nvx3 - nvx0)) * rdx;
+        const float dvz_dz = (-kC1 * (Vz[0] - Vz[-1]) + kC2 * (Vz[1] - Vz[-2])) * rdz;
+        sxx += bb * dvx_dx * rkxh * dt;
+        if (axh != 0.f)  // mem_dsxx_dx at cp-2..cp+1: pxx slot k = column col-4+k
+          sxx += axh * ((-kC1 * (pxx2 - pxx1) + kC2 * (pxx3 - pxx0)) * rdx);
+        szz += ba * dvz_dz * rkzh * dt;
+        if (azh != 0.f) {
+          const float* qv = lpsz + sp;
+          szz += azh * ((-kC1 * (qv[0] - qv[-1]) + kC2 * (qv[1] - qv[-2])) * rdz);
+        }
+        if (xlive) {
+          pB.vzx[o] = bxh * pA.vzx[o] + sxz * amu * dt;
+          pB.vxx[o] = bx * pA.vxx[o] + lam * szz * dt + l2m * sxx * dt;
+        }
+        if (zlive) {
+          pB.vxz[o] = bzh * pA.vxz[o] + sxz * amu * dt;
+          pB.vzz[o] = bz * pA.vzz[o] + l2m * szz * dt + lam * sxx * dt;
+        }
+      }
+      B.szz[o] = szz;
+      B.sxx[o] = sxx;
+      B.sxz[o] = sxz;
+    }
+    zz0 = zz1, zz1 = zz2, zz2 = zz3, zz3 = zz4;
+    xx0 = xx1, xx1 = xx2, xx2 = xx3, xx3 = xx4;
+    xz0 = xz1, xz1 = xz2, xz2 = xz3, xz3 = xz4;
+  }
+}
+
+// ------------------------------------------------------------ fused reverse
+// One reverse step: el_velocity.cu:84-117 and el_stress.cu:90-131 with
+// isFor = false, the ring restores of to_bnd (utilities.cu:394-424) and the
+// source removal of add_source, plus all gradient terms, in one column-marching
+// pass: v^{it}(c) from sigma^{it+1}(c-2..c+2), then sigma^{it}(c-2) from
+// v^{it}(c-4..c).  A workgroup owns one x-segment of TXR columns for a group of
+// shots; it walks the group's shots one after another and keeps the gradient
+// terms of its cells in LDS (each thread only touches its own row), then adds
+// them once to the group's slice of
+// the partial accumulators (one slice per group, so no two workgroups write
+// the same word).  State is read from A (step it+1) and written to B (step it).
+constexpr int TXR = 8;
+
+struct SegG {
+  int nseg, ngrp, gsz, n, xcd;
+};
+
+__global__ __launch_bounds__(1024) void k_rev_step(SegG q, Geo g, Mdl m, Wave A, Wave B, Wave adj,
+                                                   Src src, const float* __restrict__ ring,
+                                                   int it, float* __restrict__ accp) {
+  extern __shared__ float lds[];
+  const int NT = blockDim.x;
+  float* lszz = lds;          // [2][NT] sigma^{it+1} at column c
+  float* lsxz = lds + 2 * NT;
+  float* lvz = lds + 4 * NT;  // [3][NT] v^{it} of the last 3 columns
+  float* lvx = lds + 7 * NT;
+  float* lacc = lds + 10 * NT;  // [5][TXR][NT] gradient terms of the owned cells
+  const int z = threadIdx.x;
+  int b = blockIdx.x;
+  if (q.xcd) {
+    const int per = q.n / 8;
+    if (b < per * 8) b = (b % 8) * per + b / 8;
+  }
+  const int seg = b / q.ngrp, grp = b % q.ngrp;
+  const int x0 = seg * TXR;
+  const int x1 = min(x0 + TXR, g.nx);
+  const int nz = g.nz;
+  const bool zval = z < nz;
+  const bool zI = z >= g.nPml && z <= g.nzA - 1 - g.nPml;
+  const int xI0 = g.nPml, xI1 = g.nx - 1 - g.nPml;
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  for (int k = 0; k < 5 * TXR; ++k) lacc[k * NT + z] = 0.f;
+  auto acc = [&](int term, int j) -> float& { return lacc[(term * TXR + j) * NT + z]; };
+  const int s0 = grp * q.gsz, s1 = min(s0 + q.gsz, g.ns);
+  for (int s = s0; s < s1; ++s) {
+    const long long so = (long long)s * g.N;
+    auto ld = [&](const float* f, int col) -> float {
+      return (zval && col >= 0 && col < g.nx) ? f[so + (long long)col * nz + z] : 0.f;
+    };
+    const int zs = src.zx[2 * s], xs = src.zx[2 * s + 1];
+    // sigma^{it+1} rings, slot k = column c-2+k
+    float zz0 = ld(A.szz, x0 - 4), zz1 = ld(A.szz, x0 - 3), zz2 = ld(A.szz, x0 - 2),
+          zz3 = ld(A.szz, x0 - 1), zz4;
+    float xx0 = ld(A.sxx, x0 - 4), xx1 = ld(A.sxx, x0 - 3), xx2 = ld(A.sxx, x0 - 2),
+          xx3 = ld(A.sxx, x0 - 1), xx4;
+    float xz0 = ld(A.sxz, x0 - 4), xz1 = ld(A.sxz, x0 - 3), xz2 = ld(A.sxz, x0 - 2),
+          xz3 = ld(A.sxz, x0 - 1), xz4;
+    // v^{it} rings: vz at columns c-3..c, vx at c-4..c
+    float wz0 = 0.f, wz1 = 0.f, wz2 = 0.f, wz3 = 0.f;
+    float wx0 = 0.f, wx1 = 0.f, wx2 = 0.f, wx3 = 0.f, wx4 = 0.f;
+#pragma unroll 1
+    for (int j = 0; j < TXR + 4; ++j) {
+      const int col = x0 - 2 + j;
+      // ---- phase 1
+      zz4 = ld(A.szz, col + 2);
+      xx4 = ld(A.sxx, col + 2);
+      xz4 = ld(A.sxz, col + 2);
+      const int sv = (j & 1) * NT;
+      lszz[sv + z] = zz2;
+      lsxz[sv + z] = xz2;
+      __syncthreads();
+      // ---- phase 2: v^{it}(z, col)
+      float nvz = 0.f, nvx = 0.f;
+      const bool jown = j >= 2 && j < TXR + 2;  // col in [x0, x0+TXR)
+      if (zval && col >= 0 && col < g.nx) {
+        const long long i = (long long)col * nz + z;
+        const long long o = so + i;
+        nvz = A.vz[o];
+        nvx = A.vx[o];
+        const bool own = jown && col < x1;
+        const bool inI = zI && col >= xI0 && col <= xI1;
+        const int slot = ring_slot(z, col, g);
+        if (inI) {
+          const float* Szz = lszz + sv + z;
+          const float* Sxz = lsxz + sv + z;
+          const float dszz_dz = (kC1 * (Szz[1] - zz2) - kC2 * (Szz[2] - Szz[-1])) * rdz;
+          const float dsxz_dx = (kC1 * (xz2 - xz1) - kC2 * (xz3 - xz0)) * rdx;
+          const float dsxz_dz = (kC1 * (xz2 - Sxz[-1]) - kC2 * (Sxz[1] - Sxz[-2])) * rdz;
+          const float dsxx_dx = (kC1 * (xx3 - xx2) - kC2 * (xx4 - xx1)) * rdx;
+          const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
+          if (own) {
+            acc(3, j - 2) += adj.vz[o] * fz;
+            acc(4, j - 2) += adj.vx[o] * fx;
+          }
+          if (slot < 0) {
+            nvz -= fz * m.ba[i] * dt;
+            nvx -= fx * m.bb[i] * dt;
+          }
+        }
+        if (slot >= 0) {
+          nvz = *ring_ptr(const_cast<float*>(ring), g, it, 3, s, slot);
+          nvx = *ring_ptr(const_cast<float*>(ring), g, it, 4, s, slot);
+        }
+        if (own) {
+          B.vz[o] = nvz;
+          B.vx[o] = nvx;
+        }
+      }
+      const int sn = (j % 3) * NT;
+      lvz[sn + z] = nvz;
+      lvx[sn + z] = nvx;
+      wz0 = wz1, wz1 = wz2, wz2 = wz3, wz3 = nvz;
+      wx0 = wx1, wx1 = wx2, wx2 = wx3, wx3 = wx4, wx4 = nvx;
+      __syncthreads();
+      // ---- phase 3: sigma^{it}(z, col-2)
+      const int cp = col - 2;
+      if (j >= 4 && zval && cp < x1) {
+        const long long i = (long long)cp * nz + z;
+        const long long o = so + i;
+        float szz = zz0, sxx = xx0, sxz = xz0;
+        const bool inI = zI && cp >= xI0 && cp <= xI1;
+        const int slot = ring_slot(z, cp, g);
+        if (inI) {
+          const int sp = ((j - 2) % 3) * NT + z;
+          const float* Vz = lvz + sp;
+          const float* Vx = lvx + sp;
+          // wz slot k = column col-3+k (cp = slot 1); wx slot k = column col-4+k (cp = slot 2)
+          const float dvz_dz = (kC1 * (wz1 - Vz[-1]) - kC2 * (Vz[1] - Vz[-2])) * rdz;
+          const float dvx_dx = (kC1 * (wx2 - wx1) - kC2 * (wx3 - wx0)) * rdx;
+          const float dvx_dz = (kC1 * (Vx[1] - wx2) - kC2 * (Vx[2] - Vx[-1])) * rdz;
+          const float dvz_dx = (kC1 * (wz2 - wz1) - kC2 * (wz3 - wz0)) * rdx;
+          const float szza = adj.szz[o], sxxa = adj.sxx[o];
+          const int k = j - 4;
+          acc(0, k) += (szza + sxxa) * (dvz_dz + dvx_dx);
+          acc(1, k) += szza * dvz_dz + sxxa * dvx_dx;
+          acc(2, k) += adj.sxz[o] * (dvx_dz + dvz_dx);
+          if (slot < 0) {
+            const float lam = m.lam[i], mu = m.mu[i];
+            const float l2m = lam + 2.f * mu;
+            if (z == zs && cp == xs) {
+              const float inj = src.inj[(size_t)s * g.nSteps + it];
+              szz -= inj;
+              sxx -= 3.f * inj;
+            }
+            szz -= (l2m * dvz_dz + lam * dvx_dx) * dt;
+            sxx -= (lam * dvz_dz + l2m * dvx_dx) * dt;
+            sxz -= m.amu[i] * (dvx_dz + dvz_dx) * dt;
+          }
+        }
+        if (slot >= 0) {
+          szz = *ring_ptr(const_cast<float*>(ring), g, it, 0, s, slot);
+          sxz = *ring_ptr(const_cast<float*>(ring), g, it, 1, s, slot);
+          sxx = *ring_ptr(const_cast<float*>(ring), g, it, 2, s, slot);
+        }
+        B.szz[o] = szz;
+        B.sxx[o] = sxx;
+        B.sxz[o] = sxz;
+      }
+      zz0 = zz1, zz1 = zz2, zz2 = zz3, zz3 = zz4;
+      xx0 = xx1, xx1 = xx2, xx2 = xx3, xx3 = xx4;
+      xz0 = xz1, xz1 = xz2, xz2 = xz3, xz3 = xz4;
+    }
+  }
+  if (zI) {
+    const long long N = g.N;
+    float* P = accp + (size_t)grp * 5 * N;
+    for (int j = 0; j < TXR; ++j) {
+      const int col = x0 + j;
+      if (col < x1 && col >= xI0 && col <= xI1) {
+        const long long i = (long long)col * nz + z;
+        for (int t = 0; t < 5; ++t) P[t * N + i] += acc(t, j);
+      }
+    }
+  }
+}
+
+// Sum the per-group partial accumulators into slice 0.
+__global__ void k_sum_groups(float* __restrict__ accp, long long n5, int ngrp) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n5) return;
+  float t = accp[k];
+  for (int gi = 1; gi < ngrp; ++gi) t += accp[(size_t)gi * n5 + k];
+  accp[k] = t;
+}
+
 // recording (utilities.cu:557-567): szz + 3 sxx at each receiver, written to
 // row `t` of the time-major seismogram array.
 __global__ void k_record(Geo g, const float* __restrict__ szz, const float* __restrict__ sxx,
@@ -681,6 +1252,9 @@ static Tiling make_tiling(int nz_dom, int nx_dom, int bx, bool xcd) {
   return t;
 }
 
+// threads of a fused workgroup: whole wave64s covering the active rows
+static int fused_threads(const Geo& g) { return ((g.nzA + 63) / 64) * 64; }
+
 static int env_int(const char* name, int dflt) {
   const char* v = std::getenv(name);
   return v && *v ? std::atoi(v) : dflt;
@@ -694,6 +1268,10 @@ struct DeviceState {
   int ckpt_every = 0;
   int tile_x = 4;    // x extent of a workgroup tile (blockDim.y)
   bool xcd = true;   // XCD-banded workgroup order
+  bool fused = false; // column-marching fused kernels (FWIFLOW_FUSED=1)
+  int seg_x = 32;    // x extent of a fused segment
+  int rev_group = 8; // shots per workgroup of the fused reverse step
+  bool fused_rev = true;  // fused reverse step (needs its LDS to fit)
   std::vector<void*> allocs;
   long long bytes = 0;
   // model + tables
@@ -706,6 +1284,10 @@ struct DeviceState {
   int* rec = nullptr;     // [R][3]
   // wavefields
   float *fwd = nullptr, *adj = nullptr, *mem = nullptr;
+  float *fwd2 = nullptr, *mem2 = nullptr, *adj2 = nullptr;  // ping-pong partners
+  int *rptr = nullptr, *rzr = nullptr, *rrg = nullptr;       // receivers by column
+  float* accp = nullptr;  // per-group partial gradient accumulators
+  int accp_groups = 0;
   float* ring = nullptr;
   float *syn = nullptr, *obs = nullptr, *res = nullptr, *stage = nullptr;
   float *acc = nullptr, *grad = nullptr, *stf_grad = nullptr;
@@ -800,9 +1382,8 @@ struct DeviceState {
     const size_t F = (size_t)g.ns * g.N;
     return Wave{base, base + F, base + 2 * F, base + 3 * F, base + 4 * F};
   }
-  Mem memv() const {
+  Mem memv(float* b) const {
     const size_t F = (size_t)g.ns * g.N;
-    float* b = mem;
     return Mem{b, b + F, b + 2 * F, b + 3 * F, b + 4 * F, b + 5 * F, b + 6 * F, b + 7 * F};
   }
 };
@@ -824,6 +1405,9 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.dev = gpu;
   d.tile_x = std::max(1, std::min(16, env_int("FWIFLOW_TILE_X", 4)));
   d.xcd = env_int("FWIFLOW_XCD_REMAP", 1) != 0;
+  d.fused = env_int("FWIFLOW_FUSED", 0) != 0;
+  d.seg_x = std::max(4, std::min(256, env_int("FWIFLOW_SEG_X", 32)));
+  d.rev_group = std::max(1, std::min(64, env_int("FWIFLOW_REV_GROUP", 8)));
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
   Geo& g = d.g;
@@ -875,10 +1459,49 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   HIP_TRY(hipMemcpy(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
   TRY(d.alloc(&d.rec, rec.size()));
   HIP_TRY(hipMemcpy(d.rec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
+  {
+    // receivers of each shot grouped by column (CSR) for the fused adjoint
+    std::vector<int> ptr((size_t)g.ns * (g.nx + 1), 0), zr(d.R), rg(d.R);
+    size_t base = 0, rglob = 0;
+    for (int s = 0; s < g.ns; s++) {
+      const Shot& sh = survey.shots[s];
+      int* P = ptr.data() + (size_t)s * (g.nx + 1);
+      std::vector<int> cnt(g.nx + 1, 0);
+      for (size_t k = 0; k < sh.x_rec.size(); k++) cnt[sh.x_rec[k] + 1]++;
+      for (int x = 0; x < g.nx; x++) cnt[x + 1] += cnt[x];
+      for (int x = 0; x <= g.nx; x++) P[x] = (int)base + cnt[x];
+      std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+      for (size_t k = 0; k < sh.x_rec.size(); k++) {
+        const size_t pos = base + fill[sh.x_rec[k]]++;
+        zr[pos] = sh.z_rec[k];
+        rg[pos] = (int)(rglob + k);
+      }
+      base += sh.x_rec.size();
+      rglob += sh.x_rec.size();
+    }
+    TRY(d.alloc(&d.rptr, ptr.size()));
+    TRY(d.alloc(&d.rzr, zr.size()));
+    TRY(d.alloc(&d.rrg, rg.size()));
+    HIP_TRY(hipMemcpy(d.rptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.rzr, zr.data(), zr.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.rrg, rg.data(), rg.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   TRY(d.alloc(&d.inj, (size_t)g.ns * g.nSteps));
   const size_t F = (size_t)g.ns * N;
   TRY(d.alloc(&d.fwd, 5 * F));
   TRY(d.alloc(&d.mem, 8 * F));
+  if (d.fused && fused_threads(g) <= 1024) {
+    TRY(d.alloc(&d.fwd2, 5 * F));
+    TRY(d.alloc(&d.mem2, 8 * F));
+    const size_t ldsr = (size_t)(10 + 5 * TXR) * fused_threads(g) * sizeof(float);
+    d.fused_rev = env_int("FWIFLOW_FUSED_REV", 1) != 0 && ldsr <= 160 * 1024;
+    if (d.fused_rev)
+      HIP_TRY(hipFuncSetAttribute((const void*)k_rev_step,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsr));
+  } else {
+    d.fused = false;
+    d.fused_rev = false;
+  }
   const size_t T = (size_t)g.nSteps * d.R;
   TRY(d.alloc(&d.syn, T));
   d.npart = 1024;
@@ -1015,6 +1638,7 @@ Status Plan::run(int calc_id) {
   const size_t T = (size_t)g.nSteps * d.R;
   if (withAdj) {
     if (!d.adj) TRY(d.alloc(&d.adj, 5 * F));
+    if (d.fused && !d.adj2) TRY(d.alloc(&d.adj2, 5 * F));
     if (!d.ring) TRY(d.alloc(&d.ring, (size_t)g.nSteps * 5 * g.ns * (size_t)g.ringlen));
     if (!d.acc) TRY(d.alloc(&d.acc, 5 * N));
     if (!d.grad) TRY(d.alloc(&d.grad, 3 * N));
@@ -1023,31 +1647,63 @@ Status Plan::run(int calc_id) {
   hipStream_t st = d.stream;
   HIP_TRY(hipMemsetAsync(d.fwd, 0, 5 * F * sizeof(float), st));
   HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
+  if (d.fused) {
+    HIP_TRY(hipMemsetAsync(d.fwd2, 0, 5 * F * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(d.mem2, 0, 8 * F * sizeof(float), st));
+  }
   HIP_TRY(hipMemsetAsync(d.syn, 0, T * sizeof(float), st));
 
   const Mdl m = d.mdl();
   const Pml c = d.pml_ptrs();
-  const Wave w = d.wave(d.fwd);
-  const Mem p = d.memv();
+  float* fa = d.fwd;   // state n
+  float* fb = d.fwd2;  // state n+1 (fused path)
+  float* ma = d.mem;
+  float* mb = d.mem2;
   const Src src{d.src, d.inj};
   const dim3 blk(kBZ, d.tile_x);
   const Tiling tU = make_tiling(g.nzA, g.nx, d.tile_x, d.xcd);
   const dim3 grdU(tU.n);
   const int recBlocks = (d.R + 255) / 256;
   const int sample_every = 8;
+  Seg q;
+  q.tx = d.seg_x;
+  q.nseg = (g.nx + q.tx - 1) / q.tx;
+  q.n = q.nseg * g.ns;
+  q.xcd = d.xcd ? 1 : 0;
+  const int NTf = fused_threads(g);
+  const size_t ldsf = (size_t)10 * NTf * sizeof(float);
 
   // ---------------------------------------------------------- forward
   for (int it = 0; it <= g.nSteps - 2; it++) {
     const bool smp = (it % sample_every) == 0;
     float* ring = withAdj ? d.ring : nullptr;
-    d.launch("stress_fwd", smp, [&] {
-      hipLaunchKernelGGL(k_stress_fwd, grdU, blk, 0, st, tU, g, m, c, w, p, src, it, ring);
-    });
-    d.launch("vel_fwd", smp, [&] { hipLaunchKernelGGL(k_vel_fwd, grdU, blk, 0, st, tU, g, m, c, w, p); });
-    hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, w.szz, w.sxx, d.rec, d.R,
-                       d.syn + (size_t)(it + 1) * d.R);
+    if (d.fused) {
+      const Wave wa = d.wave(fa), wb = d.wave(fb);
+      const Mem pa = d.memv(ma), pb = d.memv(mb);
+      d.launch("fwd_step", smp, [&] {
+        hipLaunchKernelGGL(k_fwd_step, dim3(q.n), dim3(NTf), ldsf, st, q, g, m, c, wa, wb, pa, pb,
+                           src, it, ring);
+      });
+      hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, wb.szz, wb.sxx, d.rec,
+                         d.R, d.syn + (size_t)(it + 1) * d.R);
+      std::swap(fa, fb);
+      std::swap(ma, mb);
+    } else {
+      const Wave w = d.wave(fa);
+      const Mem p = d.memv(ma);
+      d.launch("stress_fwd", smp, [&] {
+        hipLaunchKernelGGL(k_stress_fwd, grdU, blk, 0, st, tU, g, m, c, w, p, src, it, ring);
+      });
+      d.launch("vel_fwd", smp, [&] {
+        hipLaunchKernelGGL(k_vel_fwd, grdU, blk, 0, st, tU, g, m, c, w, p);
+      });
+      hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, w.szz, w.sxx, d.rec,
+                         d.R, d.syn + (size_t)(it + 1) * d.R);
+    }
   }
   HIP_TRY(hipGetLastError());
+  const Wave w = d.wave(fa);  // final forward state
+  const Mem p = d.memv(ma);
 
   if (if_res) {
     hipLaunchKernelGGL(k_residual, dim3(d.npart), dim3(256), 0, st, g.nSteps, d.R, d.win2, d.obs,
@@ -1058,40 +1714,100 @@ Status Plan::run(int calc_id) {
 
   if (withAdj) {
     // ------------------------------------------------------- backward
-    const Wave a = d.wave(d.adj);
     HIP_TRY(hipMemsetAsync(d.adj, 0, 5 * F * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(ma, 0, 8 * F * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(d.acc, 0, 5 * N * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(d.stf_grad, 0, (size_t)g.ns * g.nSteps * sizeof(float), st));
     const Tiling tR = make_tiling(g.ringH, g.ringW, d.tile_x, d.xcd);
     const dim3 grdR(tR.n);
     float* acc = d.acc;
-    // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
-    hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, tU, g, m, c, a, p, src, d.stf_grad, -1);
-    hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx, d.rec,
-                       d.R, d.res + (size_t)(g.nSteps - 1) * d.R);
-    hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, tU, g, m, c, a, p);
-    for (int it = g.nSteps - 2; it >= 0; it--) {
-      const bool smp = (it % sample_every) == 0;
-      d.launch("rev_vel", smp, [&] {
-        hipLaunchKernelGGL(k_rev_vel, grdR, blk, 0, st, tR, g, m, w, a, acc + 3 * N, acc + 4 * N,
-                           d.ring, it);
-      });
-      d.launch("rev_stress", smp, [&] {
-        hipLaunchKernelGGL(k_rev_stress, grdR, blk, 0, st, tR, g, m, w, a, src, acc, acc + N,
-                           acc + 2 * N, d.ring, it);
-      });
-      d.launch("adj_vel", smp, [&] {
-        hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, tU, g, m, c, a, p, src, d.stf_grad, it);
-      });
+    if (d.fused) {
+      HIP_TRY(hipMemsetAsync(d.adj2, 0, 5 * F * sizeof(float), st));
+      HIP_TRY(hipMemsetAsync(mb, 0, 8 * F * sizeof(float), st));
+      float* aa = d.adj;  // adjoint state before the step
+      float* ab = d.adj2;
+      float* qa = ma;     // adjoint memory variables
+      float* qb = mb;
+      const RecIdx rx{d.rptr, d.rzr, d.rrg};
+      const size_t ldsa = (size_t)18 * NTf * sizeof(float);
+      auto adj_step = [&](int res_t, int stf_it) {
+        hipLaunchKernelGGL(k_adj_step, dim3(q.n), dim3(NTf), ldsa, st, q, g, m, c, d.wave(aa),
+                           d.wave(ab), d.memv(qa), d.memv(qb), src, rx,
+                           d.res + (size_t)res_t * d.R, d.R, d.stf_grad, stf_it);
+        std::swap(aa, ab);
+        std::swap(qa, qb);
+      };
+      SegG qr;
+      qr.nseg = (g.nx + TXR - 1) / TXR;
+      qr.gsz = d.rev_group;
+      qr.ngrp = (g.ns + qr.gsz - 1) / qr.gsz;
+      qr.n = qr.nseg * qr.ngrp;
+      qr.xcd = d.xcd ? 1 : 0;
+      if (d.accp_groups < qr.ngrp) {
+        TRY(d.alloc(&d.accp, (size_t)qr.ngrp * 5 * N));
+        d.accp_groups = qr.ngrp;
+      }
+      HIP_TRY(hipMemsetAsync(d.accp, 0, (size_t)qr.ngrp * 5 * N * sizeof(float), st));
+      const size_t ldsr = (size_t)(10 + 5 * TXR) * NTf * sizeof(float);
+      float* ra = fa;  // forward state at step it+1
+      float* rb = fb;  // forward state at step it
+      // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
+      adj_step(g.nSteps - 1, -1);
+      for (int it = g.nSteps - 2; it >= 0; it--) {
+        const bool smp = (it % sample_every) == 0;
+        const Wave a = d.wave(aa);
+        if (d.fused_rev) {
+          const Wave wa = d.wave(ra), wb = d.wave(rb);
+          d.launch("rev_step", smp, [&] {
+            hipLaunchKernelGGL(k_rev_step, dim3(qr.n), dim3(NTf), ldsr, st, qr, g, m, wa, wb, a,
+                               src, d.ring, it, d.accp);
+          });
+          std::swap(ra, rb);
+        } else {
+          d.launch("rev_vel", smp, [&] {
+            hipLaunchKernelGGL(k_rev_vel, grdR, blk, 0, st, tR, g, m, w, a, d.accp + 3 * N,
+                               d.accp + 4 * N, d.ring, it);
+          });
+          d.launch("rev_stress", smp, [&] {
+            hipLaunchKernelGGL(k_rev_stress, grdR, blk, 0, st, tR, g, m, w, a, src, d.accp,
+                               d.accp + N, d.accp + 2 * N, d.ring, it);
+          });
+        }
+        d.launch("adj_step", smp, [&] { adj_step(it, it); });
+      }
+      const long long n5 = 5 * (long long)N;
+      hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)((n5 + 255) / 256)), dim3(256), 0, st,
+                         d.accp, n5, qr.ngrp);
+      acc = d.accp;
+    } else {
+      const Wave a = d.wave(d.adj);
+      // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
+      hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, tU, g, m, c, a, p, src, d.stf_grad, -1);
       hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx, d.rec,
-                         d.R, d.res + (size_t)it * d.R);
-      d.launch("adj_stress", smp, [&] {
-        hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, tU, g, m, c, a, p);
-      });
+                         d.R, d.res + (size_t)(g.nSteps - 1) * d.R);
+      hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, tU, g, m, c, a, p);
+      for (int it = g.nSteps - 2; it >= 0; it--) {
+        const bool smp = (it % sample_every) == 0;
+        d.launch("rev_vel", smp, [&] {
+          hipLaunchKernelGGL(k_rev_vel, grdR, blk, 0, st, tR, g, m, w, a, acc + 3 * N, acc + 4 * N,
+                             d.ring, it);
+        });
+        d.launch("rev_stress", smp, [&] {
+          hipLaunchKernelGGL(k_rev_stress, grdR, blk, 0, st, tR, g, m, w, a, src, acc, acc + N,
+                             acc + 2 * N, d.ring, it);
+        });
+        d.launch("adj_vel", smp, [&] {
+          hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, tU, g, m, c, a, p, src, d.stf_grad, it);
+        });
+        hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx,
+                           d.rec, d.R, d.res + (size_t)it * d.R);
+        d.launch("adj_stress", smp, [&] {
+          hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, tU, g, m, c, a, p);
+        });
+      }
     }
     const Tiling tA = make_tiling(g.nz, g.nx, d.tile_x, d.xcd);
-    hipLaunchKernelGGL(k_finalize, dim3(tA.n), blk, 0, st, tA, g, m, d.acc, d.grad);
+    hipLaunchKernelGGL(k_finalize, dim3(tA.n), blk, 0, st, tA, g, m, acc, d.grad);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipStreamSynchronize(st));

@@ -59,18 +59,34 @@ def _parity(oracle_mod, case, ws, ids=None, check_fp64=True):
     r = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=32)
     out["misfit32"] = abs(J - Jo) / abs(Jo)
     out["lam32"], out["mu32"], out["den32"] = rel(gL, r[1]), rel(gM, r[2]), rel(gD, r[3])
-    out["stf32"] = rel(gS[:len(ids)], r[4][:len(ids)])
+    g = len(ids)
+    out["stf32"] = rel(gS[:g], r[4][:g])
+    r64 = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=64)
+    out["stf64"] = rel(gS[:g], r64[4][:g])
+    # FP32 conditioning of the source-time-function gradient: the oracle's own
+    # FP32 run against its FP64 run
+    out["stf_fp32_spread"] = rel(r[4][:g], r64[4][:g])
     if check_fp64:
-        r64 = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=64)
         out["lam64"], out["mu64"], out["den64"] = rel(gL, r64[1]), rel(gM, r64[2]), rel(gD, r64[3])
     say(case.name, {k: f"{v:.2e}" for k, v in out.items()})
     return out
 
 
 def _assert(out):
+    # the stf gradient is a point sample of the adjoint field at the source;
+    # where FP32 itself scatters it by ~1e-3 (source next to the C-PML) the
+    # bar is 3x that measured spread
+    stf_tol = max(GRAD_TOL, 3.0 * out.get("stf_fp32_spread", 0.0))
     for k, v in out.items():
-        tol = SEIS_TOL if k.startswith(("seis", "misfit")) else GRAD_TOL
-        assert v <= tol, (k, v)
+        if k == "stf_fp32_spread":
+            continue
+        if k.startswith(("seis", "misfit")):
+            tol = SEIS_TOL
+        elif k.startswith("stf"):
+            tol = stf_tol
+        else:
+            tol = GRAD_TOL
+        assert v <= tol, (k, v, tol)
 
 
 def test_small_case_parity(oracle_mod, workspace):
@@ -221,3 +237,46 @@ def test_velocity_api_and_autograd(workspace):
                                          den0.detach().numpy(), stf.detach().numpy(), 0, [0, 1],
                                          fwi.para_path)
     assert np.allclose(lam0.grad.numpy(), 2.5 * gL) and np.allclose(stf.grad.numpy(), 2.5 * gS)
+
+
+def _run_plan(para, ids, lam, mu, den, stf, env):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        with fwiflow.Plan(para, ids) as P:
+            P.set_model(lam, mu, den)
+            P.set_stf(stf)
+            P.load_obs()
+            P.run(2)
+            d = P.data()
+            P.run(1)
+            g = P.gradients()
+            P.run(0)
+            J = P.misfit()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return d, g, J
+
+
+@pytest.mark.parametrize("seg_x", ["7", "32"])
+def test_fused_forward_matches_split_kernels(workspace, seg_x):
+    """The column-marching fused forward step (ping-pong state, recomputed
+    halo columns) and the split stress/velocity kernels give the same data,
+    misfit and gradients; odd segment widths exercise partial segments."""
+    c = models.case_small(nz=40, nx=61, nSteps=240, nshots=3)
+    para, stf = models.write_case(c, workspace)
+    lam, mu, den = models.moduli(c, "true")
+    fwiflow.fwi_obs_op(lam * 1.02, mu, den, stf, 0, [0, 1, 2], para)
+    lam0, mu0, den0 = models.moduli(c, "start")
+    ids = [0, 1, 2]
+    d0, g0, J0 = _run_plan(para, ids, lam0, mu0, den0, stf, {"FWIFLOW_FUSED": "0"})
+    d1, g1, J1 = _run_plan(para, ids, lam0, mu0, den0, stf,
+                           {"FWIFLOW_FUSED": "1", "FWIFLOW_SEG_X": seg_x})
+    assert rel(d1, d0) < 1e-6
+    assert abs(J1 - J0) <= 1e-6 * abs(J0)
+    for a, b in zip(g1, g0):
+        assert rel(a, b) < 1e-5

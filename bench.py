@@ -33,7 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # shots of a launch and amortised to zero; C-PML memory variables add 8 B per
 # read-modify-write where the kernel updates them.
 KERNEL_BYTES = {
+    "fwd_tile": 40.0,     # tiled fused forward: read and write the five components
     "fwd_step": 40.0,     # fused: read vz vx szz sxx sxz, write all five (ping-pong)
+    "rev_step": 60.0,     # fused reverse: read/write 5 forward components, read 5 adjoint
     "adj_step": 40.0,     # fused adjoint: read and write the five adjoint components
     "stress_fwd": 32.0,   # read vz vx szz sxx sxz, write szz sxx sxz
     "vel_fwd": 28.0,      # read szz sxx sxz vz vx, write vz vx
@@ -43,7 +45,7 @@ KERNEL_BYTES = {
     "adj_stress": 32.0,   # read 2 adjoint velocity, 3 adjoint stress, write 3
 }
 # C-PML memory-variable traffic per PML cell-direction (2 variables RW = 16 B)
-KERNEL_PML = {"fwd_step": 32.0, "adj_step": 32.0, "stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
+KERNEL_PML = {"fwd_tile": 32.0, "fwd_step": 32.0, "adj_step": 32.0, "stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
 
 
 def parse():

@@ -827,6 +827,164 @@ __global__ void k_sum_groups(float* __restrict__ accp, long long n5, int ngrp) {
   accp[k] = t;
 }
 
+// ------------------------------------------------------------ tiled fused forward
+// One time step on a 32x32 output tile, every shot of the group walked inside
+// the block.  Per shot: v^n on the tile + 4-cell halo goes to LDS, sigma^{n+1}
+// is computed on the tile + 2-cell halo (recomputed by neighbours, same bits)
+// into LDS, then v^{n+1} on the tile.  The model parameters of the cells a
+// thread handles stay in registers for all shots.  State and memory variables
+// are read from A and written to B.
+constexpr int TT = 32;
+constexpr int TR4 = TT + 8;  // 40
+constexpr int TR2 = TT + 4;  // 36
+constexpr int NR4 = TR4 * TR4;
+constexpr int NR2 = TR2 * TR2;
+
+__global__ __launch_bounds__(1024) void k_fwd_tile(Tiling tl, Geo g, Mdl m, Pml c, Wave A, Wave B,
+                                                   Mem pA, Mem pB, Src src, int it,
+                                                   float* __restrict__ ring) {
+  __shared__ float svz[NR4], svx[NR4];
+  __shared__ float sszz[NR2], ssxx[NR2], ssxz[NR2];
+  const int t = threadIdx.x;
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int z0 = tz * TT, x0 = tx * TT;
+  const int nz = g.nz, nx = g.nx;
+  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  const int oz = t % TT, ox = t / TT;
+  const int zo = z0 + oz, xo = x0 + ox;
+  const bool updo = zo >= 2 && zo <= g.nzA - 3 && xo >= 2 && xo <= nx - 3;
+  const long long io = (long long)xo * nz + zo;
+  for (int s = 0; s < g.ns; ++s) {
+    const long long so = (long long)s * g.N;
+    // ---- v^n on the tile + 4-cell halo, sigma^n on the 2-cell halo region
+    float lvz0 = 0.f, lvx0 = 0.f, lvz1 = 0.f, lvx1 = 0.f;
+    {
+      const int k = t;
+      const int zz = z0 - 4 + k % TR4, xx = x0 - 4 + k / TR4;
+      if (zz >= 0 && zz < nz && xx >= 0 && xx < nx) {
+        const long long o = so + (long long)xx * nz + zz;
+        lvz0 = A.vz[o];
+        lvx0 = A.vx[o];
+      }
+    }
+    if (t + 1024 < NR4) {
+      const int k = t + 1024;
+      const int zz = z0 - 4 + k % TR4, xx = x0 - 4 + k / TR4;
+      if (zz >= 0 && zz < nz && xx >= 0 && xx < nx) {
+        const long long o = so + (long long)xx * nz + zz;
+        lvz1 = A.vz[o];
+        lvx1 = A.vx[o];
+      }
+    }
+    __syncthreads();  // previous shot's readers are done with the LDS tiles
+    svz[t] = lvz0;
+    svx[t] = lvx0;
+    if (t + 1024 < NR4) {
+      svz[t + 1024] = lvz1;
+      svx[t + 1024] = lvx1;
+    }
+    __syncthreads();
+    // ---- sigma^{n+1} on the tile + 2-cell halo
+    for (int k = t; k < NR2; k += 1024) {
+      const int zz = z0 - 2 + k % TR2, xx = x0 - 2 + k / TR2;
+      float szz = 0.f, sxx = 0.f, sxz = 0.f;
+      if (zz >= 0 && zz < nz && xx >= 0 && xx < nx) {
+        const long long i = (long long)xx * nz + zz;
+        const long long o = so + i;
+        szz = A.szz[o];
+        sxx = A.sxx[o];
+        sxz = A.sxz[o];
+        if (zz >= 2 && zz <= g.nzA - 3 && xx >= 2 && xx <= nx - 3) {
+          const bool own = zz >= z0 && zz < z0 + TT && xx >= x0 && xx < x0 + TT;
+          const int q = ((xx - x0 + 4) * TR4) + (zz - z0 + 4);
+          const float vz0 = svz[q], vx0 = svx[q];
+          float dvz_dz = (kC1 * (vz0 - svz[q - 1]) - kC2 * (svz[q + 1] - svz[q - 2])) * rdz;
+          float dvx_dx = (kC1 * (vx0 - svx[q - TR4]) - kC2 * (svx[q + TR4] - svx[q - 2 * TR4])) * rdx;
+          float dvx_dz = (kC1 * (svx[q + 1] - vx0) - kC2 * (svx[q + 2] - svx[q - 1])) * rdz;
+          float dvz_dx = (kC1 * (svz[q + TR4] - vz0) - kC2 * (svz[q + 2 * TR4] - svz[q - TR4])) * rdx;
+          if (zz < g.nPml || zz > g.nzA - g.nPml - 1) {
+            const float a = c.bz[zz] * pA.vzz[o] + c.az[zz] * dvz_dz;
+            dvz_dz = dvz_dz * (1.f / c.Kz[zz]) + a;
+            const float b = c.bzh[zz] * pA.vxz[o] + c.azh[zz] * dvx_dz;
+            dvx_dz = dvx_dz * (1.f / c.Kzh[zz]) + b;
+            if (own) {
+              pB.vzz[o] = a;
+              pB.vxz[o] = b;
+            }
+          }
+          if (xx < g.nPml || xx > nx - g.nPml - 1) {
+            const float a = c.bx[xx] * pA.vxx[o] + c.ax[xx] * dvx_dx;
+            dvx_dx = dvx_dx * (1.f / c.Kx[xx]) + a;
+            const float b = c.bxh[xx] * pA.vzx[o] + c.axh[xx] * dvz_dx;
+            dvz_dx = dvz_dx * (1.f / c.Kxh[xx]) + b;
+            if (own) {
+              pB.vxx[o] = a;
+              pB.vzx[o] = b;
+            }
+          }
+          if (own && ring) {
+            const int slot = ring_slot(zz, xx, g);
+            if (slot >= 0) {
+              *ring_ptr(ring, g, it, 0, s, slot) = szz;
+              *ring_ptr(ring, g, it, 1, s, slot) = sxz;
+              *ring_ptr(ring, g, it, 2, s, slot) = sxx;
+              *ring_ptr(ring, g, it, 3, s, slot) = vz0;
+              *ring_ptr(ring, g, it, 4, s, slot) = vx0;
+            }
+          }
+          const float lam = m.lam[i], l2m = lam + 2.f * m.mu[i];
+          szz += (l2m * dvz_dz + lam * dvx_dx) * dt;
+          sxx += (lam * dvz_dz + l2m * dvx_dx) * dt;
+          sxz += m.amu[i] * (dvx_dz + dvz_dx) * dt;
+          if (zz == src.zx[2 * s] && xx == src.zx[2 * s + 1]) {
+            const float inj = src.inj[(size_t)s * g.nSteps + it];
+            szz += inj;
+            sxx += 3.f * inj;
+          }
+          if (own) {
+            B.szz[o] = szz;
+            B.sxx[o] = sxx;
+            B.sxz[o] = sxz;
+          }
+        }
+      }
+      sszz[k] = szz;
+      ssxx[k] = sxx;
+      ssxz[k] = sxz;
+    }
+    __syncthreads();
+    // ---- v^{n+1} on the tile
+    if (updo) {
+      const long long o = so + io;
+      const int q = (ox + 2) * TR2 + (oz + 2);
+      const int q4 = (ox + 4) * TR4 + (oz + 4);
+      float dszz_dz = (kC1 * (sszz[q + 1] - sszz[q]) - kC2 * (sszz[q + 2] - sszz[q - 1])) * rdz;
+      float dsxz_dx = (kC1 * (ssxz[q] - ssxz[q - TR2]) - kC2 * (ssxz[q + TR2] - ssxz[q - 2 * TR2])) * rdx;
+      float dsxz_dz = (kC1 * (ssxz[q] - ssxz[q - 1]) - kC2 * (ssxz[q + 1] - ssxz[q - 2])) * rdz;
+      float dsxx_dx = (kC1 * (ssxx[q + TR2] - ssxx[q]) - kC2 * (ssxx[q + 2 * TR2] - ssxx[q - TR2])) * rdx;
+      if (zo < g.nPml || zo > g.nzA - g.nPml - 1) {
+        const float a = c.bzh[zo] * pA.szz[o] + c.azh[zo] * dszz_dz;
+        pB.szz[o] = a;
+        dszz_dz = dszz_dz * (1.f / c.Kzh[zo]) + a;
+        const float b = c.bz[zo] * pA.sxzz[o] + c.az[zo] * dsxz_dz;
+        pB.sxzz[o] = b;
+        dsxz_dz = dsxz_dz * (1.f / c.Kz[zo]) + b;
+      }
+      if (xo < g.nPml || xo > nx - g.nPml) {  // el_velocity.cu:56,71
+        const float a = c.bx[xo] * pA.sxzx[o] + c.ax[xo] * dsxz_dx;
+        pB.sxzx[o] = a;
+        dsxz_dx = dsxz_dx * (1.f / c.Kx[xo]) + a;
+        const float b = c.bxh[xo] * pA.sxx[o] + c.axh[xo] * dsxx_dx;
+        pB.sxx[o] = b;
+        dsxx_dx = dsxx_dx * (1.f / c.Kxh[xo]) + b;
+      }
+      B.vz[o] = svz[q4] + (dszz_dz + dsxz_dx) * m.ba[io] * dt;
+      B.vx[o] = svx[q4] + (dsxz_dz + dsxx_dx) * m.bb[io] * dt;
+    }
+  }
+}
+
 // recording (utilities.cu:557-567): szz + 3 sxx at each receiver, written to
 // row `t` of the time-major seismogram array.
 __global__ void k_record(Geo g, const float* __restrict__ szz, const float* __restrict__ sxx,
@@ -1269,6 +1427,7 @@ struct DeviceState {
   int tile_x = 4;    // x extent of a workgroup tile (blockDim.y)
   bool xcd = true;   // XCD-banded workgroup order
   bool fused = false; // column-marching fused kernels (FWIFLOW_FUSED=1)
+  bool tiled = false;  // tiled fused forward step (FWIFLOW_TILED)
   int seg_x = 32;    // x extent of a fused segment
   int rev_group = 8; // shots per workgroup of the fused reverse step
   bool fused_rev = true;  // fused reverse step (needs its LDS to fit)
@@ -1406,6 +1565,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.tile_x = std::max(1, std::min(16, env_int("FWIFLOW_TILE_X", 4)));
   d.xcd = env_int("FWIFLOW_XCD_REMAP", 1) != 0;
   d.fused = env_int("FWIFLOW_FUSED", 0) != 0;
+  d.tiled = env_int("FWIFLOW_TILED", 0) != 0 && !d.fused;
   d.seg_x = std::max(4, std::min(256, env_int("FWIFLOW_SEG_X", 32)));
   d.rev_group = std::max(1, std::min(64, env_int("FWIFLOW_REV_GROUP", 8)));
   HIP_TRY(hipSetDevice(gpu));
@@ -1490,7 +1650,11 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   const size_t F = (size_t)g.ns * N;
   TRY(d.alloc(&d.fwd, 5 * F));
   TRY(d.alloc(&d.mem, 8 * F));
-  if (d.fused && fused_threads(g) <= 1024) {
+  if (d.tiled) {
+    TRY(d.alloc(&d.fwd2, 5 * F));
+    TRY(d.alloc(&d.mem2, 8 * F));
+    d.fused_rev = false;
+  } else if (d.fused && fused_threads(g) <= 1024) {
     TRY(d.alloc(&d.fwd2, 5 * F));
     TRY(d.alloc(&d.mem2, 8 * F));
     const size_t ldsr = (size_t)(10 + 5 * TXR) * fused_threads(g) * sizeof(float);
@@ -1647,10 +1811,15 @@ Status Plan::run(int calc_id) {
   hipStream_t st = d.stream;
   HIP_TRY(hipMemsetAsync(d.fwd, 0, 5 * F * sizeof(float), st));
   HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
-  if (d.fused) {
+  if (d.fused || d.tiled) {
     HIP_TRY(hipMemsetAsync(d.fwd2, 0, 5 * F * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(d.mem2, 0, 8 * F * sizeof(float), st));
   }
+  Tiling tT;
+  tT.gz = (g.nzA + TT - 1) / TT;
+  tT.gx = (g.nx + TT - 1) / TT;
+  tT.n = tT.gz * tT.gx;
+  tT.xcd = d.xcd ? 1 : 0;
   HIP_TRY(hipMemsetAsync(d.syn, 0, T * sizeof(float), st));
 
   const Mdl m = d.mdl();
@@ -1677,7 +1846,18 @@ Status Plan::run(int calc_id) {
   for (int it = 0; it <= g.nSteps - 2; it++) {
     const bool smp = (it % sample_every) == 0;
     float* ring = withAdj ? d.ring : nullptr;
-    if (d.fused) {
+    if (d.tiled) {
+      const Wave wa = d.wave(fa), wb = d.wave(fb);
+      const Mem pa = d.memv(ma), pb = d.memv(mb);
+      d.launch("fwd_tile", smp, [&] {
+        hipLaunchKernelGGL(k_fwd_tile, dim3(tT.n), dim3(1024), 0, st, tT, g, m, c, wa, wb, pa, pb,
+                           src, it, ring);
+      });
+      hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, wb.szz, wb.sxx, d.rec,
+                         d.R, d.syn + (size_t)(it + 1) * d.R);
+      std::swap(fa, fb);
+      std::swap(ma, mb);
+    } else if (d.fused) {
       const Wave wa = d.wave(fa), wb = d.wave(fb);
       const Mem pa = d.memv(ma), pb = d.memv(mb);
       d.launch("fwd_step", smp, [&] {

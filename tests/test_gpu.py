@@ -262,20 +262,24 @@ def _run_plan(para, ids, lam, mu, den, stf, env):
     return d, g, J
 
 
-@pytest.mark.parametrize("seg_x", ["7", "32"])
-def test_fused_forward_matches_split_kernels(workspace, seg_x):
-    """The column-marching fused forward step (ping-pong state, recomputed
-    halo columns) and the split stress/velocity kernels give the same data,
-    misfit and gradients; odd segment widths exercise partial segments."""
+@pytest.mark.parametrize("variant", [{"FWIFLOW_FUSED": "1", "FWIFLOW_SEG_X": "7"},
+                                     {"FWIFLOW_FUSED": "1", "FWIFLOW_SEG_X": "32"},
+                                     {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "1"}],
+                         ids=["column7", "column32", "tiled"])
+def test_fused_kernels_match_split_kernels(workspace, variant):
+    """The fused kernels (column-marching fused forward/reverse/adjoint steps
+    with odd or even segment widths, or the tiled fused forward step; all
+    with ping-pong state and recomputed halos) give the same data, misfit and
+    gradients as the split one-stage kernels."""
     c = models.case_small(nz=40, nx=61, nSteps=240, nshots=3)
     para, stf = models.write_case(c, workspace)
     lam, mu, den = models.moduli(c, "true")
     fwiflow.fwi_obs_op(lam * 1.02, mu, den, stf, 0, [0, 1, 2], para)
     lam0, mu0, den0 = models.moduli(c, "start")
     ids = [0, 1, 2]
-    d0, g0, J0 = _run_plan(para, ids, lam0, mu0, den0, stf, {"FWIFLOW_FUSED": "0"})
-    d1, g1, J1 = _run_plan(para, ids, lam0, mu0, den0, stf,
-                           {"FWIFLOW_FUSED": "1", "FWIFLOW_SEG_X": seg_x})
+    d0, g0, J0 = _run_plan(para, ids, lam0, mu0, den0, stf,
+                           {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "0"})
+    d1, g1, J1 = _run_plan(para, ids, lam0, mu0, den0, stf, variant)
     assert rel(d1, d0) < 1e-6
     assert abs(J1 - J0) <= 1e-6 * abs(J0)
     for a, b in zip(g1, g0):

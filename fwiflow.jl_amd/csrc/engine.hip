@@ -6,7 +6,7 @@
 //   wavefield  vz, vx, szz, sxx, sxz  per shot        [shot][NX][NZ]
 //   adjoint    same 5 components                      [shot][NX][NZ]
 //   C-PML memory variables (8), per shot              [shot][NX][NZ]
-//   boundary ring, 5 layers x 4 sides, 5 components   [step][component][shot][ring]
+//   boundary ring, 5 layers x 4 sides, 5 components   [shot][step][component][ring]
 //   seismograms syn/obs/res, time-major               [nSteps][all receivers]
 //   gradient accumulators (5, summed over shots)      [NX][NZ]
 // One thread owns one cell and loops over every shot of the group: the model
@@ -105,7 +105,7 @@ __host__ __device__ inline int ring_slot(int z, int x, const Geo& g) {
 }
 
 __device__ inline float* ring_ptr(float* ring, const Geo& g, int it, int comp, int s, int slot) {
-  return ring + (((size_t)it * 5 + comp) * g.ns + s) * (size_t)g.ringlen + slot;
+  return ring + (((size_t)s * g.nSteps + it) * 5 + comp) * (size_t)g.ringlen + slot;
 }
 
 // 1-D launch over a 2-D tile grid.  Workgroups are dealt to the 8 XCDs
@@ -126,11 +126,37 @@ __device__ inline void tile_of(const Tiling& t, int& bz, int& bx) {
   bx = b / t.gz;
 }
 
+// Loads and stores at a byte offset from a wave-uniform base: the base stays
+// in scalar registers (per shot, per stencil neighbour) and each lane carries
+// one 32-bit offset, so a shot's loads need no 64-bit address arithmetic.
+__device__ __forceinline__ float ldo(const float* base, unsigned bo) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + bo);
+}
+__device__ __forceinline__ void sto(float* base, unsigned bo, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + bo) = v;
+}
+
 // ------------------------------------------------------------------ forward
+// The split kernels below walk all shots of the launch inside each thread, so
+// model parameters and C-PML coefficients are read once per cell.  Shots are
+// taken U at a time: every load of the U shots is issued before the first
+// store, so a wave keeps U shots' worth of reads in flight instead of one
+// (the stores of shot s may alias the loads of shot s+1 as far as the
+// compiler knows, so it cannot reorder them itself).
+
 // el_stress.cu:50-88 + add_source (utilities.cu:521-555) + from_bnd
 // (utilities.cu:361-391): the ring cells save sigma^it and v^it before the
 // update of step it.
-__global__ __launch_bounds__(1024) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p, Src src,
+struct StressIn {
+  float vz0, vzm1, vzp1, vzm2, vzpx, vzp2x, vzmx;
+  float vx0, vxmx, vxpx, vxm2x, vxp1, vxp2, vxm1;
+  float szz, sxx, sxz;
+  float mvzz, mvxz, mvxx, mvzx;
+  int sz, sx;
+};
+
+template <int U>
+__global__ __launch_bounds__(256) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p, Src src,
                                                     int it, float* __restrict__ ring) {
   int tz, tx;
   tile_of(tl, tz, tx);
@@ -139,6 +165,7 @@ __global__ __launch_bounds__(1024) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pm
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i];
   const float l2m = lam + 2.f * mu;
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
@@ -155,55 +182,85 @@ __global__ __launch_bounds__(1024) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pm
   }
   const int slot = ring ? ring_slot(z, x, g) : -1;
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s = 0; s < g.ns; ++s) {
-    const long long o = s * g.N + i;
-    const float* vz = w.vz + o;
-    const float* vx = w.vx + o;
-    const float vz0 = vz[0], vx0 = vx[0];
-    float dvz_dz = (kC1 * (vz0 - vz[-1]) - kC2 * (vz[1] - vz[-2])) * rdz;
-    float dvx_dx = (kC1 * (vx0 - vx[-nz]) - kC2 * (vx[nz] - vx[-2 * nz])) * rdx;
-    float dvx_dz = (kC1 * (vx[1] - vx0) - kC2 * (vx[2] - vx[-1])) * rdz;
-    float dvz_dx = (kC1 * (vz[nz] - vz0) - kC2 * (vz[2 * nz] - vz[-nz])) * rdx;
-    if (pz) {
-      const float a = bz * p.vzz[o] + az * dvz_dz;
-      p.vzz[o] = a;
-      dvz_dz = dvz_dz * rkz + a;
-      const float b = bzh * p.vxz[o] + azh * dvx_dz;
-      p.vxz[o] = b;
-      dvx_dz = dvx_dz * rkzh + b;
+  for (int s0 = 0; s0 < g.ns; s0 += U) {
+    StressIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const float* vz = w.vz + so;
+      const float* vx = w.vx + so;
+      StressIn& q = r[u];
+      q.vz0 = ldo(vz, bo), q.vzm1 = ldo(vz - 1, bo), q.vzp1 = ldo(vz + 1, bo), q.vzm2 = ldo(vz - 2, bo);
+      q.vzpx = ldo(vz + nz, bo), q.vzp2x = ldo(vz + 2 * nz, bo), q.vzmx = ldo(vz - nz, bo);
+      q.vx0 = ldo(vx, bo), q.vxmx = ldo(vx - nz, bo), q.vxpx = ldo(vx + nz, bo), q.vxm2x = ldo(vx - 2 * nz, bo);
+      q.vxp1 = ldo(vx + 1, bo), q.vxp2 = ldo(vx + 2, bo), q.vxm1 = ldo(vx - 1, bo);
+      q.szz = ldo(w.szz + so, bo), q.sxx = ldo(w.sxx + so, bo), q.sxz = ldo(w.sxz + so, bo);
+      if (pz) q.mvzz = ldo(p.vzz + so, bo), q.mvxz = ldo(p.vxz + so, bo);
+      if (px) q.mvxx = ldo(p.vxx + so, bo), q.mvzx = ldo(p.vzx + so, bo);
+      q.sz = src.zx[2 * s], q.sx = src.zx[2 * s + 1];
     }
-    if (px) {
-      const float a = bx * p.vxx[o] + ax * dvx_dx;
-      p.vxx[o] = a;
-      dvx_dx = dvx_dx * rkx + a;
-      const float b = bxh * p.vzx[o] + axh * dvz_dx;
-      p.vzx[o] = b;
-      dvz_dx = dvz_dx * rkxh + b;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const StressIn& q = r[u];
+      float dvz_dz = (kC1 * (q.vz0 - q.vzm1) - kC2 * (q.vzp1 - q.vzm2)) * rdz;
+      float dvx_dx = (kC1 * (q.vx0 - q.vxmx) - kC2 * (q.vxpx - q.vxm2x)) * rdx;
+      float dvx_dz = (kC1 * (q.vxp1 - q.vx0) - kC2 * (q.vxp2 - q.vxm1)) * rdz;
+      float dvz_dx = (kC1 * (q.vzpx - q.vz0) - kC2 * (q.vzp2x - q.vzmx)) * rdx;
+      if (pz) {
+        const float a = bz * q.mvzz + az * dvz_dz;
+        sto(p.vzz + so, bo, a);
+        dvz_dz = dvz_dz * rkz + a;
+        const float b = bzh * q.mvxz + azh * dvx_dz;
+        sto(p.vxz + so, bo, b);
+        dvx_dz = dvx_dz * rkzh + b;
+      }
+      if (px) {
+        const float a = bx * q.mvxx + ax * dvx_dx;
+        sto(p.vxx + so, bo, a);
+        dvx_dx = dvx_dx * rkx + a;
+        const float b = bxh * q.mvzx + axh * dvz_dx;
+        sto(p.vzx + so, bo, b);
+        dvz_dx = dvz_dx * rkxh + b;
+      }
+      float szz = q.szz, sxx = q.sxx, sxz = q.sxz;
+      if (slot >= 0) {
+        *ring_ptr(ring, g, it, 0, s, slot) = szz;
+        *ring_ptr(ring, g, it, 1, s, slot) = sxz;
+        *ring_ptr(ring, g, it, 2, s, slot) = sxx;
+        *ring_ptr(ring, g, it, 3, s, slot) = q.vz0;
+        *ring_ptr(ring, g, it, 4, s, slot) = q.vx0;
+      }
+      szz += (l2m * dvz_dz + lam * dvx_dx) * dt;
+      sxx += (lam * dvz_dz + l2m * dvx_dx) * dt;
+      sxz += amu * (dvx_dz + dvz_dx) * dt;
+      if (z == q.sz && x == q.sx) {
+        const float inj = src.inj[(size_t)s * g.nSteps + it];
+        szz += inj;
+        sxx += 3.f * inj;
+      }
+      sto(w.szz + so, bo, szz);
+      sto(w.sxx + so, bo, sxx);
+      sto(w.sxz + so, bo, sxz);
     }
-    float szz = w.szz[o], sxx = w.sxx[o], sxz = w.sxz[o];
-    if (slot >= 0) {
-      *ring_ptr(ring, g, it, 0, s, slot) = szz;
-      *ring_ptr(ring, g, it, 1, s, slot) = sxz;
-      *ring_ptr(ring, g, it, 2, s, slot) = sxx;
-      *ring_ptr(ring, g, it, 3, s, slot) = vz0;
-      *ring_ptr(ring, g, it, 4, s, slot) = vx0;
-    }
-    szz += (l2m * dvz_dz + lam * dvx_dx) * dt;
-    sxx += (lam * dvz_dz + l2m * dvx_dx) * dt;
-    sxz += amu * (dvx_dz + dvz_dx) * dt;
-    if (z == src.zx[2 * s] && x == src.zx[2 * s + 1]) {
-      const float inj = src.inj[(size_t)s * g.nSteps + it];
-      szz += inj;
-      sxx += 3.f * inj;
-    }
-    w.szz[o] = szz;
-    w.sxx[o] = sxx;
-    w.sxz[o] = sxz;
   }
 }
 
 // el_velocity.cu:45-82.  x-PML test is `x > nx - nPml` (el_velocity.cu:56,71).
-__global__ __launch_bounds__(1024) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p) {
+struct VelIn {
+  float szz0, szzp1, szzp2, szzm1;
+  float sxz0, sxzmx, sxzpx, sxzm2x, sxzm1, sxzp1, sxzm2;
+  float sxxpx, sxx0, sxxp2x, sxxmx;
+  float mszz, msxzz, msxzx, msxx;
+  float vz, vx;
+};
+
+template <int U>
+__global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p) {
   int tz, tx;
   tile_of(tl, tz, tx);
   const int z = tz * kBZ + threadIdx.x;
@@ -211,6 +268,7 @@ __global__ __launch_bounds__(1024) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const float ba = m.ba[i], bb = m.bb[i];
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
   const bool px = x < g.nPml || x > g.nx - g.nPml;
@@ -225,34 +283,57 @@ __global__ __launch_bounds__(1024) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c
     bxh = c.bxh[x], axh = c.axh[x], rkxh = 1.f / c.Kxh[x];
   }
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s = 0; s < g.ns; ++s) {
-    const long long o = s * g.N + i;
-    const float* szz = w.szz + o;
-    const float* sxx = w.sxx + o;
-    const float* sxz = w.sxz + o;
-    const float sxz0 = sxz[0];
-    float dszz_dz = (kC1 * (szz[1] - szz[0]) - kC2 * (szz[2] - szz[-1])) * rdz;
-    float dsxz_dx = (kC1 * (sxz0 - sxz[-nz]) - kC2 * (sxz[nz] - sxz[-2 * nz])) * rdx;
-    float dsxz_dz = (kC1 * (sxz0 - sxz[-1]) - kC2 * (sxz[1] - sxz[-2])) * rdz;
-    float dsxx_dx = (kC1 * (sxx[nz] - sxx[0]) - kC2 * (sxx[2 * nz] - sxx[-nz])) * rdx;
-    if (pz) {
-      const float a = bzh * p.szz[o] + azh * dszz_dz;
-      p.szz[o] = a;
-      dszz_dz = dszz_dz * rkzh + a;
-      const float b = bz * p.sxzz[o] + az * dsxz_dz;
-      p.sxzz[o] = b;
-      dsxz_dz = dsxz_dz * rkz + b;
+  for (int s0 = 0; s0 < g.ns; s0 += U) {
+    VelIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const float* szz = w.szz + so;
+      const float* sxx = w.sxx + so;
+      const float* sxz = w.sxz + so;
+      VelIn& q = r[u];
+      q.szz0 = ldo(szz, bo), q.szzp1 = ldo(szz + 1, bo), q.szzp2 = ldo(szz + 2, bo), q.szzm1 = ldo(szz - 1, bo);
+      q.sxz0 = ldo(sxz, bo), q.sxzmx = ldo(sxz - nz, bo), q.sxzpx = ldo(sxz + nz, bo), q.sxzm2x = ldo(sxz - 2 * nz, bo);
+      q.sxzm1 = ldo(sxz - 1, bo), q.sxzp1 = ldo(sxz + 1, bo), q.sxzm2 = ldo(sxz - 2, bo);
+      q.sxxpx = ldo(sxx + nz, bo), q.sxx0 = ldo(sxx, bo), q.sxxp2x = ldo(sxx + 2 * nz, bo), q.sxxmx = ldo(sxx - nz, bo);
+      if (pz) q.mszz = ldo(p.szz + so, bo), q.msxzz = ldo(p.sxzz + so, bo);
+      if (px) q.msxzx = ldo(p.sxzx + so, bo), q.msxx = ldo(p.sxx + so, bo);
+      q.vz = ldo(w.vz + so, bo), q.vx = ldo(w.vx + so, bo);
     }
-    if (px) {
-      const float a = bx * p.sxzx[o] + ax * dsxz_dx;
-      p.sxzx[o] = a;
-      dsxz_dx = dsxz_dx * rkx + a;
-      const float b = bxh * p.sxx[o] + axh * dsxx_dx;
-      p.sxx[o] = b;
-      dsxx_dx = dsxx_dx * rkxh + b;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const VelIn& q = r[u];
+      float dszz_dz = (kC1 * (q.szzp1 - q.szz0) - kC2 * (q.szzp2 - q.szzm1)) * rdz;
+      float dsxz_dx = (kC1 * (q.sxz0 - q.sxzmx) - kC2 * (q.sxzpx - q.sxzm2x)) * rdx;
+      float dsxz_dz = (kC1 * (q.sxz0 - q.sxzm1) - kC2 * (q.sxzp1 - q.sxzm2)) * rdz;
+      float dsxx_dx = (kC1 * (q.sxxpx - q.sxx0) - kC2 * (q.sxxp2x - q.sxxmx)) * rdx;
+      if (pz) {
+        const float a = bzh * q.mszz + azh * dszz_dz;
+        sto(p.szz + so, bo, a);
+        dszz_dz = dszz_dz * rkzh + a;
+        const float b = bz * q.msxzz + az * dsxz_dz;
+        sto(p.sxzz + so, bo, b);
+        dsxz_dz = dsxz_dz * rkz + b;
+      }
+      if (px) {
+        const float a = bx * q.msxzx + ax * dsxz_dx;
+        sto(p.sxzx + so, bo, a);
+        dsxz_dx = dsxz_dx * rkx + a;
+        const float b = bxh * q.msxx + axh * dsxx_dx;
+        sto(p.sxx + so, bo, b);
+        dsxx_dx = dsxx_dx * rkxh + b;
+      }
+      float vz = q.vz, vx = q.vx;
+      vz += (dszz_dz + dsxz_dx) * ba * dt;
+      vx += (dsxz_dz + dsxx_dx) * bb * dt;
+      sto(w.vz + so, bo, vz);
+      sto(w.vx + so, bo, vx);
     }
-    w.vz[o] += (dszz_dz + dsxz_dx) * ba * dt;
-    w.vx[o] += (dsxz_dz + dsxx_dx) * bb * dt;
   }
 }
 
@@ -1053,7 +1134,16 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // density-gradient accumulation.  The thread domain is the ring rectangle;
 // interior cells update, ring cells take the stored values.
 // acc_a += vz_adj (dszz/dz + dsxz/dx), acc_b += vx_adj (dsxz/dz + dsxx/dx).
-__global__ __launch_bounds__(1024) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave w, Wave a,
+// Shots are batched U at a time as in the forward kernels.
+struct RevVelIn {
+  float szzp1, szz0, szzp2, szzm1;
+  float sxz0, sxzmx, sxzpx, sxzm2x, sxzm1, sxzp1, sxzm2;
+  float sxxpx, sxx0, sxxp2x, sxxmx;
+  float avz, avx, vz, vx;
+};
+
+template <int U>
+__global__ __launch_bounds__(256) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave w, Wave a,
                                                  float* __restrict__ acc_a,
                                                  float* __restrict__ acc_b,
                                                  const float* __restrict__ ring, int it) {
@@ -1067,31 +1157,57 @@ __global__ __launch_bounds__(1024) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave 
   if (!in && slot < 0) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const float ba = m.ba[i], bb = m.bb[i];
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
   float ga = 0.f, gb = 0.f;
-  for (int s = 0; s < g.ns; ++s) {
-    const long long o = s * g.N + i;
-    if (in) {
-      const float* szz = w.szz + o;
-      const float* sxx = w.sxx + o;
-      const float* sxz = w.sxz + o;
-      const float sxz0 = sxz[0];
-      const float dszz_dz = (kC1 * (szz[1] - szz[0]) - kC2 * (szz[2] - szz[-1])) * rdz;
-      const float dsxz_dx = (kC1 * (sxz0 - sxz[-nz]) - kC2 * (sxz[nz] - sxz[-2 * nz])) * rdx;
-      const float dsxz_dz = (kC1 * (sxz0 - sxz[-1]) - kC2 * (sxz[1] - sxz[-2])) * rdz;
-      const float dsxx_dx = (kC1 * (sxx[nz] - sxx[0]) - kC2 * (sxx[2 * nz] - sxx[-nz])) * rdx;
-      const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
-      ga += a.vz[o] * fz;
-      gb += a.vx[o] * fx;
+  for (int s0 = 0; s0 < g.ns; s0 += U) {
+    RevVelIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      RevVelIn& q = r[u];
+      if (in) {
+        const float* szz = w.szz + so;
+        const float* sxx = w.sxx + so;
+        const float* sxz = w.sxz + so;
+        q.szzp1 = ldo(szz + 1, bo), q.szz0 = ldo(szz, bo), q.szzp2 = ldo(szz + 2, bo), q.szzm1 = ldo(szz - 1, bo);
+        q.sxz0 = ldo(sxz, bo), q.sxzmx = ldo(sxz - nz, bo), q.sxzpx = ldo(sxz + nz, bo), q.sxzm2x = ldo(sxz - 2 * nz, bo);
+        q.sxzm1 = ldo(sxz - 1, bo), q.sxzp1 = ldo(sxz + 1, bo), q.sxzm2 = ldo(sxz - 2, bo);
+        q.sxxpx = ldo(sxx + nz, bo), q.sxx0 = ldo(sxx, bo), q.sxxp2x = ldo(sxx + 2 * nz, bo), q.sxxmx = ldo(sxx - nz, bo);
+        q.avz = ldo(a.vz + so, bo), q.avx = ldo(a.vx + so, bo);
+      }
       if (slot < 0) {
-        w.vz[o] -= fz * ba * dt;
-        w.vx[o] -= fx * bb * dt;
+        q.vz = ldo(w.vz + so, bo), q.vx = ldo(w.vx + so, bo);
+      } else {
+        q.vz = *ring_ptr(const_cast<float*>(ring), g, it, 3, s, slot);
+        q.vx = *ring_ptr(const_cast<float*>(ring), g, it, 4, s, slot);
       }
     }
-    if (slot >= 0) {
-      w.vz[o] = *ring_ptr(const_cast<float*>(ring), g, it, 3, s, slot);
-      w.vx[o] = *ring_ptr(const_cast<float*>(ring), g, it, 4, s, slot);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const RevVelIn& q = r[u];
+      float vz = q.vz, vx = q.vx;
+      if (in) {
+        const float dszz_dz = (kC1 * (q.szzp1 - q.szz0) - kC2 * (q.szzp2 - q.szzm1)) * rdz;
+        const float dsxz_dx = (kC1 * (q.sxz0 - q.sxzmx) - kC2 * (q.sxzpx - q.sxzm2x)) * rdx;
+        const float dsxz_dz = (kC1 * (q.sxz0 - q.sxzm1) - kC2 * (q.sxzp1 - q.sxzm2)) * rdz;
+        const float dsxx_dx = (kC1 * (q.sxxpx - q.sxx0) - kC2 * (q.sxxp2x - q.sxxmx)) * rdx;
+        const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
+        ga += q.avz * fz;
+        gb += q.avx * fx;
+        if (slot < 0) {
+          vz -= fz * ba * dt;
+          vx -= fx * bb * dt;
+        }
+      }
+      sto(w.vz + so, bo, vz);
+      sto(w.vx + so, bo, vx);
     }
   }
   if (in) {
@@ -1103,7 +1219,16 @@ __global__ __launch_bounds__(1024) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave 
 // el_stress.cu:90-131 (isFor = false) fused with add_source(-) and the ring
 // restore of sigma.  acc_l += (szz_adj + sxx_adj)(dvz/dz + dvx/dx),
 // acc_m += szz_adj dvz/dz + sxx_adj dvx/dx, acc_s += sxz_adj (dvx/dz + dvz/dx).
-__global__ __launch_bounds__(1024) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wave w, Wave a, Src src,
+struct RevStressIn {
+  float vz0, vzm1, vzp1, vzm2, vzpx, vzp2x, vzmx;
+  float vx0, vxmx, vxpx, vxm2x, vxp1, vxp2, vxm1;
+  float aszz, asxx, asxz;
+  float szz, sxx, sxz;
+  int sz, sx;
+};
+
+template <int U>
+__global__ __launch_bounds__(256) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wave w, Wave a, Src src,
                                                     float* __restrict__ acc_l,
                                                     float* __restrict__ acc_m,
                                                     float* __restrict__ acc_s,
@@ -1118,40 +1243,66 @@ __global__ __launch_bounds__(1024) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wa
   if (!in && slot < 0) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i];
   const float l2m = lam + 2.f * mu;
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
   float gl = 0.f, gm = 0.f, gs = 0.f;
-  for (int s = 0; s < g.ns; ++s) {
-    const long long o = s * g.N + i;
-    if (in) {
-      const float* vz = w.vz + o;
-      const float* vx = w.vx + o;
-      const float vz0 = vz[0], vx0 = vx[0];
-      const float dvz_dz = (kC1 * (vz0 - vz[-1]) - kC2 * (vz[1] - vz[-2])) * rdz;
-      const float dvx_dx = (kC1 * (vx0 - vx[-nz]) - kC2 * (vx[nz] - vx[-2 * nz])) * rdx;
-      const float dvx_dz = (kC1 * (vx[1] - vx0) - kC2 * (vx[2] - vx[-1])) * rdz;
-      const float dvz_dx = (kC1 * (vz[nz] - vz0) - kC2 * (vz[2 * nz] - vz[-nz])) * rdx;
-      const float szza = a.szz[o], sxxa = a.sxx[o];
-      gl += (szza + sxxa) * (dvz_dz + dvx_dx);
-      gm += szza * dvz_dz + sxxa * dvx_dx;
-      gs += a.sxz[o] * (dvx_dz + dvz_dx);
+  for (int s0 = 0; s0 < g.ns; s0 += U) {
+    RevStressIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      RevStressIn& q = r[u];
+      if (in) {
+        const float* vz = w.vz + so;
+        const float* vx = w.vx + so;
+        q.vz0 = ldo(vz, bo), q.vzm1 = ldo(vz - 1, bo), q.vzp1 = ldo(vz + 1, bo), q.vzm2 = ldo(vz - 2, bo);
+        q.vzpx = ldo(vz + nz, bo), q.vzp2x = ldo(vz + 2 * nz, bo), q.vzmx = ldo(vz - nz, bo);
+        q.vx0 = ldo(vx, bo), q.vxmx = ldo(vx - nz, bo), q.vxpx = ldo(vx + nz, bo), q.vxm2x = ldo(vx - 2 * nz, bo);
+        q.vxp1 = ldo(vx + 1, bo), q.vxp2 = ldo(vx + 2, bo), q.vxm1 = ldo(vx - 1, bo);
+        q.aszz = ldo(a.szz + so, bo), q.asxx = ldo(a.sxx + so, bo), q.asxz = ldo(a.sxz + so, bo);
+      }
       if (slot < 0) {
-        float szz = w.szz[o], sxx = w.sxx[o];
-        if (z == src.zx[2 * s] && x == src.zx[2 * s + 1]) {
-          const float inj = src.inj[(size_t)s * g.nSteps + it];
-          szz -= inj;
-          sxx -= 3.f * inj;
-        }
-        w.szz[o] = szz - (l2m * dvz_dz + lam * dvx_dx) * dt;
-        w.sxx[o] = sxx - (lam * dvz_dz + l2m * dvx_dx) * dt;
-        w.sxz[o] -= amu * (dvx_dz + dvz_dx) * dt;
+        q.szz = ldo(w.szz + so, bo), q.sxx = ldo(w.sxx + so, bo), q.sxz = ldo(w.sxz + so, bo);
+        q.sz = src.zx[2 * s], q.sx = src.zx[2 * s + 1];
+      } else {
+        q.szz = *ring_ptr(const_cast<float*>(ring), g, it, 0, s, slot);
+        q.sxz = *ring_ptr(const_cast<float*>(ring), g, it, 1, s, slot);
+        q.sxx = *ring_ptr(const_cast<float*>(ring), g, it, 2, s, slot);
       }
     }
-    if (slot >= 0) {
-      w.szz[o] = *ring_ptr(const_cast<float*>(ring), g, it, 0, s, slot);
-      w.sxz[o] = *ring_ptr(const_cast<float*>(ring), g, it, 1, s, slot);
-      w.sxx[o] = *ring_ptr(const_cast<float*>(ring), g, it, 2, s, slot);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const RevStressIn& q = r[u];
+      float szz = q.szz, sxx = q.sxx, sxz = q.sxz;
+      if (in) {
+        const float dvz_dz = (kC1 * (q.vz0 - q.vzm1) - kC2 * (q.vzp1 - q.vzm2)) * rdz;
+        const float dvx_dx = (kC1 * (q.vx0 - q.vxmx) - kC2 * (q.vxpx - q.vxm2x)) * rdx;
+        const float dvx_dz = (kC1 * (q.vxp1 - q.vx0) - kC2 * (q.vxp2 - q.vxm1)) * rdz;
+        const float dvz_dx = (kC1 * (q.vzpx - q.vz0) - kC2 * (q.vzp2x - q.vzmx)) * rdx;
+        gl += (q.aszz + q.asxx) * (dvz_dz + dvx_dx);
+        gm += q.aszz * dvz_dz + q.asxx * dvx_dx;
+        gs += q.asxz * (dvx_dz + dvz_dx);
+        if (slot < 0) {
+          if (z == q.sz && x == q.sx) {
+            const float inj = src.inj[(size_t)s * g.nSteps + it];
+            szz -= inj;
+            sxx -= 3.f * inj;
+          }
+          szz = szz - (l2m * dvz_dz + lam * dvx_dx) * dt;
+          sxx = sxx - (lam * dvz_dz + l2m * dvx_dx) * dt;
+          sxz -= amu * (dvx_dz + dvz_dx) * dt;
+        }
+      }
+      sto(w.szz + so, bo, szz);
+      sto(w.sxx + so, bo, sxx);
+      sto(w.sxz + so, bo, sxz);
     }
   }
   if (in) {
@@ -1165,7 +1316,18 @@ __global__ __launch_bounds__(1024) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wa
 // `it` (the adjoint stress it reads is not touched by this kernel).  The
 // C-PML memory terms a*dpsi are evaluated only where a != 0; elsewhere they
 // are exactly zero in the reference as well.
-__global__ __launch_bounds__(1024) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
+struct AdjVelIn {
+  float szz0, szzpx, szzp2x, szzmx, szzp1, szzp2, szzm1;
+  float sxx0, sxxpx, sxxp2x, sxxmx, sxxp1, sxxp2, sxxm1;
+  float sxz0, sxzm1, sxzp1, sxzm2, sxzmx, sxzpx, sxzm2x;
+  float q1[4], q2[4], q3[4], q4[4];  // vxx (x), vxz (z), vzz (z), vzx (x) stencils
+  float m_sxx, m_sxzz, m_sxzx, m_szz;
+  float vx, vz;
+  int sz, sx;
+};
+
+template <int U>
+__global__ __launch_bounds__(256) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
                                                  float* __restrict__ stf_grad, int it) {
   int tz, tx;
   tile_of(tl, tz, tx);
@@ -1174,6 +1336,7 @@ __global__ __launch_bounds__(1024) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
   const float l2m = lam + 2.f * mu;
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
@@ -1183,47 +1346,75 @@ __global__ __launch_bounds__(1024) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c
   const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
   const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s = 0; s < g.ns; ++s) {
-    const long long o = s * g.N + i;
-    const float* szz = a.szz + o;
-    const float* sxx = a.sxx + o;
-    const float* sxz = a.sxz + o;
-    if (it >= 0 && z == src.zx[2 * s] && x == src.zx[2 * s + 1])
-      stf_grad[(size_t)s * g.nSteps + it] = -(szz[0] + 3.f * sxx[0]) * dt;
-    // update vx
-    const float dszz_dx = (-kC1 * (szz[nz] - szz[0]) + kC2 * (szz[2 * nz] - szz[-nz])) * rdx;
-    const float dsxx_dx = (-kC1 * (sxx[nz] - sxx[0]) + kC2 * (sxx[2 * nz] - sxx[-nz])) * rdx;
-    const float dsxz_dz = (-kC1 * (sxz[0] - sxz[-1]) + kC2 * (sxz[1] - sxz[-2])) * rdz;
-    float vx = a.vx[o] + (lam * dszz_dx * rkx * dt + l2m * dsxx_dx * rkx * dt +
-                          amu * rkzh * dsxz_dz * dt);
-    if (ax != 0.f) {
-      const float* q = p.vxx + o;
-      vx += ax * ((-kC1 * (q[nz] - q[0]) + kC2 * (q[2 * nz] - q[-nz])) * rdx);
+  for (int s0 = 0; s0 < g.ns; s0 += U) {
+    AdjVelIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const float* szz = a.szz + so;
+      const float* sxx = a.sxx + so;
+      const float* sxz = a.sxz + so;
+      AdjVelIn& q = r[u];
+      q.szz0 = ldo(szz, bo), q.szzpx = ldo(szz + nz, bo), q.szzp2x = ldo(szz + 2 * nz, bo), q.szzmx = ldo(szz - nz, bo);
+      q.szzp1 = ldo(szz + 1, bo), q.szzp2 = ldo(szz + 2, bo), q.szzm1 = ldo(szz - 1, bo);
+      q.sxx0 = ldo(sxx, bo), q.sxxpx = ldo(sxx + nz, bo), q.sxxp2x = ldo(sxx + 2 * nz, bo), q.sxxmx = ldo(sxx - nz, bo);
+      q.sxxp1 = ldo(sxx + 1, bo), q.sxxp2 = ldo(sxx + 2, bo), q.sxxm1 = ldo(sxx - 1, bo);
+      q.sxz0 = ldo(sxz, bo), q.sxzm1 = ldo(sxz - 1, bo), q.sxzp1 = ldo(sxz + 1, bo), q.sxzm2 = ldo(sxz - 2, bo);
+      q.sxzmx = ldo(sxz - nz, bo), q.sxzpx = ldo(sxz + nz, bo), q.sxzm2x = ldo(sxz - 2 * nz, bo);
+      if (ax != 0.f) {
+        const float* v = p.vxx + so;
+        q.q1[0] = ldo(v + nz, bo), q.q1[1] = ldo(v, bo), q.q1[2] = ldo(v + 2 * nz, bo), q.q1[3] = ldo(v - nz, bo);
+      }
+      if (azh != 0.f) {
+        const float* v = p.vxz + so;
+        q.q2[0] = ldo(v, bo), q.q2[1] = ldo(v - 1, bo), q.q2[2] = ldo(v + 1, bo), q.q2[3] = ldo(v - 2, bo);
+      }
+      if (az != 0.f) {
+        const float* v = p.vzz + so;
+        q.q3[0] = ldo(v + 1, bo), q.q3[1] = ldo(v, bo), q.q3[2] = ldo(v + 2, bo), q.q3[3] = ldo(v - 1, bo);
+      }
+      if (axh != 0.f) {
+        const float* v = p.vzx + so;
+        q.q4[0] = ldo(v, bo), q.q4[1] = ldo(v - nz, bo), q.q4[2] = ldo(v + nz, bo), q.q4[3] = ldo(v - 2 * nz, bo);
+      }
+      if (px) q.m_sxx = ldo(p.sxx + so, bo), q.m_sxzx = ldo(p.sxzx + so, bo);
+      if (pz) q.m_sxzz = ldo(p.sxzz + so, bo), q.m_szz = ldo(p.szz + so, bo);
+      q.vx = ldo(a.vx + so, bo), q.vz = ldo(a.vz + so, bo);
+      q.sz = src.zx[2 * s], q.sx = src.zx[2 * s + 1];
     }
-    if (azh != 0.f) {
-      const float* q = p.vxz + o;
-      vx += azh * ((-kC1 * (q[0] - q[-1]) + kC2 * (q[1] - q[-2])) * rdz);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const AdjVelIn& q = r[u];
+      if (it >= 0 && z == q.sz && x == q.sx)
+        stf_grad[(size_t)s * g.nSteps + it] = -(q.szz0 + 3.f * q.sxx0) * dt;
+      // update vx
+      const float dszz_dx = (-kC1 * (q.szzpx - q.szz0) + kC2 * (q.szzp2x - q.szzmx)) * rdx;
+      const float dsxx_dx = (-kC1 * (q.sxxpx - q.sxx0) + kC2 * (q.sxxp2x - q.sxxmx)) * rdx;
+      const float dsxz_dz = (-kC1 * (q.sxz0 - q.sxzm1) + kC2 * (q.sxzp1 - q.sxzm2)) * rdz;
+      float vx = q.vx + (lam * dszz_dx * rkx * dt + l2m * dsxx_dx * rkx * dt +
+                         amu * rkzh * dsxz_dz * dt);
+      if (ax != 0.f) vx += ax * ((-kC1 * (q.q1[0] - q.q1[1]) + kC2 * (q.q1[2] - q.q1[3])) * rdx);
+      if (azh != 0.f) vx += azh * ((-kC1 * (q.q2[0] - q.q2[1]) + kC2 * (q.q2[2] - q.q2[3])) * rdz);
+      sto(a.vx + so, bo, vx);
+      if (px) sto(p.sxx + so, bo, bxh * q.m_sxx + bb * vx * dt);
+      if (pz) sto(p.sxzz + so, bo, bz * q.m_sxzz + bb * vx * dt);
+      // update vz
+      const float dszz_dz = (-kC1 * (q.szzp1 - q.szz0) + kC2 * (q.szzp2 - q.szzm1)) * rdz;
+      const float dsxx_dz = (-kC1 * (q.sxxp1 - q.sxx0) + kC2 * (q.sxxp2 - q.sxxm1)) * rdz;
+      const float dsxz_dx = (-kC1 * (q.sxz0 - q.sxzmx) + kC2 * (q.sxzpx - q.sxzm2x)) * rdx;
+      float vz = q.vz + (l2m * dszz_dz * rkz * dt + lam * dsxx_dz * rkz * dt +
+                         amu * rkxh * dsxz_dx * dt);
+      if (az != 0.f) vz += az * ((-kC1 * (q.q3[0] - q.q3[1]) + kC2 * (q.q3[2] - q.q3[3])) * rdz);
+      if (axh != 0.f) vz += axh * ((-kC1 * (q.q4[0] - q.q4[1]) + kC2 * (q.q4[2] - q.q4[3])) * rdx);
+      sto(a.vz + so, bo, vz);
+      if (px) sto(p.sxzx + so, bo, bx * q.m_sxzx + ba * vz * dt);
+      if (pz) sto(p.szz + so, bo, bzh * q.m_szz + ba * vz * dt);
     }
-    a.vx[o] = vx;
-    if (px) p.sxx[o] = bxh * p.sxx[o] + bb * vx * dt;
-    if (pz) p.sxzz[o] = bz * p.sxzz[o] + bb * vx * dt;
-    // update vz
-    const float dszz_dz = (-kC1 * (szz[1] - szz[0]) + kC2 * (szz[2] - szz[-1])) * rdz;
-    const float dsxx_dz = (-kC1 * (sxx[1] - sxx[0]) + kC2 * (sxx[2] - sxx[-1])) * rdz;
-    const float dsxz_dx = (-kC1 * (sxz[0] - sxz[-nz]) + kC2 * (sxz[nz] - sxz[-2 * nz])) * rdx;
-    float vz = a.vz[o] + (l2m * dszz_dz * rkz * dt + lam * dsxx_dz * rkz * dt +
-                          amu * rkxh * dsxz_dx * dt);
-    if (az != 0.f) {
-      const float* q = p.vzz + o;
-      vz += az * ((-kC1 * (q[1] - q[0]) + kC2 * (q[2] - q[-1])) * rdz);
-    }
-    if (axh != 0.f) {
-      const float* q = p.vzx + o;
-      vz += axh * ((-kC1 * (q[0] - q[-nz]) + kC2 * (q[nz] - q[-2 * nz])) * rdx);
-    }
-    a.vz[o] = vz;
-    if (px) p.sxzx[o] = bx * p.sxzx[o] + ba * vz * dt;
-    if (pz) p.szz[o] = bzh * p.szz[o] + ba * vz * dt;
   }
 }
 
@@ -1243,7 +1434,16 @@ __global__ void k_res_inject(Geo g, float* __restrict__ szz, float* __restrict__
 // without PML gates in the reference; they are read only through a*dpsi terms,
 // i.e. where a != 0 (the C-PML layers) plus the 2-cell reach of the stencil,
 // so they are kept in that band (`zlive`/`xlive`) and nowhere else.
-__global__ __launch_bounds__(1024) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p) {
+struct AdjStressIn {
+  float vz0, vzpx, vzp2x, vzmx, vzm1, vzp1, vzm2;
+  float vx0, vxp1, vxp2, vxm1, vxmx, vxpx, vxm2x;
+  float q1[4], q2[4], q3[4], q4[4];  // sxzx (x), sxzz (z), sxx (x), szz (z) stencils
+  float m_vzx, m_vxz, m_vxx, m_vzz;
+  float sxz, sxx, szz;
+};
+
+template <int U>
+__global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p) {
   int tz, tx;
   tile_of(tl, tz, tx);
   const int z = tz * kBZ + threadIdx.x;
@@ -1251,6 +1451,7 @@ __global__ __launch_bounds__(1024) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pm
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
   const float l2m = lam + 2.f * mu;
   const bool zlive = z <= g.nPml + 1 || z >= g.nzA - g.nPml - 2;
@@ -1260,41 +1461,65 @@ __global__ __launch_bounds__(1024) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pm
   const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
   const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s = 0; s < g.ns; ++s) {
-    const long long o = s * g.N + i;
-    const float* vz = a.vz + o;
-    const float* vx = a.vx + o;
-    const float vz0 = vz[0], vx0 = vx[0];
-    const float dvz_dx = (-kC1 * (vz[nz] - vz0) + kC2 * (vz[2 * nz] - vz[-nz])) * rdx;
-    const float dvx_dz = (-kC1 * (vx[1] - vx0) + kC2 * (vx[2] - vx[-1])) * rdz;
-    float sxz = a.sxz[o] + (dvz_dx * rkx * ba * dt + dvx_dz * rkz * bb * dt);
-    if (ax != 0.f) {
-      const float* q = p.sxzx + o;
-      sxz += ax * ((-kC1 * (q[nz] - q[0]) + kC2 * (q[2 * nz] - q[-nz])) * rdx);
+  for (int s0 = 0; s0 < g.ns; s0 += U) {
+    AdjStressIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const float* vz = a.vz + so;
+      const float* vx = a.vx + so;
+      AdjStressIn& q = r[u];
+      q.vz0 = ldo(vz, bo), q.vzpx = ldo(vz + nz, bo), q.vzp2x = ldo(vz + 2 * nz, bo), q.vzmx = ldo(vz - nz, bo);
+      q.vzm1 = ldo(vz - 1, bo), q.vzp1 = ldo(vz + 1, bo), q.vzm2 = ldo(vz - 2, bo);
+      q.vx0 = ldo(vx, bo), q.vxp1 = ldo(vx + 1, bo), q.vxp2 = ldo(vx + 2, bo), q.vxm1 = ldo(vx - 1, bo);
+      q.vxmx = ldo(vx - nz, bo), q.vxpx = ldo(vx + nz, bo), q.vxm2x = ldo(vx - 2 * nz, bo);
+      if (ax != 0.f) {
+        const float* v = p.sxzx + so;
+        q.q1[0] = ldo(v + nz, bo), q.q1[1] = ldo(v, bo), q.q1[2] = ldo(v + 2 * nz, bo), q.q1[3] = ldo(v - nz, bo);
+      }
+      if (az != 0.f) {
+        const float* v = p.sxzz + so;
+        q.q2[0] = ldo(v + 1, bo), q.q2[1] = ldo(v, bo), q.q2[2] = ldo(v + 2, bo), q.q2[3] = ldo(v - 1, bo);
+      }
+      if (axh != 0.f) {
+        const float* v = p.sxx + so;
+        q.q3[0] = ldo(v, bo), q.q3[1] = ldo(v - nz, bo), q.q3[2] = ldo(v + nz, bo), q.q3[3] = ldo(v - 2 * nz, bo);
+      }
+      if (azh != 0.f) {
+        const float* v = p.szz + so;
+        q.q4[0] = ldo(v, bo), q.q4[1] = ldo(v - 1, bo), q.q4[2] = ldo(v + 1, bo), q.q4[3] = ldo(v - 2, bo);
+      }
+      if (xlive) q.m_vzx = ldo(p.vzx + so, bo), q.m_vxx = ldo(p.vxx + so, bo);
+      if (zlive) q.m_vxz = ldo(p.vxz + so, bo), q.m_vzz = ldo(p.vzz + so, bo);
+      q.sxz = ldo(a.sxz + so, bo), q.sxx = ldo(a.sxx + so, bo), q.szz = ldo(a.szz + so, bo);
     }
-    if (az != 0.f) {
-      const float* q = p.sxzz + o;
-      sxz += az * ((-kC1 * (q[1] - q[0]) + kC2 * (q[2] - q[-1])) * rdz);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= g.ns) break;
+      const long long so = s * g.N;
+      const AdjStressIn& q = r[u];
+      const float dvz_dx = (-kC1 * (q.vzpx - q.vz0) + kC2 * (q.vzp2x - q.vzmx)) * rdx;
+      const float dvx_dz = (-kC1 * (q.vxp1 - q.vx0) + kC2 * (q.vxp2 - q.vxm1)) * rdz;
+      float sxz = q.sxz + (dvz_dx * rkx * ba * dt + dvx_dz * rkz * bb * dt);
+      if (ax != 0.f) sxz += ax * ((-kC1 * (q.q1[0] - q.q1[1]) + kC2 * (q.q1[2] - q.q1[3])) * rdx);
+      if (az != 0.f) sxz += az * ((-kC1 * (q.q2[0] - q.q2[1]) + kC2 * (q.q2[2] - q.q2[3])) * rdz);
+      sto(a.sxz + so, bo, sxz);
+      if (xlive) sto(p.vzx + so, bo, bxh * q.m_vzx + sxz * amu * dt);
+      if (zlive) sto(p.vxz + so, bo, bzh * q.m_vxz + sxz * amu * dt);
+      const float dvx_dx = (-kC1 * (q.vx0 - q.vxmx) + kC2 * (q.vxpx - q.vxm2x)) * rdx;
+      const float dvz_dz = (-kC1 * (q.vz0 - q.vzm1) + kC2 * (q.vzp1 - q.vzm2)) * rdz;
+      float sxx = q.sxx + bb * dvx_dx * rkxh * dt;
+      if (axh != 0.f) sxx += axh * ((-kC1 * (q.q3[0] - q.q3[1]) + kC2 * (q.q3[2] - q.q3[3])) * rdx);
+      float szz = q.szz + ba * dvz_dz * rkzh * dt;
+      if (azh != 0.f) szz += azh * ((-kC1 * (q.q4[0] - q.q4[1]) + kC2 * (q.q4[2] - q.q4[3])) * rdz);
+      sto(a.sxx + so, bo, sxx);
+      sto(a.szz + so, bo, szz);
+      if (xlive) sto(p.vxx + so, bo, bx * q.m_vxx + lam * szz * dt + l2m * sxx * dt);
+      if (zlive) sto(p.vzz + so, bo, bz * q.m_vzz + l2m * szz * dt + lam * sxx * dt);
     }
-    a.sxz[o] = sxz;
-    if (xlive) p.vzx[o] = bxh * p.vzx[o] + sxz * amu * dt;
-    if (zlive) p.vxz[o] = bzh * p.vxz[o] + sxz * amu * dt;
-    const float dvx_dx = (-kC1 * (vx0 - vx[-nz]) + kC2 * (vx[nz] - vx[-2 * nz])) * rdx;
-    const float dvz_dz = (-kC1 * (vz0 - vz[-1]) + kC2 * (vz[1] - vz[-2])) * rdz;
-    float sxx = a.sxx[o] + bb * dvx_dx * rkxh * dt;
-    if (axh != 0.f) {
-      const float* q = p.sxx + o;
-      sxx += axh * ((-kC1 * (q[0] - q[-nz]) + kC2 * (q[nz] - q[-2 * nz])) * rdx);
-    }
-    float szz = a.szz[o] + ba * dvz_dz * rkzh * dt;
-    if (azh != 0.f) {
-      const float* q = p.szz + o;
-      szz += azh * ((-kC1 * (q[0] - q[-1]) + kC2 * (q[1] - q[-2])) * rdz);
-    }
-    a.sxx[o] = sxx;
-    a.szz[o] = szz;
-    if (xlive) p.vxx[o] = bx * p.vxx[o] + lam * szz * dt + l2m * sxx * dt;
-    if (zlive) p.vzz[o] = bz * p.vzz[o] + l2m * szz * dt + lam * sxx * dt;
   }
 }
 
@@ -1431,6 +1656,9 @@ struct DeviceState {
   int seg_x = 32;    // x extent of a fused segment
   int rev_group = 8; // shots per workgroup of the fused reverse step
   bool fused_rev = true;  // fused reverse step (needs its LDS to fit)
+  int unroll = 1;         // shots whose loads a split-kernel thread issues together
+  int shot_group = 4;     // shots per launch of the split/tiled kernels (0: all)
+  std::vector<int> roff;  // first receiver of each shot (+ total)
   std::vector<void*> allocs;
   long long bytes = 0;
   // model + tables
@@ -1545,6 +1773,28 @@ struct DeviceState {
     const size_t F = (size_t)g.ns * g.N;
     return Mem{b, b + F, b + 2 * F, b + 3 * F, b + 4 * F, b + 5 * F, b + 6 * F, b + 7 * F};
   }
+  // Views of shots [s0, s0 + n) of the launch: every per-shot array is
+  // offset to shot s0 and the geometry counts n shots.
+  Geo group_geo(int n) const {
+    Geo q = g;
+    q.ns = n;
+    return q;
+  }
+  Wave wave(float* base, int s0) const {
+    Wave w = wave(base);
+    const size_t o = (size_t)s0 * g.N;
+    return Wave{w.vz + o, w.vx + o, w.szz + o, w.sxx + o, w.sxz + o};
+  }
+  Mem memv(float* b, int s0) const {
+    Mem p = memv(b);
+    const size_t o = (size_t)s0 * g.N;
+    return Mem{p.vzz + o, p.vzx + o, p.vxz + o, p.vxx + o,
+               p.szz + o, p.sxx + o, p.sxzz + o, p.sxzx + o};
+  }
+  Src src_view(int s0) const { return Src{src + 2 * s0, inj + (size_t)s0 * g.nSteps}; }
+  float* ring_view(int s0) const {
+    return ring ? ring + (size_t)s0 * g.nSteps * 5 * (size_t)g.ringlen : nullptr;
+  }
 };
 
 Plan::Plan() = default;
@@ -1562,12 +1812,17 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d_.reset(new DeviceState());
   DeviceState& d = *d_;
   d.dev = gpu;
-  d.tile_x = std::max(1, std::min(16, env_int("FWIFLOW_TILE_X", 4)));
+  d.tile_x = std::max(1, std::min(4, env_int("FWIFLOW_TILE_X", 4)));  // <= 256 threads
   d.xcd = env_int("FWIFLOW_XCD_REMAP", 1) != 0;
   d.fused = env_int("FWIFLOW_FUSED", 0) != 0;
   d.tiled = env_int("FWIFLOW_TILED", 0) != 0 && !d.fused;
   d.seg_x = std::max(4, std::min(256, env_int("FWIFLOW_SEG_X", 32)));
   d.rev_group = std::max(1, std::min(64, env_int("FWIFLOW_REV_GROUP", 8)));
+  {
+    const int u = env_int("FWIFLOW_SHOT_UNROLL", 1);
+    d.unroll = u >= 4 ? 4 : (u >= 2 ? 2 : 1);
+  }
+  d.shot_group = std::max(0, env_int("FWIFLOW_SHOT_GROUP", 4));
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
   Geo& g = d.g;
@@ -1605,8 +1860,10 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   HIP_TRY(hipMemcpy(d.win2, w2.data(), w2.size() * sizeof(float), hipMemcpyHostToDevice));
   std::vector<int> src(2 * g.ns), rec(3 * (size_t)d.R);
   size_t r = 0;
+  d.roff.assign(g.ns + 1, 0);
   for (int s = 0; s < g.ns; s++) {
     const Shot& sh = survey.shots[s];
+    d.roff[s] = (int)r;
     src[2 * s] = sh.z_src;
     src[2 * s + 1] = sh.x_src;
     for (size_t k = 0; k < sh.z_rec.size(); k++, r++) {
@@ -1615,6 +1872,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
       rec[3 * r + 2] = s;
     }
   }
+  d.roff[g.ns] = (int)r;
   TRY(d.alloc(&d.src, src.size()));
   HIP_TRY(hipMemcpy(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
   TRY(d.alloc(&d.rec, rec.size()));
@@ -1843,21 +2101,24 @@ Status Plan::run(int calc_id) {
   const size_t ldsf = (size_t)10 * NTf * sizeof(float);
 
   // ---------------------------------------------------------- forward
-  for (int it = 0; it <= g.nSteps - 2; it++) {
-    const bool smp = (it % sample_every) == 0;
-    float* ring = withAdj ? d.ring : nullptr;
-    if (d.tiled) {
-      const Wave wa = d.wave(fa), wb = d.wave(fb);
-      const Mem pa = d.memv(ma), pb = d.memv(mb);
-      d.launch("fwd_tile", smp, [&] {
-        hipLaunchKernelGGL(k_fwd_tile, dim3(tT.n), dim3(1024), 0, st, tT, g, m, c, wa, wb, pa, pb,
-                           src, it, ring);
-      });
-      hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, wb.szz, wb.sxx, d.rec,
-                         d.R, d.syn + (size_t)(it + 1) * d.R);
-      std::swap(fa, fb);
-      std::swap(ma, mb);
-    } else if (d.fused) {
+  // The split and tiled paths take the shots in groups of `shot_group`: a
+  // group runs every time step before the next group starts, so the state of
+  // one group stays resident in the last-level cache from step to step.
+  const int G = (d.fused || d.shot_group <= 0) ? g.ns : std::min(d.shot_group, g.ns);
+  const int U = d.unroll;
+#define LAUNCH_U(K, grid, block, ...)                                        \
+  do {                                                                       \
+    if (U == 4)                                                              \
+      hipLaunchKernelGGL(K<4>, grid, block, 0, st, __VA_ARGS__);             \
+    else if (U == 2)                                                         \
+      hipLaunchKernelGGL(K<2>, grid, block, 0, st, __VA_ARGS__);             \
+    else                                                                     \
+      hipLaunchKernelGGL(K<1>, grid, block, 0, st, __VA_ARGS__);             \
+  } while (0)
+  if (d.fused) {
+    for (int it = 0; it <= g.nSteps - 2; it++) {
+      const bool smp = (it % sample_every) == 0;
+      float* ring = withAdj ? d.ring : nullptr;
       const Wave wa = d.wave(fa), wb = d.wave(fb);
       const Mem pa = d.memv(ma), pb = d.memv(mb);
       d.launch("fwd_step", smp, [&] {
@@ -1868,22 +2129,49 @@ Status Plan::run(int calc_id) {
                          d.R, d.syn + (size_t)(it + 1) * d.R);
       std::swap(fa, fb);
       std::swap(ma, mb);
-    } else {
-      const Wave w = d.wave(fa);
-      const Mem p = d.memv(ma);
-      d.launch("stress_fwd", smp, [&] {
-        hipLaunchKernelGGL(k_stress_fwd, grdU, blk, 0, st, tU, g, m, c, w, p, src, it, ring);
-      });
-      d.launch("vel_fwd", smp, [&] {
-        hipLaunchKernelGGL(k_vel_fwd, grdU, blk, 0, st, tU, g, m, c, w, p);
-      });
-      hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, w.szz, w.sxx, d.rec,
-                         d.R, d.syn + (size_t)(it + 1) * d.R);
+    }
+  } else {
+    float* fa0 = fa;
+    float* ma0 = ma;
+    for (int s0 = 0; s0 < g.ns; s0 += G) {
+      const int n = std::min(G, g.ns - s0);
+      const Geo gg = d.group_geo(n);
+      const Src sg = d.src_view(s0);
+      float* ring = withAdj ? d.ring_view(s0) : nullptr;
+      const int r0 = d.roff[s0], Rg = d.roff[s0 + n] - r0;
+      const dim3 rgrid((Rg + 255) / 256);
+      fa = fa0, fb = d.fwd2, ma = ma0, mb = d.mem2;
+      for (int it = 0; it <= g.nSteps - 2; it++) {
+        const bool smp = (it % sample_every) == 0;
+        float* out = fa;  // buffer that holds step it+1 after this step
+        if (d.tiled) {
+          const Wave wa = d.wave(fa, s0), wb = d.wave(fb, s0);
+          const Mem pa = d.memv(ma, s0), pb = d.memv(mb, s0);
+          d.launch("fwd_tile", smp, [&] {
+            hipLaunchKernelGGL(k_fwd_tile, dim3(tT.n), dim3(1024), 0, st, tT, gg, m, c, wa, wb,
+                               pa, pb, sg, it, ring);
+          });
+          out = fb;
+          std::swap(fa, fb);
+          std::swap(ma, mb);
+        } else {
+          const Wave w = d.wave(fa, s0);
+          const Mem p = d.memv(ma, s0);
+          d.launch("stress_fwd", smp, [&] {
+            LAUNCH_U(k_stress_fwd, grdU, blk, tU, gg, m, c, w, p, sg, it, ring);
+          });
+          d.launch("vel_fwd", smp, [&] { LAUNCH_U(k_vel_fwd, grdU, blk, tU, gg, m, c, w, p); });
+        }
+        if (Rg > 0) {
+          const Wave wo = d.wave(out);
+          hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, st, g, wo.szz, wo.sxx, d.rec + 3 * r0,
+                             Rg, d.syn + (size_t)(it + 1) * d.R + r0);
+        }
+      }
     }
   }
   HIP_TRY(hipGetLastError());
   const Wave w = d.wave(fa);  // final forward state
-  const Mem p = d.memv(ma);
 
   if (if_res) {
     hipLaunchKernelGGL(k_residual, dim3(d.npart), dim3(256), 0, st, g.nSteps, d.R, d.win2, d.obs,
@@ -1945,11 +2233,11 @@ Status Plan::run(int calc_id) {
           std::swap(ra, rb);
         } else {
           d.launch("rev_vel", smp, [&] {
-            hipLaunchKernelGGL(k_rev_vel, grdR, blk, 0, st, tR, g, m, w, a, d.accp + 3 * N,
+            hipLaunchKernelGGL(k_rev_vel<1>, grdR, blk, 0, st, tR, g, m, w, a, d.accp + 3 * N,
                                d.accp + 4 * N, d.ring, it);
           });
           d.launch("rev_stress", smp, [&] {
-            hipLaunchKernelGGL(k_rev_stress, grdR, blk, 0, st, tR, g, m, w, a, src, d.accp,
+            hipLaunchKernelGGL(k_rev_stress<1>, grdR, blk, 0, st, tR, g, m, w, a, src, d.accp,
                                d.accp + N, d.accp + 2 * N, d.ring, it);
           });
         }
@@ -1960,32 +2248,47 @@ Status Plan::run(int calc_id) {
                          d.accp, n5, qr.ngrp);
       acc = d.accp;
     } else {
-      const Wave a = d.wave(d.adj);
-      // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
-      hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, tU, g, m, c, a, p, src, d.stf_grad, -1);
-      hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx, d.rec,
-                         d.R, d.res + (size_t)(g.nSteps - 1) * d.R);
-      hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, tU, g, m, c, a, p);
-      for (int it = g.nSteps - 2; it >= 0; it--) {
-        const bool smp = (it % sample_every) == 0;
-        d.launch("rev_vel", smp, [&] {
-          hipLaunchKernelGGL(k_rev_vel, grdR, blk, 0, st, tR, g, m, w, a, acc + 3 * N, acc + 4 * N,
-                             d.ring, it);
-        });
-        d.launch("rev_stress", smp, [&] {
-          hipLaunchKernelGGL(k_rev_stress, grdR, blk, 0, st, tR, g, m, w, a, src, acc, acc + N,
-                             acc + 2 * N, d.ring, it);
-        });
-        d.launch("adj_vel", smp, [&] {
-          hipLaunchKernelGGL(k_adj_vel, grdU, blk, 0, st, tU, g, m, c, a, p, src, d.stf_grad, it);
-        });
-        hipLaunchKernelGGL(k_res_inject, dim3(recBlocks), dim3(256), 0, st, g, a.szz, a.sxx,
-                           d.rec, d.R, d.res + (size_t)it * d.R);
-        d.launch("adj_stress", smp, [&] {
-          hipLaunchKernelGGL(k_adj_stress, grdU, blk, 0, st, tU, g, m, c, a, p);
-        });
+      const Wave a_all = d.wave(d.adj);
+      for (int s0 = 0; s0 < g.ns; s0 += G) {
+        const int n = std::min(G, g.ns - s0);
+        const Geo gg = d.group_geo(n);
+        const Src sg = d.src_view(s0);
+        const float* ring = d.ring_view(s0);
+        const int r0 = d.roff[s0], Rg = d.roff[s0 + n] - r0;
+        const dim3 rgrid((Rg + 255) / 256);
+        const Wave wg = d.wave(fa, s0);  // final forward state of the group
+        const Wave a = d.wave(d.adj, s0);
+        const Mem pg = d.memv(ma, s0);   // adjoint memory variables
+        float* sgrad = d.stf_grad + (size_t)s0 * g.nSteps;
+        auto inject = [&](int t) {
+          if (Rg > 0)
+            hipLaunchKernelGGL(k_res_inject, rgrid, dim3(256), 0, st, g, a_all.szz, a_all.sxx,
+                               d.rec + 3 * r0, Rg, d.res + (size_t)t * d.R + r0);
+        };
+        // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
+        LAUNCH_U(k_adj_vel, grdU, blk, tU, gg, m, c, a, pg, sg, sgrad, -1);
+        inject(g.nSteps - 1);
+        LAUNCH_U(k_adj_stress, grdU, blk, tU, gg, m, c, a, pg);
+        for (int it = g.nSteps - 2; it >= 0; it--) {
+          const bool smp = (it % sample_every) == 0;
+          d.launch("rev_vel", smp, [&] {
+            LAUNCH_U(k_rev_vel, grdR, blk, tR, gg, m, wg, a, acc + 3 * N, acc + 4 * N, ring, it);
+          });
+          d.launch("rev_stress", smp, [&] {
+            LAUNCH_U(k_rev_stress, grdR, blk, tR, gg, m, wg, a, sg, acc, acc + N, acc + 2 * N,
+                     ring, it);
+          });
+          d.launch("adj_vel", smp, [&] {
+            LAUNCH_U(k_adj_vel, grdU, blk, tU, gg, m, c, a, pg, sg, sgrad, it);
+          });
+          inject(it);
+          d.launch("adj_stress", smp, [&] {
+            LAUNCH_U(k_adj_stress, grdU, blk, tU, gg, m, c, a, pg);
+          });
+        }
       }
     }
+#undef LAUNCH_U
     const Tiling tA = make_tiling(g.nz, g.nx, d.tile_x, d.xcd);
     hipLaunchKernelGGL(k_finalize, dim3(tA.n), blk, 0, st, tA, g, m, acc, d.grad);
     HIP_TRY(hipGetLastError());

@@ -198,5 +198,16 @@ int fwiflow_plan_device_bytes(fwiflow_plan* plan, long long* bytes) {
   *bytes = plan->impl.device_bytes();
   return report(Status::Ok());
 }
+int fwiflow_plan_set_illumination(fwiflow_plan* plan, int enable) {
+  PLAN_CALL(plan->impl.set_illumination(enable != 0));
+}
+int fwiflow_plan_illumination(fwiflow_plan* plan, double* illum) {
+  PLAN_CALL(plan->impl.illumination(illum));
+}
+int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* shots) {
+  if (!plan || !shots) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
+  *shots = plan->impl.shots_per_launch();
+  return report(Status::Ok());
+}
 
 }  // extern "C"

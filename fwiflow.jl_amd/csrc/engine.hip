@@ -112,23 +112,32 @@ __device__ inline float* ring_ptr(float* ring, const Geo& g, int it, int comp, i
 // round-robin by linear id; with `xcd` set, the ids are permuted so that each
 // XCD walks one contiguous band of tiles (z fastest, then x), and tiles that
 // share x-halo columns share that XCD's L2.
+// A launch may also split its shots into `chunks` ranges of `spt` shots, one
+// range per copy of the tile grid (workgroup id = chunk * n + tile before
+// the XCD permutation); tile_of returns the chunk.
 struct Tiling {
   int gz, gx, n, xcd;
+  int chunks = 1, spt = 1 << 30;
 };
 
-__device__ inline void tile_of(const Tiling& t, int& bz, int& bx) {
-  int b = blockIdx.x;
+__device__ inline int tile_of(const Tiling& t, int b, int& bz, int& bx) {
   if (t.xcd) {
-    const int per = t.n / 8;
+    const int per = t.n * t.chunks / 8;
     if (b < per * 8) b = (b % 8) * per + b / 8;
   }
+  const int c = b / t.n;
+  b -= c * t.n;
   bz = b % t.gz;
   bx = b / t.gz;
+  return c;
+}
+__device__ inline int tile_of(const Tiling& t, int& bz, int& bx) {
+  return tile_of(t, (int)blockIdx.x, bz, bx);
 }
 
-// Loads and stores at a byte offset from a wave-uniform base: the base stays
-// in scalar registers (per shot, per stencil neighbour) and each lane carries
-// one 32-bit offset, so a shot's loads need no 64-bit address arithmetic.
+// Loads and stores at a byte offset from a wave-uniform base: the per-shot,
+// per-neighbour base is uniform across the wave and the per-lane part of
+// every address is one 32-bit offset shared by all fields and shots.
 __device__ __forceinline__ float ldo(const float* base, unsigned bo) {
   return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + bo);
 }
@@ -159,7 +168,8 @@ template <int U>
 __global__ __launch_bounds__(256) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p, Src src,
                                                     int it, float* __restrict__ ring) {
   int tz, tx;
-  tile_of(tl, tz, tx);
+  const int chunk = tile_of(tl, tz, tx);
+  const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
   const int z = tz * kBZ + threadIdx.x;
   const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
@@ -182,12 +192,12 @@ __global__ __launch_bounds__(256) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pml
   }
   const int slot = ring ? ring_slot(z, x, g) : -1;
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s0 = 0; s0 < g.ns; s0 += U) {
+  for (int s0 = sb; s0 < se; s0 += U) {
     StressIn r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= g.ns) break;
+      if (s >= se) break;
       const long long so = s * g.N;
       const float* vz = w.vz + so;
       const float* vx = w.vx + so;
@@ -204,7 +214,7 @@ __global__ __launch_bounds__(256) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pml
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= g.ns) break;
+      if (s >= se) break;
       const long long so = s * g.N;
       const StressIn& q = r[u];
       float dvz_dz = (kC1 * (q.vz0 - q.vzm1) - kC2 * (q.vzp1 - q.vzm2)) * rdz;
@@ -251,6 +261,8 @@ __global__ __launch_bounds__(256) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pml
 }
 
 // el_velocity.cu:45-82.  x-PML test is `x > nx - nPml` (el_velocity.cu:56,71).
+// With `illum` set, the launch's sum over shots of vz^2 + vx^2 is added to it
+// (forward illumination; no reference counterpart).
 struct VelIn {
   float szz0, szzp1, szzp2, szzm1;
   float sxz0, sxzmx, sxzpx, sxzm2x, sxzm1, sxzp1, sxzm2;
@@ -260,9 +272,11 @@ struct VelIn {
 };
 
 template <int U>
-__global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p) {
+__global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p,
+                                                float* __restrict__ illum) {
   int tz, tx;
-  tile_of(tl, tz, tx);
+  const int chunk = tile_of(tl, tz, tx);
+  const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
   const int z = tz * kBZ + threadIdx.x;
   const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
@@ -283,12 +297,13 @@ __global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c,
     bxh = c.bxh[x], axh = c.axh[x], rkxh = 1.f / c.Kxh[x];
   }
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s0 = 0; s0 < g.ns; s0 += U) {
+  float il = 0.f;  // illumination of this launch's shots
+  for (int s0 = sb; s0 < se; s0 += U) {
     VelIn r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= g.ns) break;
+      if (s >= se) break;
       const long long so = s * g.N;
       const float* szz = w.szz + so;
       const float* sxx = w.sxx + so;
@@ -305,7 +320,7 @@ __global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= g.ns) break;
+      if (s >= se) break;
       const long long so = s * g.N;
       const VelIn& q = r[u];
       float dszz_dz = (kC1 * (q.szzp1 - q.szz0) - kC2 * (q.szzp2 - q.szzm1)) * rdz;
@@ -333,8 +348,10 @@ __global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c,
       vx += (dsxz_dz + dsxx_dx) * bb * dt;
       sto(w.vz + so, bo, vz);
       sto(w.vx + so, bo, vx);
+      il += vz * vz + vx * vx;
     }
   }
+  if (illum) illum[i] += il;
 }
 
 // ------------------------------------------------------------ fused forward
@@ -1143,12 +1160,12 @@ struct RevVelIn {
 };
 
 template <int U>
-__global__ __launch_bounds__(256) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave w, Wave a,
+__device__ __forceinline__ void k_rev_vel_body(int bid, Tiling tl, Geo g, Mdl m, Wave w, Wave a,
                                                  float* __restrict__ acc_a,
                                                  float* __restrict__ acc_b,
                                                  const float* __restrict__ ring, int it) {
   int tz, tx;
-  tile_of(tl, tz, tx);
+  tile_of(tl, bid, tz, tx);
   const int z = tz * kBZ + threadIdx.x + (g.nPml - 2);
   const int x = tx * (int)blockDim.y + threadIdx.y + (g.nPml - 2);
   if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
@@ -1216,6 +1233,14 @@ __global__ __launch_bounds__(256) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave w
   }
 }
 
+template <int U>
+__global__ __launch_bounds__(256) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave w, Wave a,
+                                                 float* __restrict__ acc_a,
+                                                 float* __restrict__ acc_b,
+                                                 const float* __restrict__ ring, int it) {
+  k_rev_vel_body<U>(blockIdx.x, tl, g, m, w, a, acc_a, acc_b, ring, it);
+}
+
 // el_stress.cu:90-131 (isFor = false) fused with add_source(-) and the ring
 // restore of sigma.  acc_l += (szz_adj + sxx_adj)(dvz/dz + dvx/dx),
 // acc_m += szz_adj dvz/dz + sxx_adj dvx/dx, acc_s += sxz_adj (dvx/dz + dvz/dx).
@@ -1228,13 +1253,13 @@ struct RevStressIn {
 };
 
 template <int U>
-__global__ __launch_bounds__(256) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wave w, Wave a, Src src,
+__device__ __forceinline__ void k_rev_stress_body(int bid, Tiling tl, Geo g, Mdl m, Wave w, Wave a, Src src,
                                                     float* __restrict__ acc_l,
                                                     float* __restrict__ acc_m,
                                                     float* __restrict__ acc_s,
                                                     const float* __restrict__ ring, int it) {
   int tz, tx;
-  tile_of(tl, tz, tx);
+  tile_of(tl, bid, tz, tx);
   const int z = tz * kBZ + threadIdx.x + (g.nPml - 2);
   const int x = tx * (int)blockDim.y + threadIdx.y + (g.nPml - 2);
   if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
@@ -1312,6 +1337,15 @@ __global__ __launch_bounds__(256) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wav
   }
 }
 
+template <int U>
+__global__ __launch_bounds__(256) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wave w, Wave a, Src src,
+                                                    float* __restrict__ acc_l,
+                                                    float* __restrict__ acc_m,
+                                                    float* __restrict__ acc_s,
+                                                    const float* __restrict__ ring, int it) {
+  k_rev_stress_body<U>(blockIdx.x, tl, g, m, w, a, src, acc_l, acc_m, acc_s, ring, it);
+}
+
 // el_velocity_adj.cu:22-108, plus source_grad (utilities.cu:582-593) for step
 // `it` (the adjoint stress it reads is not touched by this kernel).  The
 // C-PML memory terms a*dpsi are evaluated only where a != 0; elsewhere they
@@ -1327,10 +1361,11 @@ struct AdjVelIn {
 };
 
 template <int U>
-__global__ __launch_bounds__(256) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
+__device__ __forceinline__ void k_adj_vel_body(int bid, Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
                                                  float* __restrict__ stf_grad, int it) {
   int tz, tx;
-  tile_of(tl, tz, tx);
+  const int chunk = tile_of(tl, bid, tz, tx);
+  const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
   const int z = tz * kBZ + threadIdx.x;
   const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
@@ -1346,12 +1381,12 @@ __global__ __launch_bounds__(256) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c,
   const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
   const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s0 = 0; s0 < g.ns; s0 += U) {
+  for (int s0 = sb; s0 < se; s0 += U) {
     AdjVelIn r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= g.ns) break;
+      if (s >= se) break;
       const long long so = s * g.N;
       const float* szz = a.szz + so;
       const float* sxx = a.sxx + so;
@@ -1387,7 +1422,7 @@ __global__ __launch_bounds__(256) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= g.ns) break;
+      if (s >= se) break;
       const long long so = s * g.N;
       const AdjVelIn& q = r[u];
       if (it >= 0 && z == q.sz && x == q.sx)
@@ -1418,6 +1453,12 @@ __global__ __launch_bounds__(256) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c,
   }
 }
 
+template <int U>
+__global__ __launch_bounds__(256) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
+                                                 float* __restrict__ stf_grad, int it) {
+  k_adj_vel_body<U>(blockIdx.x, tl, g, m, c, a, p, src, stf_grad, it);
+}
+
 // res_injection (utilities.cu:569-580): szz_adj += res, sxx_adj += 3 res.
 __global__ void k_res_inject(Geo g, float* __restrict__ szz, float* __restrict__ sxx,
                              const int* __restrict__ rec, int R, const float* __restrict__ row) {
@@ -1443,9 +1484,10 @@ struct AdjStressIn {
 };
 
 template <int U>
-__global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p) {
+__device__ __forceinline__ void k_adj_stress_body(int bid, Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p) {
   int tz, tx;
-  tile_of(tl, tz, tx);
+  const int chunk = tile_of(tl, bid, tz, tx);
+  const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
   const int z = tz * kBZ + threadIdx.x;
   const int x = tx * (int)blockDim.y + threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
@@ -1461,12 +1503,12 @@ __global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml
   const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
   const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s0 = 0; s0 < g.ns; s0 += U) {
+  for (int s0 = sb; s0 < se; s0 += U) {
     AdjStressIn r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= g.ns) break;
+      if (s >= se) break;
       const long long so = s * g.N;
       const float* vz = a.vz + so;
       const float* vx = a.vx + so;
@@ -1498,7 +1540,7 @@ __global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= g.ns) break;
+      if (s >= se) break;
       const long long so = s * g.N;
       const AdjStressIn& q = r[u];
       const float dvz_dx = (-kC1 * (q.vzpx - q.vz0) + kC2 * (q.vzp2x - q.vzmx)) * rdx;
@@ -1520,6 +1562,49 @@ __global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml
       if (xlive) sto(p.vxx + so, bo, bx * q.m_vxx + lam * szz * dt + l2m * sxx * dt);
       if (zlive) sto(p.vzz + so, bo, bz * q.m_vzz + l2m * szz * dt + lam * sxx * dt);
     }
+  }
+}
+
+template <int U>
+__global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p) {
+  k_adj_stress_body<U>(blockIdx.x, tl, g, m, c, a, p);
+}
+
+// Backward kernel pairs with disjoint reads and writes, launched as one grid
+// so they share a launch and its ramp-up and drain: workgroups [0, nA) run
+// the first kernel's tiles, the rest the second's.  nA is a multiple of 8, so
+// both parts see the same workgroup-to-XCD dealing.
+//   pair a: rev_stress(it) + adj_vel(it) (rev_stress reads the adjoint stress
+//           that adj_vel also only reads; adj_vel writes the adjoint velocity
+//           that rev_stress does not touch)
+//   pair b: adj_stress(it) + rev_vel(it-1) (rev_vel reads the adjoint velocity
+//           that adj_stress also only reads; adj_stress writes the adjoint
+//           stress that rev_vel does not touch)
+template <int U>
+__global__ __launch_bounds__(256) void k_bwd_pair_a(int nA, Tiling tR, Tiling tU, Geo g, Mdl m, Pml c,
+                                                   Wave w, Wave a, Mem p, Src src,
+                                                   float* __restrict__ acc,
+                                                   const float* __restrict__ ring,
+                                                   float* __restrict__ stf_grad, int it) {
+  const int b = blockIdx.x;
+  if (b < nA) {
+    if (b < tR.n)
+      k_rev_stress_body<U>(b, tR, g, m, w, a, src, acc, acc + g.N, acc + 2 * g.N, ring, it);
+  } else {
+    k_adj_vel_body<U>(b - nA, tU, g, m, c, a, p, src, stf_grad, it);
+  }
+}
+
+template <int U>
+__global__ __launch_bounds__(256) void k_bwd_pair_b(int nA, Tiling tU, Tiling tR, Geo g, Mdl m, Pml c,
+                                                   Wave w, Wave a, Mem p,
+                                                   float* __restrict__ acc,
+                                                   const float* __restrict__ ring, int it_rev) {
+  const int b = blockIdx.x;
+  if (b < nA) {
+    if (b < tU.n * tU.chunks) k_adj_stress_body<U>(b, tU, g, m, c, a, p);
+  } else {
+    k_rev_vel_body<U>(b - nA, tR, g, m, w, a, acc + 3 * g.N, acc + 4 * g.N, ring, it_rev);
   }
 }
 
@@ -1658,6 +1743,11 @@ struct DeviceState {
   bool fused_rev = true;  // fused reverse step (needs its LDS to fit)
   int unroll = 1;         // shots whose loads a split-kernel thread issues together
   int shot_group = 4;     // shots per launch of the split/tiled kernels (0: all)
+  int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
+  bool pairs = true;      // backward kernel pairs share a launch
+  bool illum_on = false;  // sum the forward illumination
+  bool illum_valid = false;
+  float* illum = nullptr; // [NX][NZ]
   std::vector<int> roff;  // first receiver of each shot (+ total)
   std::vector<void*> allocs;
   long long bytes = 0;
@@ -1791,6 +1881,10 @@ struct DeviceState {
     return Mem{p.vzz + o, p.vzx + o, p.vxz + o, p.vxx + o,
                p.szz + o, p.sxx + o, p.sxzz + o, p.sxzx + o};
   }
+  // shots per launch of the time-step kernels
+  int group_size() const {
+    return (fused || shot_group <= 0) ? g.ns : std::min(shot_group, g.ns);
+  }
   Src src_view(int s0) const { return Src{src + 2 * s0, inj + (size_t)s0 * g.nSteps}; }
   float* ring_view(int s0) const {
     return ring ? ring + (size_t)s0 * g.nSteps * 5 * (size_t)g.ringlen : nullptr;
@@ -1823,6 +1917,8 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
     d.unroll = u >= 4 ? 4 : (u >= 2 ? 2 : 1);
   }
   d.shot_group = std::max(0, env_int("FWIFLOW_SHOT_GROUP", 4));
+  d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
+  d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
   Geo& g = d.g;
@@ -2046,6 +2142,8 @@ Status Plan::timing(const std::string& name, double* ms, long long* count) {
 
 long long Plan::device_bytes() const { return d_ ? d_->bytes : 0; }
 
+int Plan::shots_per_launch() const { return d_ ? d_->group_size() : 0; }
+
 Status Plan::run(int calc_id) {
   TRY(check_calc_id(calc_id));
   if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
@@ -2054,6 +2152,8 @@ Status Plan::run(int calc_id) {
   if (!d.stf_set) return Status::Err(FWIFLOW_ERR_ARG, "source time functions not set");
   const bool if_res = calc_id != 2, withAdj = calc_id == 1;
   if (if_res && !d.obs_set) return Status::Err(FWIFLOW_ERR_ARG, "observed data not set");
+  d.illum_valid = false;
+  last_calc = -1;
   HIP_TRY(hipSetDevice(d.dev));
   const Geo& g = d.g;
   const size_t N = (size_t)g.N, F = (size_t)g.ns * N;
@@ -2079,6 +2179,15 @@ Status Plan::run(int calc_id) {
   tT.n = tT.gz * tT.gx;
   tT.xcd = d.xcd ? 1 : 0;
   HIP_TRY(hipMemsetAsync(d.syn, 0, T * sizeof(float), st));
+  float* il_ptr = nullptr;
+  if (d.illum_on) {
+    if (d.fused || d.tiled)
+      return Status::Err(FWIFLOW_ERR_UNSUPPORTED,
+                         "illumination is summed by the split forward kernels only");
+    if (!d.illum) TRY(d.alloc(&d.illum, N));
+    HIP_TRY(hipMemsetAsync(d.illum, 0, N * sizeof(float), st));
+    il_ptr = d.illum;
+  }
 
   const Mdl m = d.mdl();
   const Pml c = d.pml_ptrs();
@@ -2089,7 +2198,6 @@ Status Plan::run(int calc_id) {
   const Src src{d.src, d.inj};
   const dim3 blk(kBZ, d.tile_x);
   const Tiling tU = make_tiling(g.nzA, g.nx, d.tile_x, d.xcd);
-  const dim3 grdU(tU.n);
   const int recBlocks = (d.R + 255) / 256;
   const int sample_every = 8;
   Seg q;
@@ -2104,7 +2212,7 @@ Status Plan::run(int calc_id) {
   // The split and tiled paths take the shots in groups of `shot_group`: a
   // group runs every time step before the next group starts, so the state of
   // one group stays resident in the last-level cache from step to step.
-  const int G = (d.fused || d.shot_group <= 0) ? g.ns : std::min(d.shot_group, g.ns);
+  const int G = d.group_size();
   const int U = d.unroll;
 #define LAUNCH_U(K, grid, block, ...)                                        \
   do {                                                                       \
@@ -2136,6 +2244,11 @@ Status Plan::run(int calc_id) {
     for (int s0 = 0; s0 < g.ns; s0 += G) {
       const int n = std::min(G, g.ns - s0);
       const Geo gg = d.group_geo(n);
+      Tiling tUg = tU;  // forward/adjoint split kernels: shots spread over chunks
+      // (one chunk when the illumination is summed: one writer per cell)
+      tUg.spt = d.spt > 0 && !d.illum_on ? std::min(d.spt, n) : n;
+      tUg.chunks = (n + tUg.spt - 1) / tUg.spt;
+      const dim3 grdUg(tUg.n * tUg.chunks);
       const Src sg = d.src_view(s0);
       float* ring = withAdj ? d.ring_view(s0) : nullptr;
       const int r0 = d.roff[s0], Rg = d.roff[s0 + n] - r0;
@@ -2158,9 +2271,11 @@ Status Plan::run(int calc_id) {
           const Wave w = d.wave(fa, s0);
           const Mem p = d.memv(ma, s0);
           d.launch("stress_fwd", smp, [&] {
-            LAUNCH_U(k_stress_fwd, grdU, blk, tU, gg, m, c, w, p, sg, it, ring);
+            LAUNCH_U(k_stress_fwd, grdUg, blk, tUg, gg, m, c, w, p, sg, it, ring);
           });
-          d.launch("vel_fwd", smp, [&] { LAUNCH_U(k_vel_fwd, grdU, blk, tU, gg, m, c, w, p); });
+          d.launch("vel_fwd", smp, [&] {
+            LAUNCH_U(k_vel_fwd, grdUg, blk, tUg, gg, m, c, w, p, il_ptr);
+          });
         }
         if (Rg > 0) {
           const Wave wo = d.wave(out);
@@ -2252,6 +2367,10 @@ Status Plan::run(int calc_id) {
       for (int s0 = 0; s0 < g.ns; s0 += G) {
         const int n = std::min(G, g.ns - s0);
         const Geo gg = d.group_geo(n);
+        Tiling tUg = tU;  // forward/adjoint split kernels: shots spread over chunks
+        tUg.spt = d.spt > 0 ? std::min(d.spt, n) : n;
+        tUg.chunks = (n + tUg.spt - 1) / tUg.spt;
+        const dim3 grdUg(tUg.n * tUg.chunks);
         const Src sg = d.src_view(s0);
         const float* ring = d.ring_view(s0);
         const int r0 = d.roff[s0], Rg = d.roff[s0 + n] - r0;
@@ -2266,9 +2385,34 @@ Status Plan::run(int calc_id) {
                                d.rec + 3 * r0, Rg, d.res + (size_t)t * d.R + r0);
         };
         // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
-        LAUNCH_U(k_adj_vel, grdU, blk, tU, gg, m, c, a, pg, sg, sgrad, -1);
+        LAUNCH_U(k_adj_vel, grdUg, blk, tUg, gg, m, c, a, pg, sg, sgrad, -1);
         inject(g.nSteps - 1);
-        LAUNCH_U(k_adj_stress, grdU, blk, tU, gg, m, c, a, pg);
+        if (d.pairs && g.nSteps >= 2) {
+          const int nUg = tUg.n * tUg.chunks;
+          const int nRa = (tR.n + 7) / 8 * 8, nUb = (nUg + 7) / 8 * 8;
+          const dim3 grdA(nRa + nUg), grdB(nUb + tR.n);
+          // adj_stress(nSteps-1) with rev_vel(nSteps-2)
+          LAUNCH_U(k_bwd_pair_b, grdB, blk, nUb, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
+                   g.nSteps - 2);
+          for (int it = g.nSteps - 2; it >= 0; it--) {
+            const bool smp = (it % sample_every) == 0;
+            d.launch("bwd_pair_a", smp, [&] {
+              LAUNCH_U(k_bwd_pair_a, grdA, blk, nRa, tR, tUg, gg, m, c, wg, a, pg, sg, acc, ring,
+                       sgrad, it);
+            });
+            inject(it);
+            if (it >= 1) {
+              d.launch("bwd_pair_b", smp, [&] {
+                LAUNCH_U(k_bwd_pair_b, grdB, blk, nUb, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
+                         it - 1);
+              });
+            } else {
+              LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg);
+            }
+          }
+          continue;
+        }
+        LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg);
         for (int it = g.nSteps - 2; it >= 0; it--) {
           const bool smp = (it % sample_every) == 0;
           d.launch("rev_vel", smp, [&] {
@@ -2279,11 +2423,11 @@ Status Plan::run(int calc_id) {
                      ring, it);
           });
           d.launch("adj_vel", smp, [&] {
-            LAUNCH_U(k_adj_vel, grdU, blk, tU, gg, m, c, a, pg, sg, sgrad, it);
+            LAUNCH_U(k_adj_vel, grdUg, blk, tUg, gg, m, c, a, pg, sg, sgrad, it);
           });
           inject(it);
           d.launch("adj_stress", smp, [&] {
-            LAUNCH_U(k_adj_stress, grdU, blk, tU, gg, m, c, a, pg);
+            LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg);
           });
         }
       }
@@ -2297,11 +2441,13 @@ Status Plan::run(int calc_id) {
   if (d.timing) d.collect_timing();
   if (if_res) HIP_TRY(hipMemcpy(&d.misfit_host, d.misfit, sizeof(double), hipMemcpyDeviceToHost));
   last_calc = calc_id;
+  d.illum_valid = d.illum_on;
   return Status::Ok();
 }
 
 Status Plan::misfit(double* out) {
-  if (!d_ || last_calc != 0) return Status::Err(FWIFLOW_ERR_ARG, "no calc_id 0 result");
+  if (!d_ || (last_calc != 0 && last_calc != 1))
+    return Status::Err(FWIFLOW_ERR_ARG, "no calc_id 0 or 1 result");
   if (out) *out = 0.5 * d_->misfit_host;
   return Status::Ok();
 }
@@ -2327,6 +2473,26 @@ Status Plan::gradients(double* gL, double* gM, double* gD, double* gS) {
     HIP_TRY(hipMemcpy(s.data(), d.stf_grad, s.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (size_t k = 0; k < s.size(); k++) gS[k] = s[k];
   }
+  return Status::Ok();
+}
+
+Status Plan::set_illumination(bool on) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  d_->illum_on = on;
+  return Status::Ok();
+}
+
+Status Plan::illumination(double* out) {
+  if (!d_ || !d_->illum_valid)
+    return Status::Err(FWIFLOW_ERR_ARG, "no run with the illumination enabled");
+  if (!out) return Status::Err(FWIFLOW_ERR_ARG, "null output pointer");
+  DeviceState& d = *d_;
+  HIP_TRY(hipSetDevice(d.dev));
+  const int nz = d.g.nz, nx = d.g.nx;
+  std::vector<float> h((size_t)d.g.N);
+  HIP_TRY(hipMemcpy(h.data(), d.illum, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nz; i++)
+    for (int j = 0; j < nx; j++) out[(size_t)i * nx + j] = h[(size_t)j * nz + i];
   return Status::Ok();
 }
 

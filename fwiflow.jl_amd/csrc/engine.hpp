@@ -31,6 +31,9 @@ class Plan {
   Status set_timing(bool on);
   Status timing(const std::string& name, double* ms, long long* count);
   long long device_bytes() const;
+  int shots_per_launch() const;
+  Status set_illumination(bool on);
+  Status illumination(double* out);
 
   Para para;
   Survey survey;

@@ -10,6 +10,7 @@ from .utils import (moduli_to_velocity_grad, npad, padding, padding_adjoint, par
                     sourceGene, surveyGen, velocity_to_moduli)
 from .fwi import (FWI, FWIExample, compute_misfit, compute_misfit_and_gradient,
                   compute_observation, make_fwi)
+from .inversion import PlanObjective, lbfgs
 from ._lib import FwiflowError, version
 
 __all__ = [
@@ -17,5 +18,5 @@ __all__ = [
     "moduli_to_velocity_grad", "npad", "padding", "padding_adjoint", "paraGen", "sourceGene",
     "surveyGen", "velocity_to_moduli", "FWI", "FWIExample", "compute_misfit",
     "compute_misfit_and_gradient", "compute_observation", "make_fwi", "FwiflowError",
-    "version",
+    "version", "PlanObjective", "lbfgs",
 ]

@@ -29,7 +29,8 @@ EXPORTS = [
     "fwiflow_plan_load_obs", "fwiflow_plan_run", "fwiflow_plan_misfit",
     "fwiflow_plan_gradients", "fwiflow_plan_data", "fwiflow_plan_write_data",
     "fwiflow_plan_dims", "fwiflow_plan_set_checkpoint", "fwiflow_plan_set_timing",
-    "fwiflow_plan_timing", "fwiflow_plan_device_bytes",
+    "fwiflow_plan_timing", "fwiflow_plan_device_bytes", "fwiflow_plan_shots_per_launch",
+    "fwiflow_plan_set_illumination", "fwiflow_plan_illumination",
 ]
 
 
@@ -78,6 +79,9 @@ def load():
     L.fwiflow_plan_set_timing.argtypes = [vp, C.c_int]
     L.fwiflow_plan_timing.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_longlong)]
     L.fwiflow_plan_device_bytes.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.fwiflow_plan_shots_per_launch.argtypes = [vp, C.POINTER(C.c_int)]
+    L.fwiflow_plan_set_illumination.argtypes = [vp, C.c_int]
+    L.fwiflow_plan_illumination.argtypes = [vp, dp]
     for name in EXPORTS:
         if name not in ("fwiflow_last_error", "fwiflow_version"):
             getattr(L, name).restype = C.c_int

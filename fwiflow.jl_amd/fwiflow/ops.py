@@ -182,6 +182,22 @@ class Plan:
         check(self._L.fwiflow_plan_device_bytes(self._h, _lib.C.byref(b)))
         return int(b.value)
 
+    def set_illumination(self, on=True):
+        """Also sum the forward illumination (vz^2 + vx^2 over shots and steps)
+        in later runs."""
+        check(self._L.fwiflow_plan_set_illumination(self._h, 1 if on else 0))
+
+    def illumination(self):
+        """Illumination of the last run, [NZ][NX] float64."""
+        out = np.zeros((self.nz, self.nx))
+        check(self._L.fwiflow_plan_illumination(self._h, ptr(out, _lib.C.c_double)))
+        return out
+
+    def shots_per_launch(self):
+        n = _lib.C.c_int(0)
+        check(self._L.fwiflow_plan_shots_per_launch(self._h, _lib.C.byref(n)))
+        return int(n.value)
+
 
 try:  # torch is plumbing for autodiff callers; the op itself does not need it
     import torch

@@ -108,6 +108,9 @@ int fwiflow_plan_set_obs(fwiflow_plan* plan, const float* obs);
 int fwiflow_plan_load_obs(fwiflow_plan* plan);
 /* Run calc_id 0, 1 or 2 on the device; results stay resident. */
 int fwiflow_plan_run(fwiflow_plan* plan, int calc_id);
+/* Misfit 0.5 * sum(res^2) of the last calc_id 0 or 1 run (fwiflow_cufd
+ * reports it for calc_id 0 only, as the reference does; a plan also keeps the
+ * one its gradient run computed, so an optimiser needs one run per step). */
 int fwiflow_plan_misfit(fwiflow_plan* plan, double* misfit);
 /* Gradients of the last calc_id 1 run; layouts as in fwiflow_cufd
  * (grad_stf: group_size rows).  Any pointer may be NULL. */
@@ -132,6 +135,19 @@ int fwiflow_plan_set_timing(fwiflow_plan* plan, int enable);
 int fwiflow_plan_timing(fwiflow_plan* plan, const char* name, double* total_ms, long long* count);
 /* Device bytes held by the plan. */
 int fwiflow_plan_device_bytes(fwiflow_plan* plan, long long* bytes);
+/* Shots covered by one launch of the time-step kernels (the engine runs the
+ * plan's shots in groups of this size; each group runs every time step before
+ * the next).  No reference counterpart: it sizes per-launch byte counts for
+ * kernel timing. */
+int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* shots);
+/* Forward illumination: with enable != 0, later runs also sum, over the
+ * plan's shots and the steps it = 0..nSteps-2, vz^2 + vx^2 after each
+ * velocity update on the update region [2, NZ-nPad-3] x [2, NX-3] (zero
+ * elsewhere).  fwiflow_plan_illumination writes the sum of the last run,
+ * row-major [NZ][NX].  No reference counterpart: it is the preconditioner
+ * input of the inversion driver (gradient / illumination). */
+int fwiflow_plan_set_illumination(fwiflow_plan* plan, int enable);
+int fwiflow_plan_illumination(fwiflow_plan* plan, double* illum);
 
 #ifdef __cplusplus
 }

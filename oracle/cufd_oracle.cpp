@@ -26,6 +26,10 @@
 //   scheme 1  : exact: full forward state checkpoints every `ckpt_every` steps,
 //               segments recomputed forward and replayed backwards.  In exact
 //               arithmetic both schemes give the same gradient.
+// Illumination (no reference counterpart; the preconditioner input of the
+// inversion driver): sum over shots and steps it = 0..nSteps-2 of
+// vz^2 + vx^2 after the velocity update, over the update region
+// [2, NZ-nPad-3] x [2, NX-3], accumulated in double.
 //
 // Layout inside: z fastest, index = x*NZ + z (el_stress.cu:1-19).
 #include <algorithm>
@@ -569,6 +573,7 @@ struct Runner {
   size_t n;
   std::vector<T> lam_grad, mu_grad, den_grad;
   std::vector<T> ga, gb, tsp;
+  std::vector<double> illum;  // sized only when requested
 
   Runner(const Ctx& c_, const double* L, const double* M, const double* D, int scheme_, int k)
       : c(c_), p(c_.g), m(c_.g, L, M, D), pm(c_.g, p), scheme(scheme_), ckpt_every(k) {
@@ -644,6 +649,14 @@ struct Runner {
         ckpt.push_back(std::move(cp));
       }
       forward_step(s, it, src, sh, &data);
+      if (!illum.empty()) {
+        const int nzA = g.nz - g.nPad;
+        for (int x = 2; x <= g.nx - 3; x++)
+          for (int z = 2; z <= nzA - 3; z++) {
+            const size_t i = (size_t)x * nz + z;
+            illum[i] += (double)s.vz[i] * (double)s.vz[i] + (double)s.vx[i] * (double)s.vx[i];
+          }
+      }
     }
     for (size_t j = 0; j < data.size(); j++) sh.syn[j] = (float)data[j];
     if (!if_res) return (T)0;
@@ -757,8 +770,10 @@ struct Runner {
 template <typename T>
 int run_all(const Ctx& c, int scheme, int ckpt_every, int calc_id, int group_size,
             const ShotIn* shots, const double* L, const double* M, const double* D,
-            double* misfit, double* gL, double* gM, double* gD, double* gS) {
+            double* misfit, double* gL, double* gM, double* gD, double* gS,
+            double* illum) {
   Runner<T> R(c, L, M, D, scheme, ckpt_every);
+  if (illum) R.illum.assign(R.n, 0.0);
   T h_l2Obj = 0.0;
   std::vector<double> dummy(c.g.nSteps);
   for (int s = 0; s < group_size; s++) {
@@ -775,6 +790,9 @@ int run_all(const Ctx& c, int scheme, int ckpt_every, int calc_id, int group_siz
       }
   }
   if (calc_id == 0 && misfit) *misfit = 0.5 * h_l2Obj;
+  if (illum)
+    for (int i = 0; i < nz; i++)
+      for (int j = 0; j < nx; j++) illum[(size_t)i * nx + j] = R.illum[(size_t)j * nz + i];
   return 0;
 }
 
@@ -791,13 +809,14 @@ typedef struct {
 // Lambda/Mu in MPa and Den, row-major [NZ][NX]; stf rows already selected per
 // shot ([group_size][nSteps]); receiver/source indices already offset by nPml
 // (Src_Rec.cu:86,90,105,113).  obs/syn/res are concatenated per shot,
-// [nrec][nSteps] each.  Returns 0.
+// [nrec][nSteps] each.  illum (optional, [NZ][NX]) receives the forward
+// illumination.  Returns 0.
 int oracle_cufd(int precision, int scheme, int ckpt_every, int calc_id, const oracle_geom* g,
                 int group_size, const int* z_src, const int* x_src, const int* nrec,
                 const int* z_rec, const int* x_rec, const double* Lambda, const double* Mu,
                 const double* Den, const double* stf_rows, const float* obs, float* syn,
                 float* res, double* misfit, double* grad_lambda, double* grad_mu,
-                double* grad_den, double* grad_stf, int nthreads) {
+                double* grad_den, double* grad_stf, int nthreads, double* illum) {
   Ctx c;
   c.g = Geom{g->nz, g->nx, g->nPml, g->nPad, g->nSteps, g->dz, g->dx, g->dt, g->f0};
   c.nthreads = nthreads > 0 ? nthreads : 1;
@@ -819,9 +838,9 @@ int oracle_cufd(int precision, int scheme, int ckpt_every, int calc_id, const or
   }
   if (precision == 64)
     return run_all<double>(c, scheme, ckpt_every, calc_id, group_size, shots.data(), Lambda, Mu,
-                           Den, misfit, grad_lambda, grad_mu, grad_den, grad_stf);
+                           Den, misfit, grad_lambda, grad_mu, grad_den, grad_stf, illum);
   return run_all<float>(c, scheme, ckpt_every, calc_id, group_size, shots.data(), Lambda, Mu,
-                        Den, misfit, grad_lambda, grad_mu, grad_den, grad_stf);
+                        Den, misfit, grad_lambda, grad_mu, grad_den, grad_stf, illum);
 }
 
 // C-PML profiles of one axis, out = 6 x N: K, a, b, K_half, a_half, b_half.

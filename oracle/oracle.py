@@ -47,7 +47,7 @@ def lib():
         L.oracle_cufd.restype = C.c_int
         L.oracle_cufd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Geom), C.c_int,
                                   ip, ip, ip, ip, ip, dp, dp, dp, dp, fp, fp, fp,
-                                  dp, dp, dp, dp, dp, C.c_int]
+                                  dp, dp, dp, dp, dp, C.c_int, dp]
         L.oracle_cpml.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                   C.c_double, dp]
         L.oracle_model.argtypes = [C.c_int, C.POINTER(Geom), dp, dp, dp, dp]
@@ -99,14 +99,14 @@ def ring_indices(g: Geom):
 
 
 def run(g: Geom, calc_id, lam, mu, den, stf_rows, src, recs, obs=None, precision=32,
-        scheme=0, ckpt_every=0, nthreads=None):
+        scheme=0, ckpt_every=0, nthreads=None, illumination=False):
     """Run the restated operator on explicit inputs.
 
     ``src``: list of (z, x) grid indices per shot (already + nPml);
     ``recs``: list of (z_array, x_array) per shot (already + nPml);
     ``stf_rows``: [group][nSteps]; ``obs``: list of [nrec][nSteps] float32 (calc 0/1).
     Returns dict with syn, res (lists), misfit, grad_lambda/mu/den [NZ][NX],
-    grad_stf [group][nSteps].
+    grad_stf [group][nSteps] and, with ``illumination``, illum [NZ][NX].
     """
     G = len(src)
     nrec = np.array([len(r[0]) for r in recs], dtype=np.int32)
@@ -128,6 +128,7 @@ def run(g: Geom, calc_id, lam, mu, den, stf_rows, src, recs, obs=None, precision
     gM = np.zeros((g.nz, g.nx))
     gD = np.zeros((g.nz, g.nx))
     gS = np.zeros((G, g.nSteps))
+    il = np.zeros((g.nz, g.nx)) if illumination else None
     if nthreads is None:
         nthreads = min(os.cpu_count() or 1, 16)
     rc = lib().oracle_cufd(precision, scheme, ckpt_every, calc_id, C.byref(g), G,
@@ -136,13 +137,13 @@ def run(g: Geom, calc_id, lam, mu, den, stf_rows, src, recs, obs=None, precision
                            _p(D, C.c_double), _p(S, C.c_double), _p(ob, C.c_float),
                            _p(syn, C.c_float), _p(res, C.c_float), _p(misfit, C.c_double),
                            _p(gL, C.c_double), _p(gM, C.c_double), _p(gD, C.c_double),
-                           _p(gS, C.c_double), int(nthreads))
+                           _p(gS, C.c_double), int(nthreads), _p(il, C.c_double))
     if rc != 0:
         raise RuntimeError(f"oracle_cufd failed with {rc}")
     offs = np.concatenate([[0], np.cumsum(nrec) * g.nSteps])
     split = lambda a: [a[offs[i]:offs[i + 1]].reshape(nrec[i], g.nSteps) for i in range(G)]
     return dict(syn=split(syn), res=split(res), misfit=float(misfit[0]), grad_lambda=gL,
-                grad_mu=gM, grad_den=gD, grad_stf=gS)
+                grad_mu=gM, grad_den=gD, grad_stf=gS, illum=il)
 
 
 def read_json_line(path):
@@ -152,7 +153,7 @@ def read_json_line(path):
 
 
 def cufd(lam, mu, den, stf, calc_id, shot_ids, para_fname, precision=32, scheme=0,
-         ckpt_every=0, nthreads=None):
+         ckpt_every=0, nthreads=None, illumination=False):
     """File-driven restatement of ``cufd`` (libCUFD.cu:34-580).
 
     Returns (misfit, grad_lambda, grad_mu, grad_den, grad_stf, run_result).
@@ -175,7 +176,8 @@ def cufd(lam, mu, den, stf, calc_id, shot_ids, para_fname, precision=32, scheme=
             fn = os.path.join(para["data_dir_name"], f"Shot{int(sid)}.bin")
             obs.append(np.fromfile(fn, dtype=np.float32).reshape(sh["nrec"], g.nSteps))
     r = run(g, calc_id, lam, mu, den, np.stack(rows), src, recs, obs or None,
-            precision=precision, scheme=scheme, ckpt_every=ckpt_every, nthreads=nthreads)
+            precision=precision, scheme=scheme, ckpt_every=ckpt_every, nthreads=nthreads,
+            illumination=illumination)
     if calc_id == 2:
         for sid, d in zip(shot_ids, r["syn"]):
             d.astype(np.float32).tofile(os.path.join(para["data_dir_name"], f"Shot{int(sid)}.bin"))

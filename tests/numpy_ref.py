@@ -184,6 +184,7 @@ class Op:
         zr, xr = recs
         syn = np.zeros((len(zr), nt))
         states = []  # (v^it, sigma^{it+1}) per step for the gradient
+        self.illum = zero()  # sum over steps of vz^2 + vx^2 on the update region
         for it in range(nt - 1):
             vz_it, vx_it = st[0].copy(), st[1].copy()
             st = self.forward_step(st, psi)
@@ -191,6 +192,7 @@ class Op:
             st[3][src] += 3.0 * 1500.0 ** 2 * amp[it] * dt
             states.append((vz_it, vx_it, st[2].copy(), st[3].copy(), st[4].copy()))
             st = self.velocity_step(st, psi)
+            self.illum += np.where(self.U, st[0] ** 2 + st[1] ** 2, 0.0)
             syn[:, it + 1] = st[2][zr, xr] + 3.0 * st[3][zr, xr]
         if obs is None:
             return syn

@@ -239,6 +239,79 @@ def test_velocity_api_and_autograd(workspace):
     assert np.allclose(lam0.grad.numpy(), 2.5 * gL) and np.allclose(stf.grad.numpy(), 2.5 * gS)
 
 
+def test_illumination_matches_oracle(oracle_mod, workspace):
+    """Forward illumination (sum over shots and steps of vz^2 + vx^2) from the
+    split forward kernels against the FP32 and FP64 oracles, with shot groups
+    of 1 and of all shots, and unchanged data when it is enabled."""
+    c = models.case_small(nSteps=200)
+    para, stf = models.write_case(c, workspace)
+    lam, mu, den = models.moduli(c, "true")
+    ids = [0, 1]
+    o32 = oracle_mod.cufd(lam, mu, den, stf, 2, ids, para, precision=32, illumination=True)[5]
+    o64 = oracle_mod.cufd(lam, mu, den, stf, 2, ids, para, precision=64, illumination=True)[5]
+    for group in ("1", "0"):
+        old = os.environ.get("FWIFLOW_SHOT_GROUP")
+        os.environ["FWIFLOW_SHOT_GROUP"] = group
+        try:
+            with fwiflow.Plan(para, ids) as P:
+                P.set_model(lam, mu, den)
+                P.set_stf(stf)
+                P.run(2)
+                d0 = P.data()
+                P.set_illumination(True)
+                P.run(2)
+                il = P.illumination()
+                assert np.array_equal(P.data(), d0)
+        finally:
+            if old is None:
+                os.environ.pop("FWIFLOW_SHOT_GROUP", None)
+            else:
+                os.environ["FWIFLOW_SHOT_GROUP"] = old
+        e32, e64 = rel(il, o32["illum"]), rel(il, o64["illum"])
+        say("illumination group", group, "rel32", e32, "rel64", e64)
+        assert e32 < SEIS_TOL and e64 < SEIS_TOL
+        assert np.all(il[o64["illum"] == 0.0] == 0.0)
+
+
+def test_plain_inversion_reduces_misfit(workspace):
+    """The L-BFGS-B driver on the device-resident plan: its one-run objective
+    equals the two-call compute_misfit / compute_misfit_and_gradient path, a
+    few iterations cut the misfit, and every iteration logs a (gradient,
+    illumination, update) pair."""
+    c = models.case_small(nz=40, nx=60, nSteps=400, nshots=3)
+    fwi = fwiflow.make_fwi(c.nz, c.nx, c.dz, c.dx, c.nSteps, c.dt, c.src_z, c.src_x,
+                           c.rec_z, c.rec_x, nPml=c.nPml, f0=c.f0, WORKSPACE=workspace)
+    fwiflow.compute_observation(fwi, c.cp, c.cs, c.rho, c.stf)
+    cp0 = fwiflow.padding(c.cp0, fwi.nPml, fwi.nPad)
+    obj = fwiflow.PlanObjective(fwi, c.cs, c.rho, c.stf, cp_ref=c.cp0)
+    try:
+        J0, g0, ex = obj(cp0)
+        Jt = fwiflow.compute_misfit(fwi, cp0, c.cs, c.rho, c.stf, cp_ref=c.cp0, cs_ref=c.cs,
+                                    rho_ref=c.rho)
+        _, gt, _, _ = fwiflow.compute_misfit_and_gradient(fwi, cp0, c.cs, c.rho, c.stf,
+                                                         cp_ref=c.cp0, cs_ref=c.cs,
+                                                         rho_ref=c.rho)
+        say("plan objective", J0, "two-call", Jt)
+        assert J0 == Jt
+        assert rel(g0, gt.ravel()) < 1e-6
+        assert np.all(ex["illumination"] >= 0.0) and ex["illumination"].max() > 0.0
+        log = os.path.join(workspace, "inv")
+        r = fwiflow.lbfgs(obj, cp0, maxiter=4, log_dir=log)
+    finally:
+        obj.close()
+    say("inversion", r["iterations"], "iterations", r["evaluations"], "evaluations",
+        "misfit", J0, "->", r["misfit"], "in", round(r["seconds"], 2), "s")
+    assert r["misfit"] < 0.7 * J0
+    x = r["x"]
+    assert x.min() >= 1500.0 and x.max() <= 5500.0
+    pairs = sorted(os.listdir(os.path.join(log, "pairs")))
+    assert len(pairs) == r["iterations"] >= 1
+    p = np.load(os.path.join(log, "pairs", pairs[0]))
+    assert p["gradient"].shape == p["illumination"].shape == p["update"].shape == cp0.shape
+    np.testing.assert_allclose(p["gradient"], g0.reshape(cp0.shape).astype(np.float32),
+                               rtol=1e-6, atol=1e-6 * np.abs(g0).max())
+
+
 def _run_plan(para, ids, lam, mu, den, stf, env):
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
@@ -264,13 +337,23 @@ def _run_plan(para, ids, lam, mu, den, stf, env):
 
 @pytest.mark.parametrize("variant", [{"FWIFLOW_FUSED": "1", "FWIFLOW_SEG_X": "7"},
                                      {"FWIFLOW_FUSED": "1", "FWIFLOW_SEG_X": "32"},
-                                     {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "1"}],
-                         ids=["column7", "column32", "tiled"])
+                                     {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "1"},
+                                     {"FWIFLOW_SHOT_GROUP": "0", "FWIFLOW_SHOT_UNROLL": "2"},
+                                     {"FWIFLOW_SHOT_GROUP": "2", "FWIFLOW_SHOT_UNROLL": "4"},
+                                     {"FWIFLOW_SHOT_GROUP": "1", "FWIFLOW_TILED": "1"},
+                                     {"FWIFLOW_SHOTS_PER_THREAD": "2"},
+                                     {"FWIFLOW_BWD_PAIRS": "0"}],
+                         ids=["column7", "column32", "tiled", "group_all_unroll2",
+                              "group2_unroll4", "group1_tiled", "chunks_of_2", "unpaired"])
 def test_fused_kernels_match_split_kernels(workspace, variant):
     """The fused kernels (column-marching fused forward/reverse/adjoint steps
     with odd or even segment widths, or the tiled fused forward step; all
     with ping-pong state and recomputed halos) give the same data, misfit and
-    gradients as the split one-stage kernels."""
+    gradients as the split one-stage kernels, and so do other shot groupings
+    (all shots in one launch, ragged groups of 2, single shots), shot
+    batchings of the split kernels' loads (2 or 4 shots, partial batches)
+    shot ranges spread over workgroups (ragged chunks of 2) and the backward
+    kernels launched one by one instead of in pairs."""
     c = models.case_small(nz=40, nx=61, nSteps=240, nshots=3)
     para, stf = models.write_case(c, workspace)
     lam, mu, den = models.moduli(c, "true")
@@ -278,7 +361,9 @@ def test_fused_kernels_match_split_kernels(workspace, variant):
     lam0, mu0, den0 = models.moduli(c, "start")
     ids = [0, 1, 2]
     d0, g0, J0 = _run_plan(para, ids, lam0, mu0, den0, stf,
-                           {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "0"})
+                           {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "0", "FWIFLOW_SHOT_GROUP": "4",
+                            "FWIFLOW_SHOT_UNROLL": "1", "FWIFLOW_SHOTS_PER_THREAD": "0",
+                            "FWIFLOW_BWD_PAIRS": "1"})
     d1, g1, J1 = _run_plan(para, ids, lam0, mu0, den0, stf, variant)
     assert rel(d1, d0) < 1e-6
     assert abs(J1 - J0) <= 1e-6 * abs(J0)

@@ -1,0 +1,83 @@
+"""The inversion driver's host logic on CPU objectives (no GPU): bounds,
+convergence, evaluation/iteration logs and the (gradient, illumination,
+update) pairs."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fwiflow import inversion
+
+
+def _quadratic(n=40, seed=0):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(n, n)) / np.sqrt(n) + 2.0 * np.eye(n)
+    xs = rng.uniform(1000.0, 6000.0, size=n)  # partly outside the (1500, 5500) bounds
+    calls = []
+
+    def f(x):
+        r = A @ (x - xs)
+        calls.append(x.copy())
+        return 0.5 * float(r @ r), A.T @ r, {"illumination": np.abs(x)}
+    return f, A, xs, calls
+
+
+def test_bounded_quadratic_converges_to_projected_minimum(tmp_path):
+    f, A, xs, _ = _quadratic()
+    x0 = np.full(xs.size, 3000.0)
+    r = inversion.lbfgs(f, x0, log_dir=str(tmp_path), maxiter=500, ftol=1e-14, gtol=1e-10)
+    x = r["x"]
+    assert np.all(x >= 1500.0) and np.all(x <= 5500.0)
+    # KKT: zero gradient on free variables, pushing outward on active bounds
+    g = A.T @ (A @ (x - xs))
+    free = (x > 1500.0 + 1e-6) & (x < 5500.0 - 1e-6)
+    assert np.max(np.abs(g[free])) < 1e-4 * np.max(np.abs(A.T @ A @ xs))
+    assert np.all(g[x <= 1500.0 + 1e-6] >= -1e-6) and np.all(g[x >= 5500.0 - 1e-6] <= 1e-6)
+    assert r["misfit"] < f(x0)[0]
+
+
+def test_pairs_chain_the_iterates(tmp_path):
+    f, A, xs, calls = _quadratic(seed=1)
+    x0 = np.full(xs.size, 2500.0)
+    r = inversion.lbfgs(f, x0, log_dir=str(tmp_path), maxiter=15)
+    k = r["iterations"]
+    assert k > 0
+    pairs = sorted(os.listdir(tmp_path / "pairs"))
+    assert len(pairs) == k
+    evaluated = np.array(calls)
+    x = x0.copy()
+    for name in pairs:
+        p = np.load(tmp_path / "pairs" / name)
+        J, g, ex = f(x)
+        np.testing.assert_allclose(p["gradient"], g.astype(np.float32), rtol=1e-6)
+        np.testing.assert_allclose(p["illumination"], ex["illumination"].astype(np.float32))
+        assert float(p["misfit"]) == pytest.approx(J, rel=1e-12)
+        # the update (stored in float32) leads to an evaluated point: snap to it
+        y = x + p["update"].astype(np.float64)
+        k_near = np.argmin(np.abs(evaluated - y).max(axis=1))
+        assert np.abs(evaluated[k_near] - y).max() < 1e-3
+        x = evaluated[k_near]
+    np.testing.assert_array_equal(x, r["x"])
+    evals = [json.loads(l) for l in open(tmp_path / "evals.jsonl")]
+    iters = [json.loads(l) for l in open(tmp_path / "iters.jsonl")]
+    assert len(iters) == k and len(evals) >= k + 1
+    assert all(a["seconds"] <= b["seconds"] for a, b in zip(evals, evals[1:]))
+    assert iters[-1]["misfit"] <= evals[0]["misfit"]
+
+
+def test_target_misfit_stops_early(tmp_path):
+    f, A, xs, _ = _quadratic(seed=2)
+    x0 = np.full(xs.size, 2000.0)
+    full = inversion.lbfgs(f, x0, maxiter=200)
+    J0 = f(x0)[0]
+    target = J0 - 0.5 * (J0 - full["misfit"])
+    r = inversion.lbfgs(f, x0, maxiter=200, target_misfit=target)
+    assert r["message"] == "target misfit reached"
+    assert r["misfit"] <= target
+    assert r["iterations"] < full["iterations"]
+
+
+def test_reference_defaults():
+    assert inversion.CP_BOUNDS == (1500.0, 5500.0)
+    assert (inversion.MAXITER, inversion.FTOL, inversion.GTOL) == (100, 1e-6, 1e-6)

@@ -309,6 +309,25 @@ def test_matches_independent_numpy_restatement(oracle_mod, workspace):
     np.testing.assert_allclose(gS[:2], np.stack(S), rtol=0, atol=1e-12 * np.abs(gS).max())
 
 
+def test_illumination_matches_independent_numpy_restatement(oracle_mod, workspace):
+    """Forward illumination (sum over shots and steps of vz^2 + vx^2 on the
+    update region): the C++ oracle against the numpy transcription in FP64, and
+    zero outside the update region."""
+    c = models.case_small(nz=24, nx=30, nSteps=120, nshots=2, nPml=6)
+    para, stf, ids = _setup(oracle_mod, c, workspace)
+    lam0, mu0, den0 = models.moduli(c, "start")
+    r = oracle_mod.cufd(lam0, mu0, den0, stf, 2, ids, para, precision=64, illumination=True)[5]
+    op = NR.Op(c.NZ, c.NX, c.nPml, c.nPad, c.dz, c.dx, c.dt, c.f0, lam0, mu0, den0)
+    il = np.zeros((c.NZ, c.NX))
+    for s in range(2):
+        op.run((c.src_z[s] + c.nPml, c.src_x[s] + c.nPml), (c.rec_z + c.nPml, c.rec_x + c.nPml),
+               stf[s])
+        il += op.illum
+    assert np.linalg.norm(r["illum"] - il) <= 1e-12 * np.linalg.norm(il)
+    assert np.all(r["illum"][~op.U] == 0.0) and np.all(r["illum"] >= 0.0)
+    assert np.count_nonzero(r["illum"][op.U]) > 0.9 * np.count_nonzero(op.U)
+
+
 def test_backward_schemes_agree(oracle_mod, workspace):
     """Boundary-ring time reversal (the reference) vs checkpoint recompute."""
     c = models.case_small(nSteps=300)

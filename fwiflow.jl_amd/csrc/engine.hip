@@ -1570,6 +1570,36 @@ __global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml
   k_adj_stress_body<U>(blockIdx.x, tl, g, m, c, a, p);
 }
 
+// Interior checkpoints of the reverse sweep (no reference counterpart): the 5
+// components on the interior rectangle [nPml, nzA-1-nPml] x [nPml, nx-1-nPml]
+// are saved before forward step it and written back after reverse step it,
+// for it a positive multiple of the interval, which bounds the drift of the
+// time-reversed re-materialisation.  Layout [shot][checkpoint][component]
+// [interior cell, z fastest].  Thread domain: the interior rectangle.
+__global__ __launch_bounds__(256) void k_ckpt(Tiling tl, Geo g, Wave w, float* __restrict__ ck,
+                                             int slot, int nslots, int save) {
+  int tz, tx;
+  tile_of(tl, tz, tx);
+  const int Ih = g.nzA - 2 * g.nPml, Iw = g.nx - 2 * g.nPml;
+  const int zi = tz * kBZ + threadIdx.x;
+  const int xi = tx * (int)blockDim.y + threadIdx.y;
+  if (zi >= Ih || xi >= Iw) return;
+  const long long i = (long long)(xi + g.nPml) * g.nz + (zi + g.nPml);
+  const size_t Ic = (size_t)Ih * Iw, j = (size_t)xi * Ih + zi;
+  float* f[5] = {w.vz, w.vx, w.szz, w.sxx, w.sxz};
+  for (int s = 0; s < g.ns; ++s) {
+    float* b = ck + ((size_t)s * nslots + slot) * 5 * Ic + j;
+    const long long o = s * g.N + i;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      if (save)
+        b[q * Ic] = f[q][o];
+      else
+        f[q][o] = b[q * Ic];
+    }
+  }
+}
+
 // Backward kernel pairs with disjoint reads and writes, launched as one grid
 // so they share a launch and its ramp-up and drain: workgroups [0, nA) run
 // the first kernel's tiles, the rest the second's.  nA is a multiple of 8, so
@@ -1733,7 +1763,9 @@ struct DeviceState {
   hipStream_t stream = nullptr;
   Geo g{};
   int R = 0;
-  int ckpt_every = 0;
+  int ckpt_every = 0;     // interior checkpoint interval of the reverse sweep (0: none)
+  float* ck = nullptr;    // interior checkpoints [shot][slot][5][interior]
+  size_t ck_floats = 0;
   int tile_x = 4;    // x extent of a workgroup tile (blockDim.y)
   bool xcd = true;   // XCD-banded workgroup order
   bool fused = false; // column-marching fused kernels (FWIFLOW_FUSED=1)
@@ -2119,8 +2151,9 @@ Status Plan::load_obs() {
 Status Plan::set_checkpoint(int k) {
   if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
   if (k < 0) return Status::Err(FWIFLOW_ERR_ARG, "checkpoint interval must be >= 0");
-  if (k > 0)
-    return Status::Err(FWIFLOW_ERR_UNSUPPORTED, "interior checkpoint resets are not built yet");
+  if (k > 0 && d_->fused)
+    return Status::Err(FWIFLOW_ERR_UNSUPPORTED,
+                       "interior checkpoints need the split or tiled kernels (FWIFLOW_FUSED=0)");
   d_->ckpt_every = k;
   return Status::Ok();
 }
@@ -2166,6 +2199,18 @@ Status Plan::run(int calc_id) {
     if (!d.grad) TRY(d.alloc(&d.grad, 3 * N));
     if (!d.stf_grad) TRY(d.alloc(&d.stf_grad, (size_t)g.ns * g.nSteps));
   }
+  // interior checkpoints: slot it/k for it = k, 2k, .. <= nSteps-2 (slot 0 unused)
+  const int ck_k = withAdj ? d.ckpt_every : 0;
+  const int ck_slots = ck_k > 0 ? (g.nSteps - 2) / ck_k + 1 : 0;
+  const size_t ck_cells = (size_t)(g.nzA - 2 * g.nPml) * (g.nx - 2 * g.nPml);
+  if (ck_k > 0) {
+    const size_t need = (size_t)g.ns * ck_slots * 5 * ck_cells;
+    if (d.ck_floats < need) {
+      TRY(d.alloc(&d.ck, need));  // a smaller earlier buffer stays in the plan's pool
+      d.ck_floats = need;
+    }
+  }
+  const Tiling tI = make_tiling(g.nzA - 2 * g.nPml, g.nx - 2 * g.nPml, d.tile_x, d.xcd);
   hipStream_t st = d.stream;
   HIP_TRY(hipMemsetAsync(d.fwd, 0, 5 * F * sizeof(float), st));
   HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
@@ -2254,9 +2299,13 @@ Status Plan::run(int calc_id) {
       const int r0 = d.roff[s0], Rg = d.roff[s0 + n] - r0;
       const dim3 rgrid((Rg + 255) / 256);
       fa = fa0, fb = d.fwd2, ma = ma0, mb = d.mem2;
+      float* ckg = ck_k > 0 ? d.ck + (size_t)s0 * ck_slots * 5 * ck_cells : nullptr;
       for (int it = 0; it <= g.nSteps - 2; it++) {
         const bool smp = (it % sample_every) == 0;
         float* out = fa;  // buffer that holds step it+1 after this step
+        if (ckg && it > 0 && it % ck_k == 0)
+          hipLaunchKernelGGL(k_ckpt, dim3(tI.n), blk, 0, st, tI, gg, d.wave(fa, s0), ckg,
+                             it / ck_k, ck_slots, 1);
         if (d.tiled) {
           const Wave wa = d.wave(fa, s0), wb = d.wave(fb, s0);
           const Mem pa = d.memv(ma, s0), pb = d.memv(mb, s0);
@@ -2379,6 +2428,13 @@ Status Plan::run(int calc_id) {
         const Wave a = d.wave(d.adj, s0);
         const Mem pg = d.memv(ma, s0);   // adjoint memory variables
         float* sgrad = d.stf_grad + (size_t)s0 * g.nSteps;
+        float* ckg = ck_k > 0 ? d.ck + (size_t)s0 * ck_slots * 5 * ck_cells : nullptr;
+        // after reverse step it the group's state is the one before forward step it
+        auto reset = [&](int it) {
+          if (ckg && it > 0 && it % ck_k == 0)
+            hipLaunchKernelGGL(k_ckpt, dim3(tI.n), blk, 0, st, tI, gg, wg, ckg, it / ck_k,
+                               ck_slots, 0);
+        };
         auto inject = [&](int t) {
           if (Rg > 0)
             hipLaunchKernelGGL(k_res_inject, rgrid, dim3(256), 0, st, g, a_all.szz, a_all.sxx,
@@ -2401,6 +2457,7 @@ Status Plan::run(int calc_id) {
                        sgrad, it);
             });
             inject(it);
+            reset(it);
             if (it >= 1) {
               d.launch("bwd_pair_b", smp, [&] {
                 LAUNCH_U(k_bwd_pair_b, grdB, blk, nUb, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
@@ -2422,6 +2479,7 @@ Status Plan::run(int calc_id) {
             LAUNCH_U(k_rev_stress, grdR, blk, tR, gg, m, wg, a, sg, acc, acc + N, acc + 2 * N,
                      ring, it);
           });
+          reset(it);
           d.launch("adj_vel", smp, [&] {
             LAUNCH_U(k_adj_vel, grdUg, blk, tUg, gg, m, c, a, pg, sg, sgrad, it);
           });

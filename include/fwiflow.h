@@ -126,7 +126,9 @@ int fwiflow_plan_dims(fwiflow_plan* plan, int* dims);
 /* Backward scheme: ckpt_every = 0 reproduces the reference (boundary ring
  * every step + time-reversed re-materialisation); k > 0 additionally stores
  * the interior wavefield every k steps and resets the reverse sweep to it
- * there, which bounds the FP32 drift of the reversal. */
+ * there, which bounds the FP32 drift of the reversal.  Extra device memory:
+ * shots x ((nSteps-2)/k + 1) x 5 x interior cells x 4 B.  No reference
+ * counterpart (the reference's scheme is k = 0). */
 int fwiflow_plan_set_checkpoint(fwiflow_plan* plan, int ckpt_every);
 /* Kernel timing with HIP events on the plan's stream (off by default).
  * fwiflow_plan_timing(name) returns the summed milliseconds and launch count

@@ -26,6 +26,11 @@
 //   scheme 1  : exact: full forward state checkpoints every `ckpt_every` steps,
 //               segments recomputed forward and replayed backwards.  In exact
 //               arithmetic both schemes give the same gradient.
+//   scheme 2  : scheme 0 plus interior resets: the 5 components on the interior
+//               rectangle [nPml, NZ-nPad-1-nPml] x [nPml, NX-1-nPml] are saved
+//               before forward step it and written back after reverse step it,
+//               for it a positive multiple of `ckpt_every` (bounds the FP32
+//               drift of the reversal; no reference counterpart).
 // Illumination (no reference counterpart; the preconditioner input of the
 // inversion driver): sum over shots and steps it = 0..nSteps-2 of
 // vz^2 + vx^2 after the velocity update, over the update region
@@ -634,9 +639,16 @@ struct Runner {
     std::vector<T> rbuf;  // scheme 0: [it][field][ring]
     std::vector<std::vector<T>> ckpt;  // scheme 1
     const int k = std::max(1, ckpt_every);
-    if (withAdj && scheme == 0) rbuf.assign((size_t)nSteps * 5 * rl, (T)0);
+    if (withAdj && scheme != 1) rbuf.assign((size_t)nSteps * 5 * rl, (T)0);
+    std::vector<std::vector<T>> interior;  // scheme 2: [checkpoint][5 * NZ*NX]
     for (int it = 0; it <= nSteps - 2; it++) {
-      if (withAdj && scheme == 0) {
+      if (withAdj && scheme == 2 && it > 0 && it % k == 0) {
+        std::vector<T> cp;
+        cp.reserve(5 * n);
+        for (auto* v : {&s.vz, &s.vx, &s.szz, &s.sxx, &s.sxz}) cp.insert(cp.end(), v->begin(), v->end());
+        interior.push_back(std::move(cp));
+      }
+      if (withAdj && scheme != 1) {
         T* dst = rbuf.data() + (size_t)it * 5 * rl;
         const T* f[5] = {s.szz.data(), s.sxz.data(), s.sxx.data(), s.vz.data(), s.vx.data()};
         for (int q = 0; q < 5; q++)
@@ -695,7 +707,18 @@ struct Runner {
     res_injection(nSteps - 1);
     el_stress_adj(c, p, m, pm, a);
 
-    if (scheme == 0) {
+    if (scheme != 1) {
+      auto reset = [&](int it) {  // scheme 2: interior state before forward step it
+        const std::vector<T>& cp = interior[(size_t)(it / k) - 1];
+        const int nzA = g.nz - g.nPad, P0 = g.nPml;
+        T* f[5] = {s.vz.data(), s.vx.data(), s.szz.data(), s.sxx.data(), s.sxz.data()};
+        for (int q = 0; q < 5; q++)
+          for (int x = P0; x <= g.nx - 1 - P0; x++)
+            for (int z = P0; z <= nzA - 1 - P0; z++) {
+              const size_t i = (size_t)x * nz + z;
+              f[q][i] = cp[(size_t)q * n + i];
+            }
+      };
       auto restore = [&](int it, bool stress) {
         const T* src_ = rbuf.data() + (size_t)it * 5 * rl;
         if (stress) {
@@ -717,6 +740,7 @@ struct Runner {
         el_stress_rev(c, p, m, w, true, a.szz.data(), a.sxx.data(), a.sxz.data(),
                       lam_grad.data(), mu_grad.data(), tsp);
         restore(it, true);
+        if (scheme == 2 && it > 0 && it % k == 0) reset(it);
         el_velocity_adj(c, p, m, pm, a);
         res_injection(it);
         el_stress_adj(c, p, m, pm, a);

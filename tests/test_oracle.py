@@ -342,6 +342,25 @@ def test_backward_schemes_agree(oracle_mod, workspace):
         assert np.linalg.norm(x - y) <= 1e-4 * np.linalg.norm(y)
 
 
+def test_interior_resets_restore_the_forward_states(oracle_mod, workspace):
+    """Scheme 2 (ring reversal plus interior resets every k steps) against the
+    exact checkpoint-recompute scheme in FP64: a reset that wrote back any
+    other step's state would move the gradient far beyond round-off, and the
+    distance to the exact scheme shrinks as resets get denser."""
+    c = models.case_small(nSteps=300)
+    para, stf, ids = _setup(oracle_mod, c, workspace)
+    lam0, mu0, den0 = models.moduli(c, "start")
+    ex = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=64, scheme=1,
+                         ckpt_every=25)
+    err = {}
+    for k in (0, 1, 7, 60):
+        r = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=64,
+                            scheme=2 if k else 0, ckpt_every=k)
+        err[k] = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(r[1:5], ex[1:5]))
+        assert err[k] < 1e-13, (k, err)
+    assert err[1] <= err[0]
+
+
 def test_fp32_reference_arithmetic_close_to_fp64(oracle_mod, workspace):
     c = models.case_small(nSteps=300)
     para, stf, ids = _setup(oracle_mod, c, workspace)

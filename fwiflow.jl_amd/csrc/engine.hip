@@ -272,10 +272,10 @@ struct VelIn {
 };
 
 template <int U>
-__global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p,
+__device__ __forceinline__ void k_vel_fwd_body(int bid, Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p,
                                                 float* __restrict__ illum) {
   int tz, tx;
-  const int chunk = tile_of(tl, tz, tx);
+  const int chunk = tile_of(tl, bid, tz, tx);
   const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
   const int z = tz * kBZ + threadIdx.x;
   const int x = tx * (int)blockDim.y + threadIdx.y;
@@ -352,6 +352,12 @@ __global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c,
     }
   }
   if (illum) illum[i] += il;
+}
+
+template <int U>
+__global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p,
+                                                float* __restrict__ illum) {
+  k_vel_fwd_body<U>(blockIdx.x, tl, g, m, c, w, p, illum);
 }
 
 // ------------------------------------------------------------ fused forward
@@ -1085,13 +1091,39 @@ __global__ __launch_bounds__(1024) void k_fwd_tile(Tiling tl, Geo g, Mdl m, Pml 
 
 // recording (utilities.cu:557-567): szz + 3 sxx at each receiver, written to
 // row `t` of the time-major seismogram array.
-__global__ void k_record(Geo g, const float* __restrict__ szz, const float* __restrict__ sxx,
-                         const int* __restrict__ rec, int R, float* __restrict__ row) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void record_at(const Geo& g, const float* __restrict__ szz,
+                                 const float* __restrict__ sxx, const int* __restrict__ rec,
+                                 int R, float* __restrict__ row, int r) {
   if (r >= R) return;
   const int z = rec[3 * r], x = rec[3 * r + 1], s = rec[3 * r + 2];
   const long long o = s * g.N + (long long)x * g.nz + z;
   row[r] = szz[o] + 3.f * sxx[o];
+}
+
+__global__ void k_record(Geo g, const float* __restrict__ szz, const float* __restrict__ sxx,
+                         const int* __restrict__ rec, int R, float* __restrict__ row) {
+  record_at(g, szz, sxx, rec, R, row, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The velocity update of one step with the recording of that step appended as
+// extra workgroups of the same grid: recording reads the stresses, which the
+// velocity update does not touch, so both run in one launch.  Workgroups
+// [0, nV) update velocities (nV a multiple of 8), the rest record 256
+// receivers each.
+template <int U>
+__global__ __launch_bounds__(256) void k_vel_fwd_rec(int nV, Tiling tl, Geo g, Mdl m, Pml c, Wave w,
+                                                    Mem p, float* __restrict__ illum, Geo gall,
+                                                    const float* __restrict__ szz,
+                                                    const float* __restrict__ sxx,
+                                                    const int* __restrict__ rec, int R,
+                                                    float* __restrict__ row) {
+  const int b = blockIdx.x;
+  if (b < nV) {
+    if (b < tl.n * tl.chunks) k_vel_fwd_body<U>(b, tl, g, m, c, w, p, illum);
+  } else {
+    record_at(gall, szz, sxx, rec, R, row,
+              (b - nV) * 256 + (int)(threadIdx.y * blockDim.x + threadIdx.x));
+  }
 }
 
 // Residual (libCUFD.cu:258-313 without the optional filters): window obs and
@@ -2322,11 +2354,16 @@ Status Plan::run(int calc_id) {
           d.launch("stress_fwd", smp, [&] {
             LAUNCH_U(k_stress_fwd, grdUg, blk, tUg, gg, m, c, w, p, sg, it, ring);
           });
+          // velocity update with this step's recording appended to its grid
+          const int nV = (tUg.n * tUg.chunks + 7) / 8 * 8;
+          const Wave wo = d.wave(fa);
+          float* row = d.syn + (size_t)(it + 1) * d.R + r0;
           d.launch("vel_fwd", smp, [&] {
-            LAUNCH_U(k_vel_fwd, grdUg, blk, tUg, gg, m, c, w, p, il_ptr);
+            LAUNCH_U(k_vel_fwd_rec, dim3(nV + (Rg + 255) / 256), blk, nV, tUg, gg, m, c, w, p,
+                     il_ptr, g, wo.szz, wo.sxx, d.rec + 3 * r0, Rg, row);
           });
         }
-        if (Rg > 0) {
+        if (Rg > 0 && d.tiled) {
           const Wave wo = d.wave(out);
           hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, st, g, wo.szz, wo.sxx, d.rec + 3 * r0,
                              Rg, d.syn + (size_t)(it + 1) * d.R + r0);

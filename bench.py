@@ -46,6 +46,12 @@ KERNEL_BYTES = {
 }
 # kernels that run two of the above as one grid (disjoint workgroup ranges)
 KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_stress", "rev_vel")}
+# Measured HBM-side bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, each
+# scaled by a 1 GiB copy's counters) at this workload with 4 shots per launch:
+# profiles/stencil_kernels_608x1780_4shot_groups_counters.md.  Reported as
+# roofline.traffic when the dominant kernel and shots per launch match.
+MEASURED_TRAFFIC_4SHOTS = {"bwd_pair_a": 361.6e6, "bwd_pair_b": 335.6e6,
+                           "stress_fwd": 167.8e6, "vel_fwd": 147.6e6}
 # C-PML memory-variable traffic per PML cell-direction (2 variables RW = 16 B)
 KERNEL_PML = {"fwd_tile": 32.0, "fwd_step": 32.0, "adj_step": 32.0, "stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
 
@@ -223,7 +229,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (MEASURED_TRAFFIC_4SHOTS.get(dominant)
+                        if spl == 4 and c.NZ == 608 and c.NX == 1780 else None),
             "avg_launch_ms": round(avg_ms, 4),
             "bytes_per_launch": lb,
             "shots_per_launch": spl,

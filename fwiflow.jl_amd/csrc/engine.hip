@@ -115,13 +115,18 @@ __device__ inline float* ring_ptr(float* ring, const Geo& g, int it, int comp, i
 // A launch may also split its shots into `chunks` ranges of `spt` shots, one
 // range per copy of the tile grid (workgroup id = chunk * n + tile before
 // the XCD permutation); tile_of returns the chunk.
+// With `perm` set (one chunk only), the tile of workgroup b is perm[b]: a
+// host-built order that also balances the heavy C-PML columns over the XCDs.
 struct Tiling {
   int gz, gx, n, xcd;
   int chunks = 1, spt = 1 << 30;
+  const int* perm = nullptr;
 };
 
 __device__ inline int tile_of(const Tiling& t, int b, int& bz, int& bx) {
-  if (t.xcd) {
+  if (t.perm && t.chunks == 1) {
+    b = t.perm[b];
+  } else if (t.xcd) {
     const int per = t.n * t.chunks / 8;
     if (b < per * 8) b = (b % 8) * per + b / 8;
   }
@@ -1108,8 +1113,8 @@ __global__ void k_record(Geo g, const float* __restrict__ szz, const float* __re
 // The velocity update of one step with the recording of that step appended as
 // extra workgroups of the same grid: recording reads the stresses, which the
 // velocity update does not touch, so both run in one launch.  Workgroups
-// [0, nV) update velocities (nV a multiple of 8), the rest record 256
-// receivers each.
+// [0, nV) update velocities (nV a multiple of 8), the rest record one
+// receiver per thread.
 template <int U>
 __global__ __launch_bounds__(256) void k_vel_fwd_rec(int nV, Tiling tl, Geo g, Mdl m, Pml c, Wave w,
                                                     Mem p, float* __restrict__ illum, Geo gall,
@@ -1121,8 +1126,9 @@ __global__ __launch_bounds__(256) void k_vel_fwd_rec(int nV, Tiling tl, Geo g, M
   if (b < nV) {
     if (b < tl.n * tl.chunks) k_vel_fwd_body<U>(b, tl, g, m, c, w, p, illum);
   } else {
+    const int per = (int)(blockDim.x * blockDim.y);
     record_at(gall, szz, sxx, rec, R, row,
-              (b - nV) * 256 + (int)(threadIdx.y * blockDim.x + threadIdx.x));
+              (b - nV) * per + (int)(threadIdx.y * blockDim.x + threadIdx.x));
   }
 }
 
@@ -1782,6 +1788,37 @@ static Tiling make_tiling(int nz_dom, int nx_dom, int bx, bool xcd) {
   return t;
 }
 
+// Workgroup -> tile order for an XCD-banded launch whose first `heavy_left`
+// and last `heavy_right` tile columns are heavier (the x-direction C-PML):
+// workgroup b runs on XCD b % 8, and each XCD gets one contiguous run of the
+// heavy tiles (run first) followed by one contiguous run of the others, so
+// every XCD carries the same share of heavy work and neighbouring tiles stay
+// on one XCD's L2.  Tile id = column * gz + row (rows fastest).
+static std::vector<int> balanced_tile_order(int gz, int gx, int heavy_left, int heavy_right) {
+  const int n = gz * gx;
+  std::vector<int> heavy, light;
+  for (int bx = 0; bx < gx; bx++) {
+    const bool h = bx < heavy_left || bx >= gx - heavy_right;
+    for (int bz = 0; bz < gz; bz++) (h ? heavy : light).push_back(bx * gz + bz);
+  }
+  std::vector<int> perm(n);
+  size_t ho = 0, lo = 0;
+  const size_t H = heavy.size();
+  for (int k = 0; k < 8; k++) {
+    const int len = n > k ? (n - k + 7) / 8 : 0;  // workgroups b < n with b % 8 == k
+    const size_t hk = std::min((size_t)len, H * (k + 1) / 8 - H * k / 8);
+    int j = 0;
+    for (size_t q = 0; q < hk; q++) perm[(size_t)(j++) * 8 + k] = heavy[ho++];
+    while (j < len) {
+      if (lo < light.size())
+        perm[(size_t)(j++) * 8 + k] = light[lo++];
+      else
+        perm[(size_t)(j++) * 8 + k] = heavy[ho++];
+    }
+  }
+  return perm;
+}
+
 // threads of a fused workgroup: whole wave64s covering the active rows
 static int fused_threads(const Geo& g) { return ((g.nzA + 63) / 64) * 64; }
 
@@ -1809,6 +1846,8 @@ struct DeviceState {
   int shot_group = 4;     // shots per launch of the split/tiled kernels (0: all)
   int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
   bool pairs = true;      // backward kernel pairs share a launch
+  bool balance = true;    // C-PML columns shared evenly by the XCDs (update-region launches)
+  int* perm_u = nullptr;  // workgroup -> tile order of the update-region tiling
   bool illum_on = false;  // sum the forward illumination
   bool illum_valid = false;
   float* illum = nullptr; // [NX][NZ]
@@ -1983,6 +2022,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.shot_group = std::max(0, env_int("FWIFLOW_SHOT_GROUP", 4));
   d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
   d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
+  d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
   Geo& g = d.g;
@@ -2063,6 +2103,20 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
     HIP_TRY(hipMemcpy(d.rptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.rzr, zr.data(), zr.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.rrg, rg.data(), rg.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  if (d.balance) {
+    const Tiling t = make_tiling(g.nzA, g.nx, d.tile_x, true);
+    const int hl = (g.nPml + d.tile_x - 1) / d.tile_x;
+    const int hr = t.gx - (g.nx - g.nPml) / d.tile_x;
+    const std::vector<int> perm = balanced_tile_order(t.gz, t.gx, hl, hr);
+    std::vector<char> seen(perm.size(), 0);
+    for (int v : perm) {
+      if (v < 0 || v >= t.n || seen[v])
+        return Status::Err(FWIFLOW_ERR_HIP, "internal: workgroup order is not a permutation");
+      seen[v] = 1;
+    }
+    TRY(d.alloc(&d.perm_u, perm.size()));
+    HIP_TRY(hipMemcpy(d.perm_u, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   TRY(d.alloc(&d.inj, (size_t)g.ns * g.nSteps));
   const size_t F = (size_t)g.ns * N;
@@ -2274,7 +2328,8 @@ Status Plan::run(int calc_id) {
   float* mb = d.mem2;
   const Src src{d.src, d.inj};
   const dim3 blk(kBZ, d.tile_x);
-  const Tiling tU = make_tiling(g.nzA, g.nx, d.tile_x, d.xcd);
+  Tiling tU = make_tiling(g.nzA, g.nx, d.tile_x, d.xcd);
+  tU.perm = d.perm_u;
   const int recBlocks = (d.R + 255) / 256;
   const int sample_every = 8;
   Seg q;
@@ -2359,7 +2414,8 @@ Status Plan::run(int calc_id) {
           const Wave wo = d.wave(fa);
           float* row = d.syn + (size_t)(it + 1) * d.R + r0;
           d.launch("vel_fwd", smp, [&] {
-            LAUNCH_U(k_vel_fwd_rec, dim3(nV + (Rg + 255) / 256), blk, nV, tUg, gg, m, c, w, p,
+            const int per = (int)(blk.x * blk.y);
+            LAUNCH_U(k_vel_fwd_rec, dim3(nV + (Rg + per - 1) / per), blk, nV, tUg, gg, m, c, w, p,
                      il_ptr, g, wo.szz, wo.sxx, d.rec + 3 * r0, Rg, row);
           });
         }

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "../../include/fwiflow.h"
+#include "cnn.hpp"
 #include "config.hpp"
 #include "engine.hpp"
 
@@ -14,6 +15,10 @@ using fwiflow::Survey;
 
 struct fwiflow_plan {
   Plan impl;
+};
+
+struct fwiflow_cnn {
+  fwiflow::Cnn impl;
 };
 
 namespace {
@@ -208,6 +213,51 @@ int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* shots) {
   if (!plan || !shots) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
   *shots = plan->impl.shots_per_launch();
   return report(Status::Ok());
+}
+
+#define CNN_CALL(expr)                                                      \
+  do {                                                                      \
+    if (!net) return report(Status::Err(FWIFLOW_ERR_ARG, "null network")); \
+    return report(expr);                                                    \
+  } while (0)
+
+int fwiflow_cnn_create(fwiflow_cnn** net, int gpu_id, int height, int width, unsigned seed) {
+  if (!net) return report(Status::Err(FWIFLOW_ERR_ARG, "null output handle"));
+  *net = nullptr;
+  fwiflow_cnn* n = new fwiflow_cnn();
+  const Status s = n->impl.create(gpu_id, height, width, seed);
+  if (!s.ok()) {
+    delete n;
+    return report(s);
+  }
+  *net = n;
+  return report(Status::Ok());
+}
+int fwiflow_cnn_destroy(fwiflow_cnn* net) {
+  delete net;
+  return report(Status::Ok());
+}
+int fwiflow_cnn_num_params(fwiflow_cnn* net, int* n) {
+  if (!net || !n) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
+  *n = net->impl.num_params();
+  return report(Status::Ok());
+}
+int fwiflow_cnn_get_params(fwiflow_cnn* net, float* params) {
+  CNN_CALL(net->impl.get_params(params));
+}
+int fwiflow_cnn_set_params(fwiflow_cnn* net, const float* params) {
+  CNN_CALL(net->impl.set_params(params));
+}
+int fwiflow_cnn_forward(fwiflow_cnn* net, const float* input, float* output) {
+  CNN_CALL(net->impl.forward(input, output));
+}
+int fwiflow_cnn_gradients(fwiflow_cnn* net, const float* input, const float* target,
+                          const float* mask, double* loss, float* grad_params) {
+  CNN_CALL(net->impl.gradients(input, target, mask, loss, grad_params));
+}
+int fwiflow_cnn_train_step(fwiflow_cnn* net, const float* input, const float* target,
+                           const float* mask, float lr, double* loss) {
+  CNN_CALL(net->impl.train_step(input, target, mask, lr, loss));
 }
 
 }  // extern "C"

@@ -1843,7 +1843,7 @@ struct DeviceState {
   int rev_group = 8; // shots per workgroup of the fused reverse step
   bool fused_rev = true;  // fused reverse step (needs its LDS to fit)
   int unroll = 1;         // shots whose loads a split-kernel thread issues together
-  int shot_group = 4;     // shots per launch of the split/tiled kernels (0: all)
+  int shot_group = -1;    // shots per launch of the split/tiled kernels (0: all, -1: auto)
   int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
   bool pairs = true;      // backward kernel pairs share a launch
   bool balance = true;    // C-PML columns shared evenly by the XCDs (update-region launches)
@@ -1984,9 +1984,18 @@ struct DeviceState {
     return Mem{p.vzz + o, p.vzx + o, p.vxz + o, p.vxx + o,
                p.szz + o, p.sxx + o, p.sxzz + o, p.sxzx + o};
   }
-  // shots per launch of the time-step kernels
+  // Shots per launch of the time-step kernels.  Automatic: as many shots as
+  // keep one group's backward working set (forward and adjoint state, C-PML
+  // memory variables, about 48 B per cell and shot) near 224 MB, i.e. inside
+  // the 256 MB Infinity Cache next to the model and accumulators.  Measured
+  // at 608x1780: 3, 4 and 5 shots give 55.2, 59.4 and 56.4 shot-gradients/s
+  // at nt 400; the rule picks 4.
   int group_size() const {
-    return (fused || shot_group <= 0) ? g.ns : std::min(shot_group, g.ns);
+    if (fused || shot_group == 0) return g.ns;
+    if (shot_group > 0) return std::min(shot_group, g.ns);
+    const double per_shot = 48.0 * (double)g.N;
+    const int n = (int)std::lround(224e6 / per_shot);
+    return std::max(1, std::min(n, g.ns));
   }
   Src src_view(int s0) const { return Src{src + 2 * s0, inj + (size_t)s0 * g.nSteps}; }
   float* ring_view(int s0) const {
@@ -2019,7 +2028,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
     const int u = env_int("FWIFLOW_SHOT_UNROLL", 1);
     d.unroll = u >= 4 ? 4 : (u >= 2 ? 2 : 1);
   }
-  d.shot_group = std::max(0, env_int("FWIFLOW_SHOT_GROUP", 4));
+  d.shot_group = std::max(-1, env_int("FWIFLOW_SHOT_GROUP", -1));
   d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
   d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
   d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;

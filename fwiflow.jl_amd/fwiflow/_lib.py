@@ -31,6 +31,9 @@ EXPORTS = [
     "fwiflow_plan_dims", "fwiflow_plan_set_checkpoint", "fwiflow_plan_set_timing",
     "fwiflow_plan_timing", "fwiflow_plan_device_bytes", "fwiflow_plan_shots_per_launch",
     "fwiflow_plan_set_illumination", "fwiflow_plan_illumination",
+    "fwiflow_cnn_create", "fwiflow_cnn_destroy", "fwiflow_cnn_num_params",
+    "fwiflow_cnn_get_params", "fwiflow_cnn_set_params", "fwiflow_cnn_forward",
+    "fwiflow_cnn_gradients", "fwiflow_cnn_train_step",
 ]
 
 
@@ -82,6 +85,14 @@ def load():
     L.fwiflow_plan_shots_per_launch.argtypes = [vp, C.POINTER(C.c_int)]
     L.fwiflow_plan_set_illumination.argtypes = [vp, C.c_int]
     L.fwiflow_plan_illumination.argtypes = [vp, dp]
+    L.fwiflow_cnn_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint]
+    L.fwiflow_cnn_destroy.argtypes = [vp]
+    L.fwiflow_cnn_num_params.argtypes = [vp, ip]
+    L.fwiflow_cnn_get_params.argtypes = [vp, fp]
+    L.fwiflow_cnn_set_params.argtypes = [vp, fp]
+    L.fwiflow_cnn_forward.argtypes = [vp, fp, fp]
+    L.fwiflow_cnn_gradients.argtypes = [vp, fp, fp, fp, dp, fp]
+    L.fwiflow_cnn_train_step.argtypes = [vp, fp, fp, fp, C.c_float, dp]
     for name in EXPORTS:
         if name not in ("fwiflow_last_error", "fwiflow_version"):
             getattr(L, name).restype = C.c_int

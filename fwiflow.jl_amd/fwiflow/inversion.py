@@ -95,8 +95,11 @@ class PlanObjective:
         P.run(1)
         J = P.misfit()
         gL, gM, gD, _ = P.gradients()
-        g_cp = moduli_to_velocity_grad(cpm, csm, rhom, gL, gM, gD)[0] * self.mask
-        extras = {"illumination": P.illumination()} if self.illumination else {}
+        g_cp, g_cs, g_rho = (g * self.mask for g in
+                             moduli_to_velocity_grad(cpm, csm, rhom, gL, gM, gD))
+        extras = {"gradients": (g_cp, g_cs, g_rho)}
+        if self.illumination:
+            extras["illumination"] = P.illumination()
         return J, g_cp.ravel(), extras
 
 
@@ -111,21 +114,26 @@ class _Log:
             with open(os.path.join(self.dir, name), "a") as f:
                 f.write(json.dumps(rec) + "\n")
 
-    def pair(self, k, shape, grad, illum, update, misfit):
+    def pair(self, k, shape, grad, extras, update, misfit):
         if not self.dir:
             return None
         path = os.path.join(self.dir, "pairs", f"pair_{k:04d}.npz")
         arrays = {"gradient": grad.reshape(shape).astype(np.float32),
                   "update": update.reshape(shape).astype(np.float32),
                   "misfit": np.float64(misfit)}
-        if illum is not None:
-            arrays["illumination"] = np.asarray(illum, dtype=np.float32).reshape(shape)
+        if extras.get("illumination") is not None:
+            arrays["illumination"] = np.asarray(extras["illumination"],
+                                                dtype=np.float32).reshape(shape)
+        if extras.get("gradients") is not None:
+            for name, gq in zip(("gradient_cp", "gradient_cs", "gradient_rho"),
+                                extras["gradients"]):
+                arrays[name] = np.asarray(gq, dtype=np.float32).reshape(shape)
         np.savez(path, **arrays)
         return path
 
 
 def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL,
-          log_dir=None, shape=None, target_misfit=None):
+          log_dir=None, shape=None, target_misfit=None, on_pair=None):
     """Minimise ``objective`` with L-BFGS-B from ``x0``.
 
     ``objective(x) -> (misfit, gradient, extras)`` with ``extras`` optionally
@@ -133,8 +141,10 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
     variable.  With ``log_dir``: ``evals.jsonl`` (one line per evaluation:
     misfit and seconds since the start), ``iters.jsonl`` (one per accepted
     iteration) and ``pairs/pair_NNNN.npz`` (gradient and illumination at the
-    iterate, and the accepted update to the next iterate).  ``target_misfit``
-    stops the run once an accepted iterate reaches it (time-to-misfit runs).
+    iterate, and the accepted update to the next iterate); ``on_pair(gradient,
+    extras, update, misfit)`` receives the same pairs in memory.
+    ``target_misfit`` stops the run once an accepted iterate reaches it
+    (time-to-misfit runs).
 
     Returns a dict: x, misfit, iterations, evaluations, seconds, history
     (per-iteration (seconds, misfit)), scipy's message.
@@ -176,7 +186,9 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
         k = state["iters"]
         t = time.perf_counter() - t0
         Jk = cur[0] if cur else float("nan")
-        log.pair(k, shape, gp, extras.get("illumination"), np.asarray(xk) - prev, Jp)
+        log.pair(k, shape, gp, extras, np.asarray(xk) - prev, Jp)
+        if on_pair is not None:
+            on_pair(gp, extras, np.asarray(xk) - prev, Jp)
         log.line("iters.jsonl", {"iter": k, "misfit": Jk, "seconds": t,
                                  "evals": state["evals"]})
         state["history"].append((t, Jk))

@@ -151,6 +151,34 @@ int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* shots);
 int fwiflow_plan_set_illumination(fwiflow_plan* plan, int enable);
 int fwiflow_plan_illumination(fwiflow_plan* plan, double* illum);
 
+/* ------------------------------------------------------------------------
+ * Learned preconditioner network.  No reference counterpart: FwiFlow's plain
+ * method is L-BFGS on the raw gradient.  A fixed 4-layer 3x3 convolutional
+ * network (4 -> 32 -> 32 -> 32 -> 3 channels, GELU after the first three,
+ * zero padding) on one height x width grid, FP32 weights, BF16 activations.
+ * Feature maps are planar FP32 [channel][height][width], width fastest (the
+ * row-major [NZ][NX] of the operator's arrays, one plane per channel):
+ * input 4 planes (cp, cs, rho gradients and the illumination), output 3
+ * planes (a search direction for cp, cs, rho).  Parameters are one flat FP32
+ * vector, per layer W [Cout][Cin][3][3] then b [Cout].  Training minimises
+ * the mask-weighted mean squared error
+ *   0.5 / (3 * #{mask != 0}) * sum mask^2 (output - target)^2
+ * with Adam (beta 0.9 / 0.999, eps 1e-8). */
+typedef struct fwiflow_cnn fwiflow_cnn;
+int fwiflow_cnn_create(fwiflow_cnn** net, int gpu_id, int height, int width, unsigned seed);
+int fwiflow_cnn_destroy(fwiflow_cnn* net);
+int fwiflow_cnn_num_params(fwiflow_cnn* net, int* n);
+int fwiflow_cnn_get_params(fwiflow_cnn* net, float* params);
+/* also resets the optimiser state */
+int fwiflow_cnn_set_params(fwiflow_cnn* net, const float* params);
+int fwiflow_cnn_forward(fwiflow_cnn* net, const float* input, float* output);
+/* loss and its gradient with respect to the parameters (no update) */
+int fwiflow_cnn_gradients(fwiflow_cnn* net, const float* input, const float* target,
+                          const float* mask, double* loss, float* grad_params);
+/* one Adam step at learning rate lr; loss is the value before the step */
+int fwiflow_cnn_train_step(fwiflow_cnn* net, const float* input, const float* target,
+                           const float* mask, float lr, double* loss);
+
 #ifdef __cplusplus
 }
 #endif

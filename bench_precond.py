@@ -1,0 +1,145 @@
+"""Time-to-misfit of the learned preconditioner against the plain method.
+
+Workload (the learned-preconditioner configuration over the 134x384 model
+family): seeded synthetic models of 12 dipping layers, Vp 1500-4700 m/s with
+a +/-5% smooth perturbation (correlation 15 cells), 24 m cells, 32-cell C-PML
+(224x448 padded), nt 2000, dt 2 ms, 30 shots, 384 receivers, 8 Hz; observed
+data from the true model, inversion of cp from the Gaussian-smoothed model
+with cs and rho fixed (as the reference's plain inversion does).  The top 10
+rows are elastic, not water: with a Vs = 0 layer the operator's adjoint (the
+reference's scheme) grows exponentially over 2000 steps and its gradient is
+not a descent direction, so no optimiser can use it.
+
+1. Plain inversions (L-BFGS-B, bounds 1500-5500, ftol = gtol = 1e-6, the
+   reference's settings) on the training seeds log (gradient, illumination,
+   accepted update) pairs in memory.
+2. The network (HIP kernels) trains on those pairs on the GPU.
+3. On held-out seeds: the plain method runs to its end (maxiter); the learned
+   driver then runs until it reaches the plain method's final misfit.
+   value = median over held-out seeds of (learned wall time / plain wall time);
+   lower is better, the target is <= 0.5.
+
+Prints ONE JSON line; progress goes to stderr.  Defaults are sized to finish
+in about 15 minutes on one MI355X; --train-models 20 --eval-models 5
+--maxiter 100 is the full-size experiment.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fwiflow.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--train-models", type=int, default=10)
+    p.add_argument("--eval-models", type=int, default=3)
+    p.add_argument("--maxiter", type=int, default=60)
+    p.add_argument("--nt", type=int, default=2000)
+    p.add_argument("--epochs", type=int, default=8)
+    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--step0", type=float, default=20.0)
+    p.add_argument("--seed0", type=int, default=100)
+    return p.parse_args()
+
+
+def setup(seed, nt, ws):
+    import fwiflow
+    from fwiflow import models
+
+    c = models.case2(seed=seed, water=False)
+    stf = models.sourceGene(c.f0, nt, c.dt) if nt != c.nSteps else c.stf
+    fwi = fwiflow.make_fwi(c.nz, c.nx, c.dz, c.dx, nt, c.dt, c.src_z, c.src_x, c.rec_z, c.rec_x,
+                           nPml=c.nPml, f0=c.f0, WORKSPACE=ws)
+    fwiflow.compute_observation(fwi, c.cp, c.cs, c.rho, stf)
+    obj = fwiflow.PlanObjective(fwi, c.cs, c.rho, stf, cp_ref=c.cp0)
+    x0 = fwiflow.padding(c.cp0, fwi.nPml, fwi.nPad)
+    return fwi, obj, x0
+
+
+def main():
+    a = parse()
+    import fwiflow
+    from fwiflow import inversion, precond
+
+    t_all = time.perf_counter()
+    mask = None
+    samples = []
+    plain_train = []
+    for k in range(a.train_models):
+        seed = a.seed0 + k
+        ws = tempfile.mkdtemp(prefix=f"fwiflow_pc_{seed}_")
+        fwi, obj, x0 = setup(seed, a.nt, ws)
+        mask = fwi.mask
+
+        def on_pair(g, extras, update, J):
+            x = precond.features(extras["gradients"], extras["illumination"], mask)
+            samples.append((x, precond.target_from_update(update, mask)))
+
+        r = inversion.lbfgs(obj, x0, maxiter=a.maxiter, on_pair=on_pair)
+        obj.close()
+        plain_train.append((seed, r["iterations"], r["evaluations"], r["seconds"]))
+        log(f"train model {seed}: {r['iterations']} iterations, {r['evaluations']} evaluations, "
+            f"{r['seconds']:.1f} s, misfit {r['history'][0][1] if r['history'] else 0:.4g} -> "
+            f"{r['misfit']:.4g}; pairs {len(samples)}")
+    H, W = mask.shape
+    net = precond.CNN(H, W, seed=0)
+    t0 = time.perf_counter()
+    hist = precond.train(net, samples, mask.astype(np.float32), epochs=a.epochs, lr=a.lr,
+                         log=log)
+    t_train = time.perf_counter() - t0
+    log(f"trained on {len(samples)} pairs in {t_train:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    evals = []
+    for k in range(a.eval_models):
+        seed = a.seed0 + 1000 + k
+        ws = tempfile.mkdtemp(prefix=f"fwiflow_pc_{seed}_")
+        fwi, obj, x0 = setup(seed, a.nt, ws)
+        J0 = obj(x0)[0]
+        rp = inversion.lbfgs(obj, x0, maxiter=a.maxiter)
+        rl = precond.learned_descent(obj, x0, net, fwi.mask, maxiter=2 * a.maxiter,
+                                     target_misfit=rp["misfit"], step0=a.step0)
+        obj.close()
+        reached = rl["misfit"] <= rp["misfit"]
+        used = [h[2] for h in rl["history"]]
+        e = {"seed": seed, "misfit_start": J0, "plain_misfit": rp["misfit"],
+             "plain_seconds": round(rp["seconds"], 2), "plain_iterations": rp["iterations"],
+             "plain_evaluations": rp["evaluations"], "learned_misfit": rl["misfit"],
+             "learned_seconds": round(rl["seconds"], 2), "learned_iterations": rl["iterations"],
+             "learned_evaluations": rl["evaluations"], "reached": bool(reached),
+             "learned_steps": used.count("learned"), "fallback_steps": used.count("fallback"),
+             "ratio": (rl["seconds"] / rp["seconds"]) if reached else None}
+        evals.append(e)
+        log("eval", json.dumps(e))
+    ratios = [e["ratio"] for e in evals]
+    value = float(np.median([r if r is not None else np.inf for r in ratios])) if evals else None
+    out = {
+        "metric": "time-to-misfit, learned preconditioner / plain L-BFGS (134x384 model family, "
+                  "elastic top layer)",
+        "value": value if value is None or np.isfinite(value) else None,
+        "unit": "ratio of wall times", "higher_is_better": False,
+        "target": 0.5, "n_gpus": 1, "data": "synthetic (seeded layered models)",
+        "config": {"grid": [int(H), int(W)], "nt": a.nt, "shots": 30, "train_models":
+                   a.train_models, "eval_models": a.eval_models, "maxiter": a.maxiter,
+                   "pairs": len(samples), "epochs": a.epochs, "lr": a.lr},
+        "training": {"seconds": round(t_train, 1), "loss_first_epoch": hist[0],
+                     "loss_last_epoch": hist[-1]},
+        "plain_training_runs": [{"seed": s, "iterations": i, "evaluations": e,
+                                 "seconds": round(t, 1)} for s, i, e, t in plain_train],
+        "eval": evals, "total_seconds": round(time.perf_counter() - t_all, 1),
+    }
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

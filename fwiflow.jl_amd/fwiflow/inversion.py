@@ -133,7 +133,7 @@ class _Log:
 
 
 def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL,
-          log_dir=None, shape=None, target_misfit=None, on_pair=None):
+          log_dir=None, shape=None, target_misfit=None, on_pair=None, first_step_rms=10.0):
     """Minimise ``objective`` with L-BFGS-B from ``x0``.
 
     ``objective(x) -> (misfit, gradient, extras)`` with ``extras`` optionally
@@ -146,12 +146,25 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
     ``target_misfit`` stops the run once an accepted iterate reaches it
     (time-to-misfit runs).
 
+    ``first_step_rms``: L-BFGS-B's first trial step has unit 2-norm in its
+    variables; with cp in m/s over ~1e5 cells that is ~0.005 m/s per cell,
+    below what an FP32 forward model's misfit resolves, and its line search
+    then fails.  The variables are therefore scaled so that a unit step is
+    `first_step_rms` m/s RMS per cell (gtol then applies to the scaled
+    gradient); None runs on cp itself as the reference's
+    ScipyOptimizerInterface does.
+
     Returns a dict: x, misfit, iterations, evaluations, seconds, history
     (per-iteration (seconds, misfit)), scipy's message.
     """
     x0 = np.asarray(x0, dtype=np.float64)
     shape = x0.shape if shape is None else shape
     x0 = x0.ravel()
+    k = 1.0 if first_step_rms is None else 1.0 / (float(first_step_rms) * np.sqrt(x0.size))
+    lo, hi = bounds
+    y0 = x0 * k
+    to_x = lambda y: np.clip(np.asarray(y) / k, lo, hi)  # noqa: E731
+    x0 = to_x(y0)  # the point scipy evaluates first, bit for bit
     log = _Log(log_dir)
     t0 = time.perf_counter()
     cache = {}   # x bytes -> (misfit, gradient, extras) of recent evaluations
@@ -196,13 +209,19 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
         if target_misfit is not None and cur and cur[0] <= target_misfit:
             raise _Stop()
 
-    lo, hi = bounds
+    def fun_y(y):
+        J, g = fun(to_x(y))
+        return J, g / k
+
+    def callback_y(yk):
+        callback(to_x(yk))
+
     res_x, msg = None, ""
     try:
-        r = minimize(fun, x0, jac=True, method="L-BFGS-B",
-                     bounds=[(lo, hi)] * x0.size, callback=callback,
+        r = minimize(fun_y, y0, jac=True, method="L-BFGS-B",
+                     bounds=[(lo * k, hi * k)] * x0.size, callback=callback_y,
                      options={"maxiter": maxiter, "ftol": ftol, "gtol": gtol})
-        res_x, msg = r.x, str(r.message)
+        res_x, msg = to_x(r.x), str(r.message)
     except _Stop:
         res_x, msg = state["x"], "target misfit reached"
     final = cache.get(res_x.tobytes())

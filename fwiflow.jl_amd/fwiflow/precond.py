@@ -178,7 +178,9 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
     step = float(step0)
 
     def try_direction(d):
-        """Backtracking Armijo along unit-RMS direction d; returns the accepted
+        """Armijo backtracking along unit-RMS direction d, each new trial at the
+        minimiser of the quadratic through J(0), its slope and the rejected
+        J(alpha), kept within [0.1, 0.5] alpha; returns the accepted
         (x, J, g, ex, alpha) or None."""
         slope = float(np.dot(g, d))
         if not slope < 0:
@@ -189,7 +191,9 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
             Jn, gn, exn = evaluate(xn)
             if Jn <= J + c_armijo * float(np.dot(g, xn - x)) and Jn < J:
                 return xn, Jn, gn, exn, alpha
-            alpha *= 0.5
+            curv = Jn - J - slope * alpha
+            a_q = -slope * alpha * alpha / (2.0 * curv) if curv > 0 else 0.5 * alpha
+            alpha = min(max(a_q, 0.1 * alpha), 0.5 * alpha)
         return None
 
     for _ in range(maxiter):
@@ -209,7 +213,7 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
         if res is None:
             break
         x, J, g, ex, alpha = res
-        step = alpha * 1.5
+        step = alpha * 2.0
         hist.append((time.perf_counter() - t0, J, used))
         if target_misfit is not None and J <= target_misfit:
             break

@@ -394,3 +394,30 @@ def test_reference_adjoint_is_approximate_in_heterogeneous_media(oracle_mod, wor
     fd = (J(lam0 + h * d) - J(lam0 - h * d)) / (2 * h)
     rel = abs(np.sum(gL * d) - fd) / abs(fd)
     assert 1e-4 < rel < 0.2
+
+
+def test_reference_adjoint_grows_with_a_zero_shear_layer(oracle_mod, workspace):
+    """Property of the restated reference scheme (not of this engine): its
+    hand-derived adjoint applies the material parameters at the updated cell
+    instead of inside the difference stencils, which is not the transpose of
+    the forward operator at a fluid-solid contrast.  With Vs = 0 in the top
+    rows the adjoint field (and the gradient) grows exponentially with the
+    record length, while the same model with an elastic top stays bounded.
+    The engine reproduces this scheme for parity; inversion experiments use
+    models without a Vs = 0 layer."""
+    rms = {}
+    for water in (True, False):
+        for nt in (400, 1200):
+            c = models.case_small(nz=40, nx=60, nSteps=nt, nshots=1)
+            if water:
+                for a in (c.cs, c.cs0):
+                    a[:5] = 0.0
+            ws = os.path.join(workspace, f"w{int(water)}_{nt}")
+            para, stf = models.write_case(c, ws)
+            lam, mu, den = models.moduli(c, "true")
+            oracle_mod.cufd(lam, mu, den, stf, 2, [0], para, precision=64)
+            l0, m0, d0 = models.moduli(c, "start")
+            gL = oracle_mod.cufd(l0, m0, d0, stf, 1, [0], para, precision=64)[1]
+            rms[(water, nt)] = float(np.sqrt(np.mean(gL ** 2)))
+    assert rms[(True, 1200)] > 1e3 * rms[(True, 400)], rms
+    assert rms[(False, 1200)] < 1e2 * rms[(False, 400)], rms

@@ -1639,40 +1639,60 @@ __global__ __launch_bounds__(256) void k_ckpt(Tiling tl, Geo g, Wave w, float* _
 }
 
 // Backward kernel pairs with disjoint reads and writes, launched as one grid
-// so they share a launch and its ramp-up and drain: workgroups [0, nA) run
-// the first kernel's tiles, the rest the second's.  nA is a multiple of 8, so
-// both parts see the same workgroup-to-XCD dealing.
+// so they share a launch and its ramp-up and drain.  The grid holds nA
+// workgroups of the first kernel and nB of the second (both multiples of 8,
+// so both parts see the same workgroup-to-XCD dealing).  With `mix` set,
+// rounds of 8 workgroups (one per XCD) alternate between the parts in
+// proportion to their sizes, so both sweep their tiles at the same pace and
+// the model parameters and shared adjoint values that both read are fetched
+// close together in time, one L2 fill serving both; otherwise the first nA
+// workgroups run the first kernel.
 //   pair a: rev_stress(it) + adj_vel(it) (rev_stress reads the adjoint stress
 //           that adj_vel also only reads; adj_vel writes the adjoint velocity
 //           that rev_stress does not touch)
 //   pair b: adj_stress(it) + rev_vel(it-1) (rev_vel reads the adjoint velocity
 //           that adj_stress also only reads; adj_stress writes the adjoint
 //           stress that rev_vel does not touch)
+__device__ inline bool pair_part(int b, int nA, int nB, int mix, int& local) {
+  if (!mix) {
+    local = b < nA ? b : b - nA;
+    return b < nA;
+  }
+  const long long RA = nA / 8, R = RA + nB / 8, r = b / 8;
+  const int before = (int)(r * RA / R);  // first-part rounds before round r
+  if ((int)((r + 1) * RA / R) > before) {
+    local = before * 8 + b % 8;
+    return true;
+  }
+  local = (int)(r - before) * 8 + b % 8;
+  return false;
+}
+
 template <int U>
-__global__ __launch_bounds__(256) void k_bwd_pair_a(int nA, Tiling tR, Tiling tU, Geo g, Mdl m, Pml c,
-                                                   Wave w, Wave a, Mem p, Src src,
-                                                   float* __restrict__ acc,
+__global__ __launch_bounds__(256) void k_bwd_pair_a(int nA, int nB, int mix, Tiling tR, Tiling tU,
+                                                   Geo g, Mdl m, Pml c, Wave w, Wave a, Mem p,
+                                                   Src src, float* __restrict__ acc,
                                                    const float* __restrict__ ring,
                                                    float* __restrict__ stf_grad, int it) {
-  const int b = blockIdx.x;
-  if (b < nA) {
+  int b;
+  if (pair_part(blockIdx.x, nA, nB, mix, b)) {
     if (b < tR.n)
       k_rev_stress_body<U>(b, tR, g, m, w, a, src, acc, acc + g.N, acc + 2 * g.N, ring, it);
-  } else {
-    k_adj_vel_body<U>(b - nA, tU, g, m, c, a, p, src, stf_grad, it);
+  } else if (b < tU.n * tU.chunks) {
+    k_adj_vel_body<U>(b, tU, g, m, c, a, p, src, stf_grad, it);
   }
 }
 
 template <int U>
-__global__ __launch_bounds__(256) void k_bwd_pair_b(int nA, Tiling tU, Tiling tR, Geo g, Mdl m, Pml c,
-                                                   Wave w, Wave a, Mem p,
+__global__ __launch_bounds__(256) void k_bwd_pair_b(int nA, int nB, int mix, Tiling tU, Tiling tR,
+                                                   Geo g, Mdl m, Pml c, Wave w, Wave a, Mem p,
                                                    float* __restrict__ acc,
                                                    const float* __restrict__ ring, int it_rev) {
-  const int b = blockIdx.x;
-  if (b < nA) {
+  int b;
+  if (pair_part(blockIdx.x, nA, nB, mix, b)) {
     if (b < tU.n * tU.chunks) k_adj_stress_body<U>(b, tU, g, m, c, a, p);
-  } else {
-    k_rev_vel_body<U>(b - nA, tR, g, m, w, a, acc + 3 * g.N, acc + 4 * g.N, ring, it_rev);
+  } else if (b < tR.n) {
+    k_rev_vel_body<U>(b, tR, g, m, w, a, acc + 3 * g.N, acc + 4 * g.N, ring, it_rev);
   }
 }
 
@@ -1846,6 +1866,7 @@ struct DeviceState {
   int shot_group = -1;    // shots per launch of the split/tiled kernels (0: all, -1: auto)
   int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
   bool pairs = true;      // backward kernel pairs share a launch
+  bool mix = true;        // ... with their workgroups interleaved round by round
   bool balance = true;    // C-PML columns shared evenly by the XCDs (update-region launches)
   int* perm_u = nullptr;  // workgroup -> tile order of the update-region tiling
   bool illum_on = false;  // sum the forward illumination
@@ -2031,6 +2052,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.shot_group = std::max(-1, env_int("FWIFLOW_SHOT_GROUP", -1));
   d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
   d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
+  d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
   d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
@@ -2546,24 +2568,24 @@ Status Plan::run(int calc_id) {
         LAUNCH_U(k_adj_vel, grdUg, blk, tUg, gg, m, c, a, pg, sg, sgrad, -1);
         inject(g.nSteps - 1);
         if (d.pairs && g.nSteps >= 2) {
-          const int nUg = tUg.n * tUg.chunks;
-          const int nRa = (tR.n + 7) / 8 * 8, nUb = (nUg + 7) / 8 * 8;
-          const dim3 grdA(nRa + nUg), grdB(nUb + tR.n);
+          const int nU8 = (tUg.n * tUg.chunks + 7) / 8 * 8, nR8 = (tR.n + 7) / 8 * 8;
+          const dim3 grdP(nR8 + nU8);
+          const int mix = d.mix ? 1 : 0;
           // adj_stress(nSteps-1) with rev_vel(nSteps-2)
-          LAUNCH_U(k_bwd_pair_b, grdB, blk, nUb, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
+          LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
                    g.nSteps - 2);
           for (int it = g.nSteps - 2; it >= 0; it--) {
             const bool smp = (it % sample_every) == 0;
             d.launch("bwd_pair_a", smp, [&] {
-              LAUNCH_U(k_bwd_pair_a, grdA, blk, nRa, tR, tUg, gg, m, c, wg, a, pg, sg, acc, ring,
-                       sgrad, it);
+              LAUNCH_U(k_bwd_pair_a, grdP, blk, nR8, nU8, mix, tR, tUg, gg, m, c, wg, a, pg, sg, acc,
+                       ring, sgrad, it);
             });
             inject(it);
             reset(it);
             if (it >= 1) {
               d.launch("bwd_pair_b", smp, [&] {
-                LAUNCH_U(k_bwd_pair_b, grdB, blk, nUb, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
-                         it - 1);
+                LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix, tUg, tR, gg, m, c, wg, a, pg, acc,
+                         ring, it - 1);
               });
             } else {
               LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg);

@@ -28,6 +28,7 @@ from __future__ import annotations
 import json
 import os
 import time
+from collections import OrderedDict
 
 import numpy as np
 from scipy.optimize import minimize
@@ -103,6 +104,25 @@ class PlanObjective:
         return J, g_cp.ravel(), extras
 
 
+class _EvalCache:
+    """The last `size` evaluations by point (least recently stored first out);
+    storing a point again refreshes it."""
+
+    def __init__(self, size=8):
+        self.size = size
+        self.d = OrderedDict()
+
+    def put(self, x, value):
+        key = np.asarray(x).tobytes()
+        self.d[key] = value
+        self.d.move_to_end(key)
+        while len(self.d) > self.size:
+            self.d.popitem(last=False)
+
+    def get(self, x):
+        return self.d.get(np.asarray(x).tobytes())
+
+
 class _Log:
     def __init__(self, log_dir):
         self.dir = log_dir
@@ -167,19 +187,14 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
     x0 = to_x(y0)  # the point scipy evaluates first, bit for bit
     log = _Log(log_dir)
     t0 = time.perf_counter()
-    cache = {}   # x bytes -> (misfit, gradient, extras) of recent evaluations
-    order = []
+    cache = _EvalCache()  # (misfit, gradient, extras) of recent evaluations
     state = {"evals": 0, "iters": 0, "x": x0.copy(), "history": []}
 
     def fun(x):
         J, g, extras = objective(x)
         g = np.asarray(g, dtype=np.float64).ravel()
         state["evals"] += 1
-        key = x.tobytes()
-        cache[key] = (float(J), g.copy(), extras)
-        order.append(key)
-        while len(order) > 8:
-            cache.pop(order.pop(0), None)
+        cache.put(x, (float(J), g.copy(), extras))
         log.line("evals.jsonl", {"eval": state["evals"], "misfit": float(J),
                                  "seconds": time.perf_counter() - t0})
         return float(J), g
@@ -189,12 +204,12 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
 
     def callback(xk):
         prev = state["x"]
-        ent = cache.get(prev.tobytes())
+        ent = cache.get(prev)
         if ent is None:  # the previous iterate was evicted: evaluate it again
-            J, g = fun(prev)
-            ent = cache[prev.tobytes()]
+            fun(prev)
+            ent = cache.get(prev)
         Jp, gp, extras = ent
-        cur = cache.get(np.asarray(xk).tobytes())
+        cur = cache.get(xk)
         state["iters"] += 1
         k = state["iters"]
         t = time.perf_counter() - t0
@@ -224,7 +239,7 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
         res_x, msg = to_x(r.x), str(r.message)
     except _Stop:
         res_x, msg = state["x"], "target misfit reached"
-    final = cache.get(res_x.tobytes())
+    final = cache.get(res_x)
     J = final[0] if final else fun(res_x)[0]
     return {"x": res_x.reshape(shape), "misfit": J, "iterations": state["iters"],
             "evaluations": state["evals"], "seconds": time.perf_counter() - t0,

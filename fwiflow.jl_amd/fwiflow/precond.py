@@ -210,6 +210,9 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
             dp /= _rms(dp.reshape(shape), np.asarray(mask))
             used = "fallback"
             res = try_direction(dp)
+        if res is None and step > 1e-3 * step0:
+            step *= 0.05  # both directions rejected at this scale: retry much shorter
+            continue
         if res is None:
             break
         x, J, g, ex, alpha = res

@@ -379,10 +379,11 @@ def _run_plan(para, ids, lam, mu, den, stf, env):
                                      {"FWIFLOW_SHOT_GROUP": "1", "FWIFLOW_TILED": "1"},
                                      {"FWIFLOW_SHOTS_PER_THREAD": "2"},
                                      {"FWIFLOW_BWD_PAIRS": "0"},
-                                     {"FWIFLOW_XCD_BALANCE": "0", "FWIFLOW_TILE_X": "2"}],
+                                     {"FWIFLOW_XCD_BALANCE": "0", "FWIFLOW_TILE_X": "2"},
+                                     {"FWIFLOW_PAIR_MIX": "0"}],
                          ids=["column7", "column32", "tiled", "group_all_unroll2",
                               "group2_unroll4", "group1_tiled", "chunks_of_2", "unpaired",
-                              "unbalanced_tile_x2"])
+                              "unbalanced_tile_x2", "pair_halves_in_order"])
 def test_fused_kernels_match_split_kernels(workspace, variant):
     """The fused kernels (column-marching fused forward/reverse/adjoint steps
     with odd or even segment widths, or the tiled fused forward step; all
@@ -401,7 +402,8 @@ def test_fused_kernels_match_split_kernels(workspace, variant):
     d0, g0, J0 = _run_plan(para, ids, lam0, mu0, den0, stf,
                            {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "0", "FWIFLOW_SHOT_GROUP": "4",
                             "FWIFLOW_SHOT_UNROLL": "1", "FWIFLOW_SHOTS_PER_THREAD": "0",
-                            "FWIFLOW_BWD_PAIRS": "1", "FWIFLOW_XCD_BALANCE": "1"})
+                            "FWIFLOW_BWD_PAIRS": "1", "FWIFLOW_XCD_BALANCE": "1",
+                            "FWIFLOW_PAIR_MIX": "1"})
     d1, g1, J1 = _run_plan(para, ids, lam0, mu0, den0, stf, variant)
     assert rel(d1, d0) < 1e-6
     assert abs(J1 - J0) <= 1e-6 * abs(J0)

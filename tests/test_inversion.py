@@ -81,3 +81,19 @@ def test_target_misfit_stops_early(tmp_path):
 def test_reference_defaults():
     assert inversion.CP_BOUNDS == (1500.0, 5500.0)
     assert (inversion.MAXITER, inversion.FTOL, inversion.GTOL) == (100, 1e-6, 1e-6)
+
+
+def test_evaluation_cache_keeps_a_restored_point():
+    """Regression: a point stored again after other points must survive the
+    eviction of its earlier entry (the driver's callback re-evaluates an
+    evicted iterate and then reads it back)."""
+    c = inversion._EvalCache(size=3)
+    pts = [np.full(4, float(k)) for k in range(6)]
+    for k in range(3):
+        c.put(pts[k], k)
+    c.put(pts[0], "again")  # refresh point 0
+    c.put(pts[3], 3)        # evicts point 1, not point 0
+    assert c.get(pts[0]) == "again" and c.get(pts[1]) is None
+    c.put(pts[4], 4)
+    c.put(pts[5], 5)        # evicts 2 then 0
+    assert c.get(pts[0]) is None and c.get(pts[5]) == 5

@@ -1866,7 +1866,8 @@ struct DeviceState {
   int shot_group = -1;    // shots per launch of the split/tiled kernels (0: all, -1: auto)
   int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
   bool pairs = true;      // backward kernel pairs share a launch
-  bool mix = true;        // ... with their workgroups interleaved round by round
+  bool mix = true;        // ... with their workgroups interleaved round by round (pair a;
+  bool mix_b = false;     // pair b measured slower interleaved: 55.1 vs 53.4 us)
   bool balance = true;    // C-PML columns shared evenly by the XCDs (update-region launches)
   int* perm_u = nullptr;  // workgroup -> tile order of the update-region tiling
   bool illum_on = false;  // sum the forward illumination
@@ -2053,6 +2054,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
   d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
   d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
+  d.mix_b = env_int("FWIFLOW_PAIR_MIX_B", 0) != 0;
   d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
@@ -2570,9 +2572,9 @@ Status Plan::run(int calc_id) {
         if (d.pairs && g.nSteps >= 2) {
           const int nU8 = (tUg.n * tUg.chunks + 7) / 8 * 8, nR8 = (tR.n + 7) / 8 * 8;
           const dim3 grdP(nR8 + nU8);
-          const int mix = d.mix ? 1 : 0;
+          const int mix = d.mix ? 1 : 0, mix_b = d.mix_b ? 1 : 0;
           // adj_stress(nSteps-1) with rev_vel(nSteps-2)
-          LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
+          LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix_b, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
                    g.nSteps - 2);
           for (int it = g.nSteps - 2; it >= 0; it--) {
             const bool smp = (it % sample_every) == 0;
@@ -2584,7 +2586,7 @@ Status Plan::run(int calc_id) {
             reset(it);
             if (it >= 1) {
               d.launch("bwd_pair_b", smp, [&] {
-                LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix, tUg, tR, gg, m, c, wg, a, pg, acc,
+                LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix_b, tUg, tR, gg, m, c, wg, a, pg, acc,
                          ring, it - 1);
               });
             } else {

@@ -380,7 +380,7 @@ def _run_plan(para, ids, lam, mu, den, stf, env):
                                      {"FWIFLOW_SHOTS_PER_THREAD": "2"},
                                      {"FWIFLOW_BWD_PAIRS": "0"},
                                      {"FWIFLOW_XCD_BALANCE": "0", "FWIFLOW_TILE_X": "2"},
-                                     {"FWIFLOW_PAIR_MIX": "0"}],
+                                     {"FWIFLOW_PAIR_MIX": "0", "FWIFLOW_PAIR_MIX_B": "1"}],
                          ids=["column7", "column32", "tiled", "group_all_unroll2",
                               "group2_unroll4", "group1_tiled", "chunks_of_2", "unpaired",
                               "unbalanced_tile_x2", "pair_halves_in_order"])
@@ -403,7 +403,7 @@ def test_fused_kernels_match_split_kernels(workspace, variant):
                            {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "0", "FWIFLOW_SHOT_GROUP": "4",
                             "FWIFLOW_SHOT_UNROLL": "1", "FWIFLOW_SHOTS_PER_THREAD": "0",
                             "FWIFLOW_BWD_PAIRS": "1", "FWIFLOW_XCD_BALANCE": "1",
-                            "FWIFLOW_PAIR_MIX": "1"})
+                            "FWIFLOW_PAIR_MIX": "1", "FWIFLOW_PAIR_MIX_B": "0"})
     d1, g1, J1 = _run_plan(para, ids, lam0, mu0, den0, stf, variant)
     assert rel(d1, d0) < 1e-6
     assert abs(J1 - J0) <= 1e-6 * abs(J0)

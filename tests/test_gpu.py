@@ -98,6 +98,31 @@ def test_config1_parity(oracle_mod, workspace):
     _assert(_parity(oracle_mod, models.case1(), workspace))
 
 
+@pytest.mark.parametrize("grid,nt", [((200, 600), 1000), ((300, 900), 300)],
+                         ids=["200x600", "300x900"])
+def test_config4_time_lapse_parity(oracle_mod, workspace, grid, nt):
+    """Time-lapse surveys on the 12 m grids (32-cell C-PML, dt 1.5 ms, 10 Hz):
+    baseline and monitor (-6% Vp / +2% density ellipse) each against the
+    oracle, an edge shot and a central one (0 and 8 of 16) over the first nt of
+    3000 steps.  The monitor's data equal the baseline's until waves that met
+    the ellipse come back (checked on the first 300 steps); for the central
+    shot, 1000 steps on the 200x600 grid are long enough for the ellipse's
+    reflection (its top is 1.3 km deep) to return and the data to differ."""
+    nz, nx = grid
+    ids = [0, 8]
+    data = {}
+    for monitor in (False, True):
+        c = models.case4(nz, nx, monitor=monitor, nSteps=nt, nshots=16)
+        ws = os.path.join(workspace, "mon" if monitor else "base")
+        _assert(_parity(oracle_mod, c, ws, ids=ids, check_fp64=False))
+        data[monitor] = _obs_files(ws, ids, len(c.rec_x), c.nSteps)
+    for b, m in zip(data[False], data[True]):
+        assert np.array_equal(b[:, :300], m[:, :300])
+    if nt >= 1000:
+        b, m = data[False][1], data[True][1]
+        assert rel(m, b) > 1e-4  # the oracle gives 3.6e-4, from sample 591 on
+
+
 def test_docs_marmousi_parity(oracle_mod, workspace):
     """The reference's tutorial setup (docs/src/tutorials/fwi.md:12-43): the
     Marmousi Vp of docs/data (interior of the 224x448 file, re-padded),

@@ -21,7 +21,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HOST_SRC = ["json.cpp", "config.cpp", "capi.cpp"]
-HIP_SRC = ["engine.hip", "cnn.hip"]
+HIP_SRC = ["engine.hip", "cnn.hip", "cond.hip"]
 
 CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 # packed-FP32 instructions are disabled for all device code.  Multiply-add

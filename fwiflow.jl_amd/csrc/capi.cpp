@@ -86,6 +86,7 @@ Status cufd_impl(double* misfit, double* gL, double* gM, double* gD, double* gS,
   TRY(plan.run(calc_id));
   if (calc_id == 0) TRY(plan.misfit(misfit));
   if (calc_id == 1) TRY(plan.gradients(gL, gM, gD, gS));
+  if (calc_id == 1 && plan.para.save_scratch) TRY(plan.write_scratch());
   if (calc_id == 2) TRY(plan.write_data());
   return Status::Ok();
 }
@@ -176,6 +177,8 @@ int fwiflow_plan_gradients(fwiflow_plan* plan, double* grad_Lambda, double* grad
 }
 int fwiflow_plan_data(fwiflow_plan* plan, float* syn) { PLAN_CALL(plan->impl.data(syn)); }
 int fwiflow_plan_write_data(fwiflow_plan* plan) { PLAN_CALL(plan->impl.write_data()); }
+int fwiflow_plan_residual(fwiflow_plan* plan, float* res) { PLAN_CALL(plan->impl.residual(res)); }
+int fwiflow_plan_source(fwiflow_plan* plan, float* rows) { PLAN_CALL(plan->impl.source(rows)); }
 int fwiflow_plan_dims(fwiflow_plan* plan, int* dims) {
   if (!plan || !dims) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
   const Plan& p = plan->impl;

@@ -1,5 +1,6 @@
 #include "config.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 
@@ -52,6 +53,40 @@ Status int_array(const json::Value& o, const char* key, std::vector<int>& out,
   return Status::Ok();
 }
 
+Status num_array(const json::Value& o, const char* key, std::vector<float>& out,
+                 const std::string& where) {
+  const json::Value* v = o.get(key);
+  if (!v) return Status::Err(FWIFLOW_ERR_JSON, where + ": missing key \"" + key + "\"");
+  if (v->kind != json::Value::Array)
+    return Status::Err(FWIFLOW_ERR_JSON, where + ": key \"" + key + "\" is not an array");
+  out.clear();
+  for (const auto& e : v->arr) {
+    if (e.kind != json::Value::Number)
+      return Status::Err(FWIFLOW_ERR_JSON, where + ": \"" + key + "\" holds a non-number");
+    out.push_back((float)e.num);  // Src_Rec.cu:164,172,190: GetDouble() into float
+  }
+  return Status::Ok();
+}
+
+Status need_bool(const json::Value& o, const char* key, bool& out, const std::string& file) {
+  const json::Value* v = o.get(key);
+  if (!v) return Status::Ok();
+  if (v->kind != json::Value::Bool)
+    return Status::Err(FWIFLOW_ERR_JSON, file + ": \"" + key + "\" is not a bool");
+  out = v->b;
+  return Status::Ok();
+}
+
+int largest_prime_factor(int n) {
+  int best = 1;
+  for (int p = 2; (long long)p * p <= n; p++)
+    while (n % p == 0) {
+      best = p;
+      n /= p;
+    }
+  return n > 1 ? std::max(best, n) : best;
+}
+
 #define TRY(expr)                  \
   do {                             \
     Status _s = (expr);            \
@@ -102,22 +137,28 @@ Status parse_para(const std::string& fname, Para& p) {
     p.scratch_dir = s->str;
     p.save_scratch = true;
   }
-  if (const json::Value* w = j.get("if_win")) {
-    if (w->kind != json::Value::Bool)
-      return Status::Err(FWIFLOW_ERR_JSON, fname + ": \"if_win\" is not a bool");
-    if (w->b)
+  TRY(need_bool(j, "if_win", p.if_win, fname));
+  TRY(need_bool(j, "if_src_update", p.if_src_update, fname));
+  if (const json::Value* f = j.get("filter")) {
+    // Parameter.cpp:155-167: any "filter" key switches the band-pass on
+    std::vector<float> v;
+    TRY(num_array(j, "filter", v, fname));
+    if (v.size() != 4)
+      return Status::Err(FWIFLOW_ERR_JSON, fname + ": \"filter\" needs 4 frequencies");
+    for (int q = 0; q < 4; q++) p.filter[q] = v[q];
+    p.if_filter = true;
+    (void)f;
+  }
+  TRY(check_geometry(p));
+  if (p.if_filter || p.if_src_update) {
+    const int L = 2 * p.nSteps;
+    if (L > kMaxTransform || largest_prime_factor(L) > kMaxRadix)
       return Status::Err(FWIFLOW_ERR_UNSUPPORTED,
-                         "per-trace windows (\"if_win\": true) are not implemented");
+                         "band-pass filter / source update: 2*nSteps = " + std::to_string(L) +
+                             " must be <= " + std::to_string(kMaxTransform) +
+                             " with prime factors <= " + std::to_string(kMaxRadix));
   }
-  if (j.get("filter"))
-    return Status::Err(FWIFLOW_ERR_UNSUPPORTED, "band-pass \"filter\" is not implemented");
-  if (const json::Value* u = j.get("if_src_update")) {
-    if (u->kind != json::Value::Bool)
-      return Status::Err(FWIFLOW_ERR_JSON, fname + ": \"if_src_update\" is not a bool");
-    if (u->b)
-      return Status::Err(FWIFLOW_ERR_UNSUPPORTED, "\"if_src_update\" is not implemented");
-  }
-  return check_geometry(p);
+  return Status::Ok();
 }
 
 Status check_geometry(const Para& p) {
@@ -167,6 +208,15 @@ Status parse_survey(const Para& p, int group_size, const int* shot_ids, Survey& 
     TRY(int_array(*o, "x_rec", s.x_rec, key));
     if (nrec <= 0 || (int)s.z_rec.size() != nrec || (int)s.x_rec.size() != nrec)
       return Status::Err(FWIFLOW_ERR_SHAPE, key + ": nrec does not match z_rec/x_rec sizes");
+    if (p.if_win) {
+      TRY(num_array(*o, "win_start", s.win_start, key));
+      TRY(num_array(*o, "win_end", s.win_end, key));
+      TRY(num_array(*o, "weights", s.weights, key));
+      if ((int)s.win_start.size() != nrec || (int)s.win_end.size() != nrec ||
+          (int)s.weights.size() != nrec)
+        return Status::Err(FWIFLOW_ERR_SHAPE,
+                           key + ": win_start/win_end/weights need nrec values each");
+    }
     s.z_src += p.nPml;
     s.x_src += p.nPml;
     if (s.z_src < zlo || s.z_src > zhi || s.x_src < xlo || s.x_src > xhi)

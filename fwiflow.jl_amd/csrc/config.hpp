@@ -30,14 +30,26 @@ struct Para {
   float dz = 0, dx = 0, dt = 0, f0 = 0;
   std::string survey_fname, data_dir, scratch_dir;
   bool save_scratch = false;
+  // optional residual stages (Parameter.cpp:146-174), off by default
+  bool if_win = false, if_filter = false, if_src_update = false;
+  float filter[4] = {0, 0, 0, 0};  // Hz: taper-in start/end, taper-out start/end
   int nzA() const { return nz - nPad; }  // rows the stencils touch
+  bool conditioned() const { return if_win || if_filter || if_src_update; }
 };
+
+// Longest trace transform of the band-pass filter and the source update:
+// 2*nSteps samples, held in the 160 KB of LDS of one workgroup.
+constexpr int kMaxTransform = 20480;
+// Largest prime factor of 2*nSteps those transforms handle.
+constexpr int kMaxRadix = 61;
 
 // One shot of the group, indices already offset by nPml (Src_Rec.cu:86,90,105,113).
 struct Shot {
   int id = 0;
   int z_src = 0, x_src = 0;
   std::vector<int> z_rec, x_rec;
+  // per-trace windows (s) and weights, "if_win" only (Src_Rec.cu:156-199)
+  std::vector<float> win_start, win_end, weights;
 };
 
 struct Survey {

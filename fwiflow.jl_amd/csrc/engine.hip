@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "cond.hpp"
 #include "engine.hpp"
 
 namespace fwiflow {
@@ -47,6 +48,7 @@ namespace fwiflow {
 namespace {
 
 constexpr int kBZ = 64;  // threads along z (one wave64 row)
+constexpr int kCondBlocks = 64;  // misfit partial sums per shot of the conditioned residual
 constexpr float kC1 = 9.0f / 8.0f;
 constexpr float kC2 = 1.0f / 24.0f;
 
@@ -1877,6 +1879,25 @@ struct DeviceState {
   int* perm_u = nullptr;  // workgroup -> tile order of the update-region tiling
   bool illum_on = false;  // sum the forward illumination
   bool illum_valid = false;
+  // residual conditioning, shot by shot: the optional stages, and the path
+  // that keeps the conditioned data for the scratch files
+  bool cond = false;
+  FftDesc fft{};
+  int* fperm = nullptr;                    // [2*nSteps] digit-reversal permutation
+  float2* ftw = nullptr;                   // [2*nSteps] twiddles
+  float2* bpH = nullptr;                   // [nSteps+1] band-pass amp^2
+  float* wpad = nullptr;                   // [2*nSteps] window ratio 0.01 over the padded length
+  float *wst = nullptr, *wen = nullptr, *wwt = nullptr;  // [R] per-trace windows and weights
+  float *cobs = nullptr, *csyn = nullptr;  // [nSteps][nrec of one shot] conditioned data
+  float2 *spo = nullptr, *spc = nullptr;   // [nrec][nSteps+1] spectra (source update)
+  float2* coef = nullptr;                  // [nSteps+1] source-update filter of one shot
+  float* srcamp = nullptr;                 // [ns][nSteps] source rows as injected (stf * window)
+  float* srcout = nullptr;                 // [ns][nSteps] updated source rows
+  float* inj_rev = nullptr;                // [ns][nSteps] increments of the reversal
+  double* cpart = nullptr;                 // [ns][kCondBlocks] misfit partial sums
+  int nrec_max = 0;
+  bool src_updated = false;                // the last run updated the source rows
+  std::vector<float> keep_obs, keep_syn;   // conditioned data [R][nSteps] for the scratch files
   float* illum = nullptr; // [NX][NZ]
   std::vector<int> roff;  // first receiver of each shot (+ total)
   std::vector<void*> allocs;
@@ -2026,6 +2047,9 @@ struct DeviceState {
     return std::max(1, std::min(n, g.ns));
   }
   Src src_view(int s0) const { return Src{src + 2 * s0, inj + (size_t)s0 * g.nSteps}; }
+  Src src_view(int s0, const float* rows) const {
+    return Src{src + 2 * s0, rows + (size_t)s0 * g.nSteps};
+  }
   float* ring_view(int) const { return ring; }  // ring_ptr adds the group's first shot
 };
 
@@ -2176,6 +2200,55 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
     d.fused = false;
     d.fused_rev = false;
   }
+  TRY(d.alloc(&d.srcamp, (size_t)g.ns * g.nSteps));
+  d.cond = para.conditioned() || para.save_scratch;
+  if (d.cond) {
+    const int nt = g.nSteps, L = 2 * nt;
+    for (int s = 0; s < g.ns; s++) d.nrec_max = std::max(d.nrec_max, d.roff[s + 1] - d.roff[s]);
+    TRY(d.alloc(&d.cobs, (size_t)nt * d.nrec_max));
+    TRY(d.alloc(&d.csyn, (size_t)nt * d.nrec_max));
+    TRY(d.alloc(&d.cpart, (size_t)g.ns * kCondBlocks));
+    if (para.if_win) {
+      std::vector<float> a, b, c;
+      for (const Shot& sh : survey.shots) {
+        a.insert(a.end(), sh.win_start.begin(), sh.win_start.end());
+        b.insert(b.end(), sh.win_end.begin(), sh.win_end.end());
+        c.insert(c.end(), sh.weights.begin(), sh.weights.end());
+      }
+      TRY(d.alloc(&d.wst, a.size()));
+      TRY(d.alloc(&d.wen, b.size()));
+      TRY(d.alloc(&d.wwt, c.size()));
+      HIP_TRY(hipMemcpy(d.wst, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d.wen, b.data(), b.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d.wwt, c.data(), c.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (para.if_filter || para.if_src_update) {
+      std::vector<int> perm;
+      std::vector<float2> tw;
+      TRY(fft_plan(L, d.fft, perm, tw));
+      TRY(d.alloc(&d.fperm, perm.size()));
+      TRY(d.alloc(&d.ftw, tw.size()));
+      HIP_TRY(hipMemcpy(d.fperm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d.ftw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    if (para.if_filter) {
+      std::vector<float2> H;
+      bandpass_table(nt, para.dt, para.filter, H);
+      TRY(d.alloc(&d.bpH, H.size()));
+      HIP_TRY(hipMemcpy(d.bpH, H.data(), H.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    if (para.if_src_update) {
+      std::vector<float> wp(L);
+      window_squared(L, para.dt, 0.01f, wp.data());
+      TRY(d.alloc(&d.wpad, wp.size()));
+      HIP_TRY(hipMemcpy(d.wpad, wp.data(), wp.size() * sizeof(float), hipMemcpyHostToDevice));
+      TRY(d.alloc(&d.spo, (size_t)d.nrec_max * (nt + 1)));
+      TRY(d.alloc(&d.spc, (size_t)d.nrec_max * (nt + 1)));
+      TRY(d.alloc(&d.coef, (size_t)nt + 1));
+      TRY(d.alloc(&d.srcout, (size_t)g.ns * nt));
+      TRY(d.alloc(&d.inj_rev, (size_t)g.ns * nt));
+    }
+  }
   const size_t T = (size_t)g.nSteps * d.R;
   TRY(d.alloc(&d.syn, T));
   d.npart = 1024;
@@ -2212,17 +2285,19 @@ Status Plan::set_stf(const double* stf) {
   const int nSteps = d.g.nSteps;
   // Src_Rec.cu:132-143: row shot_id -> float -> window ratio 0.001; the
   // increment added to szz is 1500^2 * stf * dt (add_source, utilities.cu:528-538)
-  std::vector<float> w2(nSteps), inj((size_t)d.g.ns * nSteps);
+  std::vector<float> w2(nSteps), inj((size_t)d.g.ns * nSteps), amps(inj.size());
   window_squared(nSteps, para.dt, 0.001f, w2.data());
   for (int s = 0; s < d.g.ns; s++) {
     const double* row = stf + (size_t)survey.shots[s].id * nSteps;
     for (int it = 0; it < nSteps; it++) {
       const float amp = (float)row[it] * w2[it];
+      amps[(size_t)s * nSteps + it] = amp;
       inj[(size_t)s * nSteps + it] = (float)(2250000.0 * (double)amp * (double)para.dt);
     }
   }
   HIP_TRY(hipSetDevice(d.dev));
   HIP_TRY(hipMemcpy(d.inj, inj.data(), inj.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d.srcamp, amps.data(), amps.size() * sizeof(float), hipMemcpyHostToDevice));
   d.stf_set = true;
   return Status::Ok();
 }
@@ -2300,6 +2375,76 @@ Status Plan::timing(const std::string& name, double* ms, long long* count) {
 long long Plan::device_bytes() const { return d_ ? d_->bytes : 0; }
 
 int Plan::shots_per_launch() const { return d_ ? d_->group_size() : 0; }
+
+// Residual with the optional conditioning stages, one shot at a time
+// (libCUFD.cu:254-313): window obs and syn (per-trace windows and weights, or
+// the default ratio-0.005 window), band-pass both, source update (syn and the
+// source row replaced), res = obs - syn and its misfit, then the adjoint
+// stages on res: source-update filter, band-pass, window (ratio 0.1 with
+// per-trace windows).  `keep` copies the conditioned obs and syn to the host
+// for the scratch files.
+static Status condition_residual(DeviceState& d, const Para& para, bool keep) {
+  const Geo& g = d.g;
+  const int nt = g.nSteps, nf = nt + 1, kc = touched_bins(nt);
+  const float scale = 1.0f / (float)(2 * nt);
+  const float dt = para.dt;
+  hipStream_t st = d.stream;
+  if (keep) {
+    d.keep_obs.assign((size_t)d.R * nt, 0.f);
+    d.keep_syn.assign((size_t)d.R * nt, 0.f);
+  }
+  std::vector<float> h;
+  for (int s = 0; s < g.ns; s++) {
+    const int r0 = d.roff[s], n = d.roff[s + 1] - r0;
+    const float* ws = para.if_win ? d.wst + r0 : nullptr;
+    const float* we = para.if_win ? d.wen + r0 : nullptr;
+    const float* wt = para.if_win ? d.wwt + r0 : nullptr;
+    const TraceSet O{d.cobs, n, d.cobs, n, n}, Y{d.csyn, n, d.csyn, n, n};
+    const TraceSet Rs{d.res + r0, d.R, d.res + r0, d.R, n};
+    HIP_TRY(window_traces(st, nt, TraceSet{d.obs + r0, d.R, d.cobs, n, n}, d.win2, ws, we, wt,
+                          0.005f, dt));
+    HIP_TRY(window_traces(st, nt, TraceSet{d.syn + r0, d.R, d.csyn, n, n}, d.win2, ws, we, wt,
+                          0.005f, dt));
+    if (para.if_filter) {
+      HIP_TRY(trace_filter(st, d.fft, d.fperm, d.ftw, nt, O, nullptr, d.bpH, kc, scale));
+      HIP_TRY(trace_filter(st, d.fft, d.fperm, d.ftw, nt, Y, nullptr, d.bpH, kc, scale));
+    }
+    if (para.if_src_update) {
+      HIP_TRY(trace_spectra(st, d.fft, d.fperm, d.ftw, nt, O, d.wpad, d.spo));
+      HIP_TRY(trace_spectra(st, d.fft, d.fperm, d.ftw, nt, Y, d.wpad, d.spc));
+      HIP_TRY(spectrum_ratio(st, d.spo, d.spc, n, nf, d.coef));
+      HIP_TRY(trace_filter(st, d.fft, d.fperm, d.ftw, nt, Y, d.wpad, d.coef, nf, scale));
+      const size_t so = (size_t)s * nt;
+      HIP_TRY(trace_filter(st, d.fft, d.fperm, d.ftw, nt,
+                           TraceSet{d.srcamp + so, 1, d.srcout + so, 1, 1}, nullptr, d.coef, nf,
+                           scale));
+      HIP_TRY(source_increment(st, nt, d.srcout + so, dt, d.inj_rev + so));
+    }
+    HIP_TRY(shot_residual(st, nt, n, d.cobs, d.csyn, n, d.res + r0, d.R,
+                          d.cpart + (size_t)s * kCondBlocks, kCondBlocks));
+    if (para.if_src_update)
+      HIP_TRY(trace_filter(st, d.fft, d.fperm, d.ftw, nt, Rs, d.wpad, d.coef, kc, scale));
+    if (para.if_filter)
+      HIP_TRY(trace_filter(st, d.fft, d.fperm, d.ftw, nt, Rs, nullptr, d.bpH, kc, scale));
+    HIP_TRY(window_traces(st, nt, Rs, d.win2, ws, we, wt, 0.1f, dt));
+    if (keep) {
+      // [nSteps][n] -> [n][nSteps] (the Shot file layout)
+      h.resize((size_t)nt * n);
+      for (int q = 0; q < 2; q++) {
+        HIP_TRY(hipMemcpyAsync(h.data(), q ? d.csyn : d.cobs, h.size() * sizeof(float),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        float* dst = (q ? d.keep_syn.data() : d.keep_obs.data()) + (size_t)r0 * nt;
+        for (int t = 0; t < nt; t++)
+          for (int r = 0; r < n; r++) dst[(size_t)r * nt + t] = h[(size_t)t * n + r];
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d.cpart, g.ns * kCondBlocks,
+                     d.misfit);
+  HIP_TRY(hipGetLastError());
+  return Status::Ok();
+}
 
 Status Plan::run(int calc_id) {
   TRY(check_calc_id(calc_id));
@@ -2468,12 +2613,20 @@ Status Plan::run(int calc_id) {
   HIP_TRY(hipGetLastError());
   const Wave w = d.wave(fa);  // final forward state
 
-  if (if_res) {
+  d.src_updated = false;
+  if (if_res && d.cond) {
+    TRY(condition_residual(d, para, para.save_scratch && withAdj));
+    d.src_updated = para.if_src_update;
+  } else if (if_res) {
     hipLaunchKernelGGL(k_residual, dim3(d.npart), dim3(256), 0, st, g.nSteps, d.R, d.win2, d.obs,
                        d.syn, d.res, d.partial);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d.partial, d.npart, d.misfit);
     HIP_TRY(hipGetLastError());
   }
+  // the reversal subtracts the source rows the residual stage left: updated
+  // ones after a source update (libCUFD.cu:326-329, 390-394)
+  const float* inj_b = d.src_updated ? d.inj_rev : d.inj;
+  const Src srcb{d.src, inj_b};
 
   if (withAdj) {
     // ------------------------------------------------------- backward
@@ -2495,7 +2648,7 @@ Status Plan::run(int calc_id) {
       const size_t ldsa = (size_t)18 * NTf * sizeof(float);
       auto adj_step = [&](int res_t, int stf_it) {
         hipLaunchKernelGGL(k_adj_step, dim3(q.n), dim3(NTf), ldsa, st, q, g, m, c, d.wave(aa),
-                           d.wave(ab), d.memv(qa), d.memv(qb), src, rx,
+                           d.wave(ab), d.memv(qa), d.memv(qb), srcb, rx,
                            d.res + (size_t)res_t * d.R, d.R, d.stf_grad, stf_it);
         std::swap(aa, ab);
         std::swap(qa, qb);
@@ -2523,7 +2676,7 @@ Status Plan::run(int calc_id) {
           const Wave wa = d.wave(ra), wb = d.wave(rb);
           d.launch("rev_step", smp, [&] {
             hipLaunchKernelGGL(k_rev_step, dim3(qr.n), dim3(NTf), ldsr, st, qr, g, m, wa, wb, a,
-                               src, d.ring, it, d.accp);
+                               srcb, d.ring, it, d.accp);
           });
           std::swap(ra, rb);
         } else {
@@ -2551,7 +2704,7 @@ Status Plan::run(int calc_id) {
         tUg.spt = d.spt > 0 ? std::min(d.spt, n) : n;
         tUg.chunks = (n + tUg.spt - 1) / tUg.spt;
         const dim3 grdUg(tUg.n * tUg.chunks);
-        const Src sg = d.src_view(s0);
+        const Src sg = d.src_view(s0, inj_b);
         const float* ring = d.ring_view(s0);
         const int r0 = d.roff[s0], Rg = d.roff[s0 + n] - r0;
         const dim3 rgrid((Rg + 255) / 256);
@@ -2719,6 +2872,60 @@ Status Plan::write_data() {
   return Status::Ok();
 }
 
-Status Plan::write_scratch() { return Status::Ok(); }
+Status Plan::residual(float* res) {
+  if (!d_ || (last_calc != 0 && last_calc != 1))
+    return Status::Err(FWIFLOW_ERR_ARG, "no calc_id 0 or 1 result");
+  if (!res) return Status::Err(FWIFLOW_ERR_ARG, "null output pointer");
+  DeviceState& d = *d_;
+  HIP_TRY(hipSetDevice(d.dev));
+  const size_t T = (size_t)d.g.nSteps * d.R;
+  dim3 blk(32, 8), grd((d.R + 31) / 32, (d.g.nSteps + 31) / 32);
+  hipLaunchKernelGGL(k_transpose, grd, blk, 0, d.stream, d.res, d.stage, d.g.nSteps, d.R);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(res, d.stage, T * sizeof(float), hipMemcpyDeviceToHost, d.stream));
+  HIP_TRY(hipStreamSynchronize(d.stream));
+  return Status::Ok();
+}
+
+Status Plan::source(float* rows) {
+  if (!d_ || !d_->stf_set) return Status::Err(FWIFLOW_ERR_ARG, "no source set");
+  if (!rows) return Status::Err(FWIFLOW_ERR_ARG, "null output pointer");
+  DeviceState& d = *d_;
+  HIP_TRY(hipSetDevice(d.dev));
+  const float* p = d.src_updated ? d.srcout : d.srcamp;
+  HIP_TRY(hipMemcpy(rows, p, (size_t)d.g.ns * d.g.nSteps * sizeof(float), hipMemcpyDeviceToHost));
+  return Status::Ok();
+}
+
+// libCUFD.cu:493-511: after a gradient run, per shot the residual, the
+// conditioned synthetic and observed data and the source row.
+Status Plan::write_scratch() {
+  if (!d_ || last_calc != 1) return Status::Err(FWIFLOW_ERR_ARG, "no calc_id 1 result");
+  DeviceState& d = *d_;
+  if (!para.save_scratch) return Status::Ok();
+  const int nt = para.nSteps;
+  if (d.keep_obs.size() != (size_t)d.R * nt)
+    return Status::Err(FWIFLOW_ERR_ARG, "conditioned data of the last run were not kept");
+  std::vector<float> res((size_t)d.R * nt), src((size_t)d.g.ns * nt);
+  TRY(residual(res.data()));
+  TRY(source(src.data()));
+  auto put = [&](const std::string& fn, const float* p, size_t n) -> Status {
+    FILE* fp = std::fopen(fn.c_str(), "wb");
+    if (!fp) return Status::Err(FWIFLOW_ERR_IO, "cannot write \"" + fn + "\"");
+    const size_t w = std::fwrite(p, sizeof(float), n, fp);
+    std::fclose(fp);
+    if (w != n) return Status::Err(FWIFLOW_ERR_IO, "short write to \"" + fn + "\"");
+    return Status::Ok();
+  };
+  for (int s = 0; s < d.g.ns; s++) {
+    const std::string id = std::to_string(survey.shots[s].id), dir = para.scratch_dir + "/";
+    const size_t off = (size_t)d.roff[s] * nt, n = (size_t)(d.roff[s + 1] - d.roff[s]) * nt;
+    TRY(put(dir + "Residual_Shot" + id + ".bin", res.data() + off, n));
+    TRY(put(dir + "Syn_Shot" + id + ".bin", d.keep_syn.data() + off, n));
+    TRY(put(dir + "CondObs_Shot" + id + ".bin", d.keep_obs.data() + off, n));
+    TRY(put(dir + "src_updated" + id + ".bin", src.data() + (size_t)s * nt, (size_t)nt));
+  }
+  return Status::Ok();
+}
 
 }  // namespace fwiflow

@@ -25,6 +25,8 @@ class Plan {
   Status misfit(double* out);
   Status gradients(double* gL, double* gM, double* gD, double* gS);
   Status data(float* syn);
+  Status residual(float* res);  // adjoint source of the last calc 0/1 run, [R][nSteps]
+  Status source(float* rows);   // source rows [group][nSteps] after the last run
   Status write_data();
   Status write_scratch();
   Status set_checkpoint(int k);

@@ -28,6 +28,7 @@ EXPORTS = [
     "fwiflow_plan_set_model_zfast", "fwiflow_plan_set_stf", "fwiflow_plan_set_obs",
     "fwiflow_plan_load_obs", "fwiflow_plan_run", "fwiflow_plan_misfit",
     "fwiflow_plan_gradients", "fwiflow_plan_data", "fwiflow_plan_write_data",
+    "fwiflow_plan_residual", "fwiflow_plan_source",
     "fwiflow_plan_dims", "fwiflow_plan_set_checkpoint", "fwiflow_plan_set_timing",
     "fwiflow_plan_timing", "fwiflow_plan_device_bytes", "fwiflow_plan_shots_per_launch",
     "fwiflow_plan_set_illumination", "fwiflow_plan_illumination",
@@ -77,6 +78,8 @@ def load():
     L.fwiflow_plan_gradients.argtypes = [vp, dp, dp, dp, dp]
     L.fwiflow_plan_data.argtypes = [vp, fp]
     L.fwiflow_plan_write_data.argtypes = [vp]
+    L.fwiflow_plan_residual.argtypes = [vp, fp]
+    L.fwiflow_plan_source.argtypes = [vp, fp]
     L.fwiflow_plan_dims.argtypes = [vp, ip]
     L.fwiflow_plan_set_checkpoint.argtypes = [vp, C.c_int]
     L.fwiflow_plan_set_timing.argtypes = [vp, C.c_int]

@@ -164,6 +164,20 @@ class Plan:
     def write_data(self):
         check(self._L.fwiflow_plan_write_data(self._h))
 
+    def residual(self):
+        """Adjoint source of the last calc_id 0/1 run (after the conditioning
+        stages), every shot [nrec][nSteps] concatenated."""
+        out = np.zeros(self.nrec_total * self.nSteps, dtype=np.float32)
+        check(self._L.fwiflow_plan_residual(self._h, ptr(out, _lib.C.c_float)))
+        return out
+
+    def source(self):
+        """Source rows [group][nSteps] after the last run (updated by
+        ``if_src_update``)."""
+        out = np.zeros((self.ns, self.nSteps), dtype=np.float32)
+        check(self._L.fwiflow_plan_source(self._h, ptr(out, _lib.C.c_float)))
+        return out
+
     def set_checkpoint(self, k):
         check(self._L.fwiflow_plan_set_checkpoint(self._h, int(k)))
 

@@ -120,6 +120,14 @@ int fwiflow_plan_gradients(fwiflow_plan* plan, double* grad_Lambda, double* grad
 int fwiflow_plan_data(fwiflow_plan* plan, float* syn);
 /* Write the synthetic data to {data_dir}/Shot{id}.bin (calc_id 2's side effect). */
 int fwiflow_plan_write_data(fwiflow_plan* plan);
+/* Adjoint source of the last calc_id 0/1 run, every shot [nrec][nSteps] f32,
+ * concatenated: the residual after the conditioning stages (the reference's
+ * host copy vec_res, libCUFD.cu:315-318, written as Residual_Shot{id}.bin). */
+int fwiflow_plan_residual(fwiflow_plan* plan, float* res);
+/* Source rows [group_size][nSteps] f32 after the last run: the stf row times
+ * its window (Src_Rec.cu:132-143), replaced by the updated signature after a
+ * run with "if_src_update" (vec_source, libCUFD.cu:326-329; src_updated{id}.bin). */
+int fwiflow_plan_source(fwiflow_plan* plan, float* rows);
 /* Geometry of the plan: dims[0..5] = NZ, NX, nSteps, group_size, total receivers,
  * nPml. */
 int fwiflow_plan_dims(fwiflow_plan* plan, int* dims);

@@ -39,6 +39,7 @@
 // Layout inside: z fastest, index = x*NZ + z (el_stress.cu:1-19).
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -220,6 +221,212 @@ void window_default(int nt, int ntr, T dt, T ratio, T* data) {
   }
 }
 
+// ---------------------------------------------------------------- data conditioning
+// The optional residual stages of libCUFD.cu:254-309 (all off by default,
+// Parameter.cpp:146-174): per-trace windows and weights, band-pass filter,
+// source-signature update.  Data are [ntr][nt], time fastest.
+//
+// The reference transforms with cuFFT in FP32; here every transform is an
+// exact DFT evaluated in double by a mixed-radix decimation-in-time
+// recursion, and values are rounded to T where the reference stores floats.
+// The reference zero-pads a trace to 2*nt samples with intialArrayGPU
+// launched over only ceil(nt/32)*32 columns of each 2*nt row
+// (utilities.cu:886-887, 946-947, 1051-1052), so it reads uninitialised
+// device memory beyond that; the restatement pads with zeros (what a fresh
+// allocation holds).
+using cplx = std::complex<double>;
+
+// y[k] = sum_n x[n] exp(sign * 2 pi i n k / N); tw[j] = exp(sign * 2 pi i j / N0)
+// of the top-level length N0, `twm` = N0 / N.
+void dft_rec(const cplx* x, size_t stride, cplx* y, int N, const cplx* tw, size_t twm, int N0) {
+  if (N == 1) {
+    y[0] = x[0];
+    return;
+  }
+  int p = 2;
+  while (N % p) p++;
+  const int m = N / p;
+  std::vector<cplx> sub(N);
+  for (int q = 0; q < p; q++) dft_rec(x + q * stride, stride * p, &sub[(size_t)q * m], m, tw, twm * p, N0);
+  for (int k = 0; k < m; k++)
+    for (int j = 0; j < p; j++) {
+      cplx acc = 0.0;
+      for (int q = 0; q < p; q++)
+        acc += sub[(size_t)q * m + k] * tw[((size_t)q * (k + (size_t)m * j) % N) * twm];
+      y[k + (size_t)m * j] = acc;
+    }
+}
+
+std::vector<cplx> dft(const std::vector<cplx>& x, int sign) {
+  const int N = (int)x.size();
+  std::vector<cplx> tw(N), y(N);
+  for (int j = 0; j < N; j++) tw[j] = std::polar(1.0, sign * 2.0 * kPI * j / N);
+  dft_rec(x.data(), 1, y.data(), N, tw.data(), 1, N);
+  return y;
+}
+
+// R2C of a length-nt trace zero-padded to L: bins 0..L/2.
+template <typename T>
+std::vector<cplx> rfft_pad(const T* x, int nt, int L) {
+  std::vector<cplx> a(L, 0.0);
+  for (int t = 0; t < nt && t < L; t++) a[t] = (double)x[t];
+  std::vector<cplx> X = dft(a, -1);
+  X.resize(L / 2 + 1);
+  return X;
+}
+
+// C2R (unnormalised) of half spectrum X (bins 0..L/2): the Hermitian
+// extension, imaginary parts of the DC and Nyquist bins ignored.
+std::vector<double> irfft(const std::vector<cplx>& X, int L) {
+  std::vector<cplx> a(L);
+  for (int k = 0; k <= L / 2; k++) a[k] = X[k];
+  a[0] = a[0].real();
+  if (L % 2 == 0) a[L / 2] = a[L / 2].real();
+  for (int k = L / 2 + 1; k < L; k++) a[k] = std::conj(X[L - k]);
+  std::vector<cplx> y = dft(a, +1);
+  std::vector<double> out(L);
+  for (int t = 0; t < L; t++) out[t] = y[t].real();
+  return out;
+}
+
+// cuda_window with per-trace windows and weights (utilities.cu:653-704).
+// An element is left as it is when ratio > 0.5 or the window is empty
+// (the kernel's error paths return before scaling).
+template <typename T>
+void window_trace(int nt, int ntr, T dt, const T* wstart, const T* wend, const T* weight, T ratio,
+                  T* data) {
+  if (ratio > 0.5) return;
+  for (int r = 0; r < ntr; r++) {
+    T t0 = wstart[r], t3 = wend[r];
+    const T t_max = nt * dt;
+    if (t0 < 0.0) t0 = 0.0;
+    if (t0 > t_max) t0 = t_max;
+    if (t3 < 0.0) t3 = 0.0;
+    if (t3 > t_max) t3 = t_max;
+    const T offset = (t3 - t0) * ratio;
+    if (offset <= 0.0) continue;
+    const T t1 = t0 + offset, t2 = t3 - offset;
+    for (int idt = 0; idt < nt; idt++) {
+      T window_amp = 1.0;
+      const T t = idt * dt;
+      if (t >= t0 && t < t1)
+        window_amp = std::sin(kPI / 2.0 * (t - t0) / (t1 - t0));
+      else if (t >= t1 && t < t2)
+        window_amp = 1.0;
+      else if (t >= t2 && t < t3)
+        window_amp = std::cos(kPI / 2.0 * (t - t2) / (t3 - t2));
+      else
+        window_amp = 0.0;
+      data[(size_t)r * nt + idt] *= window_amp * window_amp * weight[r];
+    }
+  }
+}
+
+// Bins 0..kcov-1 of the half spectrum that the reference's frequency-domain
+// kernels touch: they run on a grid sized for nt (not nt+1) columns of TX = 32
+// threads (utilities.cu:879-880, 1042, utilities.h:18), so the Nyquist bin
+// k = nt is skipped when nt is a multiple of 32.
+inline int touched_bins(int nt) { return std::min(nt + 1, (nt + 31) / 32 * 32); }
+
+// Squared band-pass response amp(f)^2 per bin of the 2*nt-point grid
+// (cuda_bp_filter1d, utilities.cu:596-626); 1 on bins the kernel skips.
+template <typename T>
+std::vector<T> bandpass_response(int nt, T dt, const T* f) {
+  const int L = 2 * nt, nf = nt + 1, kc = touched_bins(nt);
+  std::vector<T> h(nf, (T)1);
+  const T df = 1.0 / dt / L;
+  for (int k = 0; k < kc; k++) {
+    const T freq = k * df;
+    T a;
+    if (freq >= f[0] && freq < f[1])
+      a = std::sin(kPI / 2.0 * (freq - f[0]) / (f[1] - f[0]));
+    else if (freq >= f[1] && freq < f[2])
+      a = 1.0;
+    else if (freq >= f[2] && freq < f[3])
+      a = std::cos(kPI / 2.0 * (freq - f[2]) / (f[3] - f[2]));
+    else
+      a = 0.0;
+    h[k] = a * a;
+  }
+  return h;
+}
+
+// bp_filter1d (utilities.cu:872-925): pad to 2*nt, R2C, times amp^2, C2R,
+// crop, times 1/(2*nt).
+template <typename T>
+void bp_filter(int nt, int ntr, T dt, const T* f, T* data) {
+  const int L = 2 * nt;
+  const std::vector<T> h = bandpass_response<T>(nt, dt, f);
+  const T factor = (T)1 / (T)L;
+  for (int r = 0; r < ntr; r++) {
+    T* x = data + (size_t)r * nt;
+    std::vector<cplx> X = rfft_pad(x, nt, L);
+    for (size_t k = 0; k < X.size(); k++) X[k] *= (double)h[k];
+    const std::vector<double> y = irfft(X, L);
+    for (int t = 0; t < nt; t++) x[t] = (T)y[t] * factor;
+  }
+}
+
+// source_update (utilities.cu:927-1035): obs and cal padded to 2*nt and
+// windowed (ratio 0.01 over the padded length), transformed; per bin
+// coef = sum_r conj(C_r) O_r / (sum_r |C_r|^2 + 1e-6); cal and the source
+// (padded, not windowed) are multiplied by coef and transformed back
+// (crop, times 1/(2*nt)).  Returns coef (bins 0..nt).
+template <typename T>
+std::vector<cplx> source_update(int nt, int ntr, T dt, const T* obs, T* cal, T* src) {
+  const int L = 2 * nt, nf = nt + 1;
+  std::vector<std::vector<cplx>> O(ntr), Cc(ntr);
+  for (int r = 0; r < ntr; r++) {
+    std::vector<T> po(L, (T)0), pc(L, (T)0);
+    std::copy(obs + (size_t)r * nt, obs + (size_t)(r + 1) * nt, po.begin());
+    std::copy(cal + (size_t)r * nt, cal + (size_t)(r + 1) * nt, pc.begin());
+    window_default<T>(L, 1, dt, (T)0.01, po.data());
+    window_default<T>(L, 1, dt, (T)0.01, pc.data());
+    O[r] = rfft_pad(po.data(), L, L);
+    Cc[r] = rfft_pad(pc.data(), L, L);
+  }
+  std::vector<cplx> S = rfft_pad(src, nt, L), coef(nf);
+  for (int k = 0; k < nf; k++) {
+    cplx num = 0.0;
+    double den = 0.0;
+    for (int r = 0; r < ntr; r++) {
+      num += std::conj(Cc[r][k]) * O[r][k];
+      den += std::norm(Cc[r][k]);
+    }
+    cplx c = num / (den + 1e-6);
+    coef[k] = cplx((T)c.real(), (T)c.imag());  // the reference keeps coef in FP32 complex
+    S[k] *= coef[k];
+    for (int r = 0; r < ntr; r++) Cc[r][k] *= coef[k];
+  }
+  const T factor = (T)1 / (T)L;
+  for (int r = 0; r < ntr; r++) {
+    const std::vector<double> y = irfft(Cc[r], L);
+    for (int t = 0; t < nt; t++) cal[(size_t)r * nt + t] = (T)y[t] * factor;
+  }
+  const std::vector<double> ys = irfft(S, L);
+  for (int t = 0; t < nt; t++) src[t] = (T)ys[t] * factor;
+  return coef;
+}
+
+// source_update_adj (utilities.cu:1037-1082) with amp_ratio = 1
+// (libCUFD.cu:287): pad, window (0.01 over the padded length), R2C, times coef
+// (not its conjugate) on the touched bins, C2R, crop, times 1/(2*nt).
+template <typename T>
+void source_update_adj(int nt, int ntr, T dt, const std::vector<cplx>& coef, T* data) {
+  const int L = 2 * nt, kc = touched_bins(nt);
+  const T factor = (T)1 / (T)L;
+  for (int r = 0; r < ntr; r++) {
+    T* x = data + (size_t)r * nt;
+    std::vector<T> p(L, (T)0);
+    std::copy(x, x + nt, p.begin());
+    window_default<T>(L, 1, dt, (T)0.01, p.data());
+    std::vector<cplx> X = rfft_pad(p.data(), L, L);
+    for (int k = 0; k < kc; k++) X[k] *= coef[k];
+    const std::vector<double> y = irfft(X, L);
+    for (int t = 0; t < nt; t++) x[t] = (T)y[t] * factor;
+  }
+}
+
 // ---------------------------------------------------------------- state
 template <typename T>
 struct State {
@@ -238,9 +445,16 @@ struct State {
   }
 };
 
+// Optional residual stages (Parameter.cpp:146-174, Src_Rec.cu:156-199).
+struct Cond {
+  bool if_win = false, if_filter = false, if_src_update = false;
+  double filter[4] = {0, 0, 0, 0};
+};
+
 struct Ctx {
   Geom g;
   int nthreads;
+  Cond cond;
 };
 
 #define OMP_FOR _Pragma("omp parallel for schedule(static) num_threads(nthreads)")
@@ -566,6 +780,11 @@ struct ShotIn {
   const float* obs;          // [nrec][nSteps] (calc 0/1)
   float* syn;                // out [nrec][nSteps]
   float* res;                // out [nrec][nSteps] (calc 0/1)
+  // per-trace windows and weights of the shot (if_win), nrec each
+  const double *win_start, *win_end, *weights;
+  // optional outputs (calc 0/1): conditioned obs and syn [nrec][nSteps] and
+  // the source after the update [nSteps] (libCUFD.cu:315-329)
+  float *cond_obs, *cond_syn, *src_out;
 };
 
 template <typename T>
@@ -631,6 +850,7 @@ struct Runner {
     std::vector<T> src(nSteps);
     for (int it = 0; it < nSteps; it++) src[it] = (T)sh.stf[it];
     window_default<T>(nSteps, 1, p.dt, (T)0.001, src.data());
+    const std::vector<T> src_fwd = src;  // the source update replaces src for the reversal
 
     State<T> s(n);
     std::vector<T> data((size_t)sh.nrec * nSteps, (T)0);
@@ -674,19 +894,50 @@ struct Runner {
     if (!if_res) return (T)0;
 
     // residual (libCUFD.cu:254-313)
+    const Cond& cd = c.cond;
     std::vector<T> obs((size_t)sh.nrec * nSteps), res(obs.size());
     for (size_t j = 0; j < obs.size(); j++) obs[j] = (T)sh.obs[j];
     const T win_ratio = 0.005;
-    window_default<T>(nSteps, sh.nrec, p.dt, win_ratio, obs.data());
-    window_default<T>(nSteps, sh.nrec, p.dt, win_ratio, data.data());
+    std::vector<T> ws, we, wt;
+    if (cd.if_win) {
+      for (int r = 0; r < sh.nrec; r++) {
+        ws.push_back((T)sh.win_start[r]);
+        we.push_back((T)sh.win_end[r]);
+        wt.push_back((T)sh.weights[r]);
+      }
+      window_trace<T>(nSteps, sh.nrec, p.dt, ws.data(), we.data(), wt.data(), win_ratio, obs.data());
+      window_trace<T>(nSteps, sh.nrec, p.dt, ws.data(), we.data(), wt.data(), win_ratio, data.data());
+    } else {
+      window_default<T>(nSteps, sh.nrec, p.dt, win_ratio, obs.data());
+      window_default<T>(nSteps, sh.nrec, p.dt, win_ratio, data.data());
+    }
+    T fl[4];
+    for (int q = 0; q < 4; q++) fl[q] = (T)cd.filter[q];
+    if (cd.if_filter) {
+      bp_filter<T>(nSteps, sh.nrec, p.dt, fl, obs.data());
+      bp_filter<T>(nSteps, sh.nrec, p.dt, fl, data.data());
+    }
+    std::vector<cplx> coef;
+    if (cd.if_src_update) coef = source_update<T>(nSteps, sh.nrec, p.dt, obs.data(), data.data(), src.data());
     for (int r = 0; r < sh.nrec; r++)
       for (int t = 0; t < nSteps; t++) {
         const size_t j = (size_t)r * nSteps + t;
         res[j] = t > 0 ? obs[j] - data[j] : (T)0;
       }
     T obj = cal_objective(res.data(), res.size());
-    window_default<T>(nSteps, sh.nrec, p.dt, win_ratio, res.data());
+    if (cd.if_src_update) source_update_adj<T>(nSteps, sh.nrec, p.dt, coef, res.data());
+    if (cd.if_filter) bp_filter<T>(nSteps, sh.nrec, p.dt, fl, res.data());
+    if (cd.if_win)
+      window_trace<T>(nSteps, sh.nrec, p.dt, ws.data(), we.data(), wt.data(), (T)0.1, res.data());
+    else
+      window_default<T>(nSteps, sh.nrec, p.dt, win_ratio, res.data());
     for (size_t j = 0; j < res.size(); j++) sh.res[j] = (float)res[j];
+    if (sh.cond_obs)
+      for (size_t j = 0; j < obs.size(); j++) sh.cond_obs[j] = (float)obs[j];
+    if (sh.cond_syn)
+      for (size_t j = 0; j < data.size(); j++) sh.cond_syn[j] = (float)data[j];
+    if (sh.src_out)
+      for (int it = 0; it < nSteps; it++) sh.src_out[it] = (float)src[it];
     if (!withAdj) return obj;
 
     // ------------------------------------------------ backward (libCUFD.cu:334-457)
@@ -763,7 +1014,7 @@ struct Runner {
           const size_t q = (size_t)(it - s0);
           std::copy(s.vz.begin(), s.vz.end(), vel.begin() + (q * 2 + 0) * n);
           std::copy(s.vx.begin(), s.vx.end(), vel.begin() + (q * 2 + 1) * n);
-          forward_step(s, it, src, sh, nullptr);
+          forward_step(s, it, src_fwd, sh, nullptr);
           std::copy(s.szz.begin(), s.szz.end(), sig.begin() + (q * 3 + 0) * n);
           std::copy(s.sxx.begin(), s.sxx.end(), sig.begin() + (q * 3 + 1) * n);
           std::copy(s.sxz.begin(), s.sxz.end(), sig.begin() + (q * 3 + 2) * n);
@@ -829,21 +1080,42 @@ typedef struct {
   double dz, dx, dt, f0;
 } oracle_geom;
 
+// Optional residual stages ("if_win", "filter", "if_src_update" of the
+// parameter file) and their outputs.  win_start/win_end/weights are
+// concatenated over the group's shots (one value per receiver, seconds and
+// weight); cond_obs/cond_syn are laid out like obs, src_out is
+// [group_size][nSteps].  Output pointers may be NULL.
+typedef struct {
+  int if_win, if_filter, if_src_update;
+  double filter[4];
+  const double *win_start, *win_end, *weights;
+  float *cond_obs, *cond_syn, *src_out;
+} oracle_cond;
+
 // Full cufd restatement.  Inputs follow the reference's conventions:
 // Lambda/Mu in MPa and Den, row-major [NZ][NX]; stf rows already selected per
 // shot ([group_size][nSteps]); receiver/source indices already offset by nPml
 // (Src_Rec.cu:86,90,105,113).  obs/syn/res are concatenated per shot,
 // [nrec][nSteps] each.  illum (optional, [NZ][NX]) receives the forward
-// illumination.  Returns 0.
+// illumination; cond (optional) switches on the conditioning stages.
+// Returns 0, or -1 when if_win is set without window arrays.
 int oracle_cufd(int precision, int scheme, int ckpt_every, int calc_id, const oracle_geom* g,
                 int group_size, const int* z_src, const int* x_src, const int* nrec,
                 const int* z_rec, const int* x_rec, const double* Lambda, const double* Mu,
                 const double* Den, const double* stf_rows, const float* obs, float* syn,
                 float* res, double* misfit, double* grad_lambda, double* grad_mu,
-                double* grad_den, double* grad_stf, int nthreads, double* illum) {
+                double* grad_den, double* grad_stf, int nthreads, double* illum,
+                const oracle_cond* cond) {
   Ctx c;
   c.g = Geom{g->nz, g->nx, g->nPml, g->nPad, g->nSteps, g->dz, g->dx, g->dt, g->f0};
   c.nthreads = nthreads > 0 ? nthreads : 1;
+  if (cond) {
+    c.cond.if_win = cond->if_win != 0;
+    c.cond.if_filter = cond->if_filter != 0;
+    c.cond.if_src_update = cond->if_src_update != 0;
+    for (int q = 0; q < 4; q++) c.cond.filter[q] = cond->filter[q];
+    if (c.cond.if_win && (!cond->win_start || !cond->win_end || !cond->weights)) return -1;
+  }
   std::vector<ShotIn> shots(group_size);
   size_t roff = 0, doff = 0;
   for (int s = 0; s < group_size; s++) {
@@ -857,6 +1129,13 @@ int oracle_cufd(int precision, int scheme, int ckpt_every, int calc_id, const or
     sh.obs = obs ? obs + doff : nullptr;
     sh.syn = syn + doff;
     sh.res = res ? res + doff : nullptr;
+    const bool w = cond && cond->if_win;
+    sh.win_start = w ? cond->win_start + roff : nullptr;
+    sh.win_end = w ? cond->win_end + roff : nullptr;
+    sh.weights = w ? cond->weights + roff : nullptr;
+    sh.cond_obs = cond && cond->cond_obs ? cond->cond_obs + doff : nullptr;
+    sh.cond_syn = cond && cond->cond_syn ? cond->cond_syn + doff : nullptr;
+    sh.src_out = cond && cond->src_out ? cond->src_out + (size_t)s * g->nSteps : nullptr;
     roff += nrec[s];
     doff += (size_t)nrec[s] * g->nSteps;
   }
@@ -915,6 +1194,57 @@ void oracle_window(int precision, int nt, int ntr, double dt, double ratio, doub
     window_default<float>(nt, ntr, (float)dt, (float)ratio, v.data());
     for (size_t i = 0; i < v.size(); i++) data[i] = v[i];
   }
+}
+
+// Exact DFT of n complex values (interleaved re, im), sign -1 forward, +1
+// inverse (unnormalised), in place.
+void oracle_dft(int n, int sign, double* data) {
+  std::vector<cplx> x(n);
+  for (int i = 0; i < n; i++) x[i] = cplx(data[2 * i], data[2 * i + 1]);
+  const std::vector<cplx> y = dft(x, sign);
+  for (int i = 0; i < n; i++) {
+    data[2 * i] = y[i].real();
+    data[2 * i + 1] = y[i].imag();
+  }
+}
+
+// Conditioning stages on [ntr][nt] data in place (restated reference
+// functions, for pinning).  op 0: per-trace window (params: ratio; arrays
+// wstart, wend, weight); op 1: band-pass (params: f0..f3); op 2: source
+// update (data = cal, aux = obs [ntr][nt], src = [nt] in/out, coef_out =
+// 2*(nt+1) doubles); op 3: its adjoint (coef_in = 2*(nt+1) doubles).
+int oracle_condition(int precision, int op, int nt, int ntr, double dt, const double* params,
+                     const double* wstart, const double* wend, const double* weight,
+                     double* data, const double* aux, double* src, double* coef) {
+  auto run = [&](auto tag) -> int {
+    using T = decltype(tag);
+    const size_t n = (size_t)nt * ntr;
+    std::vector<T> d(data, data + n);
+    if (op == 0) {
+      std::vector<T> a(wstart, wstart + ntr), b(wend, wend + ntr), w(weight, weight + ntr);
+      window_trace<T>(nt, ntr, (T)dt, a.data(), b.data(), w.data(), (T)params[0], d.data());
+    } else if (op == 1) {
+      T f[4] = {(T)params[0], (T)params[1], (T)params[2], (T)params[3]};
+      bp_filter<T>(nt, ntr, (T)dt, f, d.data());
+    } else if (op == 2) {
+      std::vector<T> o(aux, aux + n), s(src, src + nt);
+      const std::vector<cplx> cf = source_update<T>(nt, ntr, (T)dt, o.data(), d.data(), s.data());
+      for (int t = 0; t < nt; t++) src[t] = s[t];
+      for (int k = 0; k <= nt; k++) {
+        coef[2 * k] = cf[k].real();
+        coef[2 * k + 1] = cf[k].imag();
+      }
+    } else if (op == 3) {
+      std::vector<cplx> cf(nt + 1);
+      for (int k = 0; k <= nt; k++) cf[k] = cplx(coef[2 * k], coef[2 * k + 1]);
+      source_update_adj<T>(nt, ntr, (T)dt, cf, d.data());
+    } else {
+      return -1;
+    }
+    for (size_t i = 0; i < n; i++) data[i] = d[i];
+    return 0;
+  };
+  return precision == 64 ? run(0.0) : run(0.0f);
 }
 
 // Ring cell indices (z-fastest linear index), count returned; out may be null.

@@ -24,6 +24,14 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
+class Cond(C.Structure):
+    _fields_ = [("if_win", C.c_int), ("if_filter", C.c_int), ("if_src_update", C.c_int),
+                ("filter", C.c_double * 4), ("win_start", C.POINTER(C.c_double)),
+                ("win_end", C.POINTER(C.c_double)), ("weights", C.POINTER(C.c_double)),
+                ("cond_obs", C.POINTER(C.c_float)), ("cond_syn", C.POINTER(C.c_float)),
+                ("src_out", C.POINTER(C.c_float))]
+
+
 class Geom(C.Structure):
     _fields_ = [("nz", C.c_int), ("nx", C.c_int), ("nPml", C.c_int), ("nPad", C.c_int),
                 ("nSteps", C.c_int), ("dz", C.c_double), ("dx", C.c_double),
@@ -47,7 +55,11 @@ def lib():
         L.oracle_cufd.restype = C.c_int
         L.oracle_cufd.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Geom), C.c_int,
                                   ip, ip, ip, ip, ip, dp, dp, dp, dp, fp, fp, fp,
-                                  dp, dp, dp, dp, dp, C.c_int, dp]
+                                  dp, dp, dp, dp, dp, C.c_int, dp, C.POINTER(Cond)]
+        L.oracle_dft.argtypes = [C.c_int, C.c_int, dp]
+        L.oracle_condition.restype = C.c_int
+        L.oracle_condition.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, dp, dp,
+                                       dp, dp, dp, dp, dp, dp]
         L.oracle_cpml.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                   C.c_double, dp]
         L.oracle_model.argtypes = [C.c_int, C.POINTER(Geom), dp, dp, dp, dp]
@@ -91,6 +103,53 @@ def window(data, dt, ratio, precision=32):
     return d
 
 
+def dft(x, sign=-1):
+    """Exact DFT (sign -1 forward, +1 inverse unnormalised) of a complex vector."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).copy()
+    lib().oracle_dft(a.size, int(sign), a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)))
+    return a
+
+
+def _cond_call(op, data, dt, precision, params=(0, 0, 0, 0), wstart=None, wend=None,
+               weight=None, aux=None, src=None, coef=None):
+    d = np.ascontiguousarray(np.atleast_2d(data), dtype=np.float64).copy()
+    ntr, nt = d.shape
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    arr = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    ws, we, wt, ax = arr(wstart), arr(wend), arr(weight), arr(aux)
+    sr = None if src is None else np.ascontiguousarray(src, dtype=np.float64).copy()
+    cf = np.zeros(2 * (nt + 1)) if coef is None else np.ascontiguousarray(
+        np.column_stack([np.real(coef), np.imag(coef)]).ravel(), dtype=np.float64)
+    rc = lib().oracle_condition(precision, op, nt, ntr, float(dt), _p(prm, C.c_double),
+                                _p(ws, C.c_double), _p(we, C.c_double), _p(wt, C.c_double),
+                                _p(d, C.c_double), _p(ax, C.c_double), _p(sr, C.c_double),
+                                _p(cf, C.c_double))
+    if rc != 0:
+        raise RuntimeError(f"oracle_condition({op}) failed")
+    return d, sr, cf[0::2] + 1j * cf[1::2]
+
+
+def window_traces(data, dt, wstart, wend, weights, ratio, precision=32):
+    """Per-trace window and weights (utilities.cu:653-704) on [ntr][nt]."""
+    return _cond_call(0, data, dt, precision, (ratio, 0, 0, 0), wstart, wend, weights)[0]
+
+
+def bandpass(data, dt, f, precision=32):
+    """bp_filter1d (utilities.cu:872-925) on [ntr][nt]."""
+    return _cond_call(1, data, dt, precision, tuple(f))[0]
+
+
+def source_update(obs, cal, src, dt, precision=32):
+    """source_update (utilities.cu:927-1035): returns (cal', src', coef[nt+1])."""
+    d, s, cf = _cond_call(2, cal, dt, precision, aux=obs, src=src)
+    return d, s, cf
+
+
+def source_update_adj(data, coef, dt, precision=32):
+    """source_update_adj (utilities.cu:1037-1082) with amp_ratio 1."""
+    return _cond_call(3, data, dt, precision, coef=coef)[0]
+
+
 def ring_indices(g: Geom):
     n = lib().oracle_ring(C.byref(g), None)
     out = np.zeros(n, dtype=np.int64)
@@ -99,14 +158,18 @@ def ring_indices(g: Geom):
 
 
 def run(g: Geom, calc_id, lam, mu, den, stf_rows, src, recs, obs=None, precision=32,
-        scheme=0, ckpt_every=0, nthreads=None, illumination=False):
+        scheme=0, ckpt_every=0, nthreads=None, illumination=False, cond=None):
     """Run the restated operator on explicit inputs.
 
     ``src``: list of (z, x) grid indices per shot (already + nPml);
     ``recs``: list of (z_array, x_array) per shot (already + nPml);
     ``stf_rows``: [group][nSteps]; ``obs``: list of [nrec][nSteps] float32 (calc 0/1).
+    ``cond``: optional dict of the conditioning stages: ``if_win`` with
+    ``win_start``/``win_end``/``weights`` (lists of per-shot arrays),
+    ``filter`` (four frequencies) and ``if_src_update``.
     Returns dict with syn, res (lists), misfit, grad_lambda/mu/den [NZ][NX],
-    grad_stf [group][nSteps] and, with ``illumination``, illum [NZ][NX].
+    grad_stf [group][nSteps], with ``illumination`` illum [NZ][NX], and with
+    ``cond`` the conditioned obs and syn (lists) and the updated sources.
     """
     G = len(src)
     nrec = np.array([len(r[0]) for r in recs], dtype=np.int32)
@@ -131,19 +194,45 @@ def run(g: Geom, calc_id, lam, mu, den, stf_rows, src, recs, obs=None, precision
     il = np.zeros((g.nz, g.nx)) if illumination else None
     if nthreads is None:
         nthreads = min(os.cpu_count() or 1, 16)
+    cd, keep = None, []
+    if cond is not None:
+        cd = Cond()
+        cd.if_win = int(bool(cond.get("if_win", False)))
+        f = cond.get("filter")
+        cd.if_filter = int(f is not None)
+        if f is not None:
+            for q in range(4):
+                cd.filter[q] = float(f[q])
+        cd.if_src_update = int(bool(cond.get("if_src_update", False)))
+        if cd.if_win:
+            for name in ("win_start", "win_end", "weights"):
+                a = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64)
+                                                         for v in cond[name]]))
+                keep.append(a)
+                setattr(cd, name, _p(a, C.c_double))
+        cobs = np.zeros(tot, dtype=np.float32)
+        csyn = np.zeros(tot, dtype=np.float32)
+        sout = np.zeros((G, g.nSteps), dtype=np.float32)
+        keep += [cobs, csyn, sout]
+        cd.cond_obs, cd.cond_syn, cd.src_out = (_p(cobs, C.c_float), _p(csyn, C.c_float),
+                                                _p(sout, C.c_float))
     rc = lib().oracle_cufd(precision, scheme, ckpt_every, calc_id, C.byref(g), G,
                            _p(zs, C.c_int), _p(xs, C.c_int), _p(nrec, C.c_int), _p(zr, C.c_int),
                            _p(xr, C.c_int), _p(L, C.c_double), _p(M, C.c_double),
                            _p(D, C.c_double), _p(S, C.c_double), _p(ob, C.c_float),
                            _p(syn, C.c_float), _p(res, C.c_float), _p(misfit, C.c_double),
                            _p(gL, C.c_double), _p(gM, C.c_double), _p(gD, C.c_double),
-                           _p(gS, C.c_double), int(nthreads), _p(il, C.c_double))
+                           _p(gS, C.c_double), int(nthreads), _p(il, C.c_double),
+                           C.byref(cd) if cd is not None else None)
     if rc != 0:
         raise RuntimeError(f"oracle_cufd failed with {rc}")
     offs = np.concatenate([[0], np.cumsum(nrec) * g.nSteps])
     split = lambda a: [a[offs[i]:offs[i + 1]].reshape(nrec[i], g.nSteps) for i in range(G)]
-    return dict(syn=split(syn), res=split(res), misfit=float(misfit[0]), grad_lambda=gL,
-                grad_mu=gM, grad_den=gD, grad_stf=gS, illum=il)
+    out = dict(syn=split(syn), res=split(res), misfit=float(misfit[0]), grad_lambda=gL,
+               grad_mu=gM, grad_den=gD, grad_stf=gS, illum=il)
+    if cd is not None:
+        out.update(cond_obs=split(cobs), cond_syn=split(csyn), src_updated=sout)
+    return out
 
 
 def read_json_line(path):
@@ -167,8 +256,16 @@ def cufd(lam, mu, den, stf, calc_id, shot_ids, para_fname, precision=32, scheme=
              para["dx"], para["dt"], para["f0"])
     stf = np.asarray(stf, dtype=np.float64).reshape(-1, g.nSteps)
     src, recs, rows, obs = [], [], [], []
+    cond = None
+    if para.get("if_win") or "filter" in para or para.get("if_src_update"):
+        cond = {"if_win": bool(para.get("if_win", False)), "filter": para.get("filter"),
+                "if_src_update": bool(para.get("if_src_update", False)),
+                "win_start": [], "win_end": [], "weights": []}
     for sid in shot_ids:
         sh = survey[f"shot{int(sid)}"]
+        if cond is not None and cond["if_win"]:
+            for name in ("win_start", "win_end", "weights"):
+                cond[name].append(np.asarray(sh[name], dtype=np.float64))
         src.append((sh["z_src"] + nPml, sh["x_src"] + nPml))
         recs.append((np.asarray(sh["z_rec"]) + nPml, np.asarray(sh["x_rec"]) + nPml))
         rows.append(stf[int(sid)])
@@ -177,7 +274,7 @@ def cufd(lam, mu, den, stf, calc_id, shot_ids, para_fname, precision=32, scheme=
             obs.append(np.fromfile(fn, dtype=np.float32).reshape(sh["nrec"], g.nSteps))
     r = run(g, calc_id, lam, mu, den, np.stack(rows), src, recs, obs or None,
             precision=precision, scheme=scheme, ckpt_every=ckpt_every, nthreads=nthreads,
-            illumination=illumination)
+            illumination=illumination, cond=cond)
     if calc_id == 2:
         for sid, d in zip(shot_ids, r["syn"]):
             d.astype(np.float32).tofile(os.path.join(para["data_dir_name"], f"Shot{int(sid)}.bin"))

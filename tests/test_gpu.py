@@ -434,3 +434,133 @@ def test_fused_kernels_match_split_kernels(workspace, variant):
     assert abs(J1 - J0) <= 1e-6 * abs(J0)
     for a, b in zip(g1, g0):
         assert rel(a, b) < 1e-5
+
+
+# ------------------------------------------------------------ data conditioning
+COND_VARIANTS = {
+    "windows": dict(if_win=True),
+    "bandpass": dict(filter_para=[2.0, 5.0, 25.0, 40.0]),
+    "bandpass_nt_multiple_of_32": dict(filter_para=[2.0, 5.0, 25.0, 40.0], nSteps=320),
+    "source_update": dict(if_src_update=True),
+    "all_three": dict(if_win=True, filter_para=[2.0, 5.0, 25.0, 40.0], if_src_update=True),
+}
+
+
+def _cond_setup(ws, if_win=False, filter_para=None, if_src_update=False, nSteps=300,
+                scratch=False):
+    from fwiflow.utils import paraGen, surveyGen
+    c = models.case_small(nz=40, nx=60, nSteps=nSteps, nshots=3)
+    ns, nrec = len(c.src_x), len(c.rec_x)
+    T = c.nSteps * c.dt
+    win = {f"shot{s}": {"start": list(0.02 + 0.001 * np.arange(nrec) + 0.01 * s),
+                        "end": list(np.full(nrec, 0.85 * T) - 0.003 * s)} for s in range(ns)}
+    wts = {f"shot{s}": {"weights": list(1.0 + 0.01 * np.arange(nrec) + 0.1 * s)}
+           for s in range(ns)}
+    os.makedirs(ws, exist_ok=True)
+    para = os.path.join(ws, "para_file.json")
+    survey = os.path.join(ws, "survey_file.json")
+    paraGen(c.NZ, c.NX, c.dz, c.dx, c.nSteps, c.dt, c.f0, c.nPml, c.nPad, para, survey,
+            os.path.join(ws, "Data"), if_win=if_win, filter_para=filter_para,
+            if_src_update=if_src_update, scratch_dir_name=os.path.join(ws, "scr") if scratch else "")
+    surveyGen(c.src_z, c.src_x, c.rec_z, c.rec_x, survey, Windows=win if if_win else None,
+              Weights=wts if if_win else None)
+    stf = np.repeat(c.stf.reshape(1, -1), ns, axis=0)
+    cond = None
+    if if_win or filter_para is not None or if_src_update:
+        cond = {"if_win": if_win, "filter": filter_para, "if_src_update": if_src_update,
+                "win_start": [np.array(win[f"shot{s}"]["start"]) for s in range(ns)],
+                "win_end": [np.array(win[f"shot{s}"]["end"]) for s in range(ns)],
+                "weights": [np.array(wts[f"shot{s}"]["weights"]) for s in range(ns)]}
+    return c, para, stf, cond
+
+
+@pytest.mark.parametrize("variant", list(COND_VARIANTS))
+def test_conditioning_parity(oracle_mod, workspace, variant):
+    """Per-trace windows and weights, band-pass and source update
+    (libCUFD.cu:254-309) on the GPU against the oracle: misfit, adjoint source
+    (residual after the stages), updated source rows and gradients."""
+    kw = COND_VARIANTS[variant]
+    c, para, stf, cond = _cond_setup(workspace, **kw)
+    ids = list(range(len(c.src_x)))
+    say("conditioning", variant, "grid", c.NZ, c.NX, "nt", c.nSteps)
+    g = oracle_mod.geom(c.NZ, c.NX, c.nPml, c.nPad, c.nSteps, c.dz, c.dx, c.dt, c.f0)
+    src = [(z + c.nPml, x + c.nPml) for z, x in zip(c.src_z, c.src_x)]
+    recs = [(c.rec_z + c.nPml, c.rec_x + c.nPml)] * len(src)
+    lam, mu, den = models.moduli(c, "true")
+    obs = oracle_mod.run(g, 2, lam, mu, den, stf, src, recs, precision=32)["syn"]
+    lam0, mu0, den0 = models.moduli(c, "start")
+    o32 = oracle_mod.run(g, 1, lam0, mu0, den0, stf, src, recs, obs, precision=32, cond=cond)
+    o64 = oracle_mod.run(g, 1, lam0, mu0, den0, stf, src, recs, obs, precision=64, cond=cond)
+    j32 = oracle_mod.run(g, 0, lam0, mu0, den0, stf, src, recs, obs, precision=32,
+                         cond=cond)["misfit"]
+    P = fwiflow.Plan(para, ids)
+    try:
+        P.set_model(lam0, mu0, den0)
+        P.set_stf(stf)
+        P.set_obs(obs)
+        P.run(1)
+        J = P.misfit()
+        res = P.residual()
+        srcrows = P.source()
+        gL, gM, gD, gS = P.gradients()
+    finally:
+        P.close()
+    nrec, nt = len(c.rec_x), c.nSteps
+    res = res.reshape(len(ids), nrec, nt)
+    out = {"misfit32": abs(J - j32) / abs(j32),
+           "res32": max(rel(res[s], o32["res"][s]) for s in ids),
+           "res64": max(rel(res[s], o64["res"][s]) for s in ids),
+           "lam32": rel(gL, o32["grad_lambda"]), "mu32": rel(gM, o32["grad_mu"]),
+           "den32": rel(gD, o32["grad_den"]),
+           "lam64": rel(gL, o64["grad_lambda"]), "mu64": rel(gM, o64["grad_mu"]),
+           "den64": rel(gD, o64["grad_den"])}
+    if cond is not None and cond["if_src_update"]:
+        out["src32"] = max(rel(srcrows[s], o32["src_updated"][s]) for s in ids)
+    say(variant, {k: f"{v:.2e}" for k, v in out.items()})
+    for k, v in out.items():
+        tol = SEIS_TOL if k.startswith(("misfit", "res", "src")) else GRAD_TOL
+        assert v <= tol, (k, v, tol)
+
+
+def test_scratch_files_and_unconditioned_path_agree(oracle_mod, workspace):
+    """A parameter file with scratch_dir_name takes the shot-by-shot residual
+    path (it keeps the conditioned data): its residual and gradients equal
+    the default path's bit for bit, its misfit to summation order, and the
+    scratch files (libCUFD.cu:493-511) hold the residual, the windowed data and
+    the source rows."""
+    out = {}
+    for scratch in (False, True):
+        ws = os.path.join(workspace, "scr" if scratch else "plain")
+        c, para, stf, _ = _cond_setup(ws, scratch=scratch)
+        ids = list(range(len(c.src_x)))
+        lam, mu, den = models.moduli(c, "true")
+        fwiflow.fwi_obs_op(lam, mu, den, stf, 0, ids, para)
+        lam0, mu0, den0 = models.moduli(c, "start")
+        P = fwiflow.Plan(para, ids)
+        try:
+            P.set_model(lam0, mu0, den0)
+            P.set_stf(stf)
+            P.load_obs()
+            P.run(1)
+            out[scratch] = (P.misfit(), P.residual(), P.gradients(), P.data())
+        finally:
+            P.close()
+        if scratch:  # fwiflow_cufd writes the scratch files after a gradient run
+            fwiflow.fwi_op_grad(lam0, mu0, den0, stf, 0, ids, para)
+    (J0, r0, g0, d0), (J1, r1, g1, d1) = out[False], out[True]
+    assert J1 == pytest.approx(J0, rel=1e-12)
+    assert np.array_equal(r0, r1)
+    for a, b in zip(g0, g1):
+        assert np.array_equal(a, b)
+    nrec, nt = len(c.rec_x), c.nSteps
+    scr = os.path.join(workspace, "scr", "scr")
+    w2 = fwiflow.host_tables(para, c.NZ - c.nPad, c.NX, c.nSteps)[2]
+    for s in ids:
+        res = np.fromfile(os.path.join(scr, f"Residual_Shot{s}.bin"), dtype=np.float32)
+        assert np.array_equal(res, r1.reshape(len(ids), -1)[s])
+        syn = np.fromfile(os.path.join(scr, f"Syn_Shot{s}.bin"), dtype=np.float32)
+        want = (d1.reshape(len(ids), nrec, nt)[s] * w2[None, :].astype(np.float32))
+        assert np.array_equal(syn.reshape(nrec, nt), want.astype(np.float32))
+        assert os.path.getsize(os.path.join(scr, f"CondObs_Shot{s}.bin")) == 4 * nrec * nt
+        src = np.fromfile(os.path.join(scr, f"src_updated{s}.bin"), dtype=np.float32)
+        assert src.size == nt and np.abs(src).max() > 0

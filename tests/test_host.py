@@ -71,22 +71,43 @@ def test_garbled_and_incomplete_json(workspace):
     assert _code(e) == "FWIFLOW_ERR_JSON" and "nSteps" in str(e.value)
 
 
-def test_unsupported_options(workspace):
+def test_conditioning_options(workspace):
+    """if_win / filter / if_src_update (Parameter.cpp:146-174, Src_Rec.cu:156-199)
+    are parsed and validated; what the transforms cannot hold is refused."""
     c, para, stf, lam, mu, den = _small(workspace)
     d = json.loads(open(para).read())
-    for key, val in (("filter", [0, 0.1, 100, 200]), ("if_src_update", True), ("if_win", True)):
-        dd = dict(d)
-        dd[key] = val
+    survey = d["survey_fname"]
+    sv = json.loads(open(survey).read())
+
+    def try_para(dd, code=None):
         open(para, "w").write(json.dumps(dd))
+        if code is None:
+            fwiflow.check_call(2, [0], para, lam, mu, den)
+            return
         with pytest.raises(fwiflow.FwiflowError) as e:
             fwiflow.check_call(2, [0], para, lam, mu, den)
-        assert _code(e) == "FWIFLOW_ERR_UNSUPPORTED"
-    # explicit false flags are accepted
-    dd = dict(d)
-    dd["if_win"] = False
-    dd["if_src_update"] = False
-    open(para, "w").write(json.dumps(dd))
-    fwiflow.check_call(2, [0], para, lam, mu, den)
+        assert _code(e) == code
+
+    try_para(dict(d, filter=[0, 0.1, 100, 200], if_src_update=True))
+    try_para(dict(d, filter=[0, 0.1, 100]), "FWIFLOW_ERR_JSON")
+    try_para(dict(d, if_src_update=1), "FWIFLOW_ERR_JSON")
+    try_para(dict(d, if_win=False, if_src_update=False))
+    # per-trace windows need win_start / win_end / weights of nrec values
+    try_para(dict(d, if_win=True), "FWIFLOW_ERR_JSON")
+    nrec = sv["shot0"]["nrec"]
+    sv2 = json.loads(json.dumps(sv))
+    for k in range(sv2["nShots"]):
+        sv2[f"shot{k}"].update(win_start=[0.0] * nrec, win_end=[0.1] * nrec, weights=[1.0] * nrec)
+    open(survey, "w").write(json.dumps(sv2))
+    try_para(dict(d, if_win=True))
+    sv2["shot0"]["weights"] = [1.0] * (nrec - 1)
+    open(survey, "w").write(json.dumps(sv2))
+    try_para(dict(d, if_win=True), "FWIFLOW_ERR_SHAPE")
+    open(survey, "w").write(json.dumps(sv))
+    # 2*nSteps beyond one workgroup's LDS, or with a prime factor > 61
+    try_para(dict(d, filter=[0, 1, 2, 3], nSteps=11000), "FWIFLOW_ERR_UNSUPPORTED")
+    try_para(dict(d, if_src_update=True, nSteps=1009), "FWIFLOW_ERR_UNSUPPORTED")
+    try_para(dict(d, nSteps=1009))  # no transform needed: accepted
 
 
 def test_shot_id_out_of_range(workspace):

@@ -189,14 +189,16 @@ __global__ __launch_bounds__(kThreads) void k_trace_spectra(FftDesc f, const int
   }
 }
 
-__global__ __launch_bounds__(256) void k_spectrum_ratio(const float2* __restrict__ so,
-                                                        const float2* __restrict__ sc, int ntr,
-                                                        int nf, float2* __restrict__ coef) {
-  __shared__ double red[3][4][64];
+constexpr int kRatioRows = 16;  // receiver slices of one k_spectrum_ratio block
+
+__global__ __launch_bounds__(1024) void k_spectrum_ratio(const float2* __restrict__ so,
+                                                         const float2* __restrict__ sc, int ntr,
+                                                         int nf, float2* __restrict__ coef) {
+  __shared__ double red[3][kRatioRows][64];
   const int k = blockIdx.x * 64 + threadIdx.x;
   double nr = 0.0, ni = 0.0, dd = 0.0;
   if (k < nf)
-    for (int r = threadIdx.y; r < ntr; r += 4) {
+    for (int r = threadIdx.y; r < ntr; r += kRatioRows) {
       const float2 c = sc[(size_t)r * nf + k], o = so[(size_t)r * nf + k];
       const double cx = c.x, cy = c.y, ox = o.x, oy = o.y;
       nr += cx * ox + cy * oy;  // conj(c) * o
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(256) void k_spectrum_ratio(const float2* __restrict
   red[2][threadIdx.y][threadIdx.x] = dd;
   __syncthreads();
   if (threadIdx.y == 0 && k < nf) {
-    for (int y = 1; y < 4; y++) {
+    for (int y = 1; y < kRatioRows; y++) {
       nr += red[0][y][threadIdx.x];
       ni += red[1][y][threadIdx.x];
       dd += red[2][y][threadIdx.x];
@@ -388,8 +390,8 @@ hipError_t trace_spectra(hipStream_t st, const FftDesc& f, const int* perm, cons
 
 hipError_t spectrum_ratio(hipStream_t st, const float2* spec_obs, const float2* spec_cal, int ntr,
                           int nf, float2* coef) {
-  hipLaunchKernelGGL(k_spectrum_ratio, dim3((nf + 63) / 64), dim3(64, 4), 0, st, spec_obs, spec_cal,
-                     ntr, nf, coef);
+  hipLaunchKernelGGL(k_spectrum_ratio, dim3((nf + 63) / 64), dim3(64, kRatioRows), 0, st, spec_obs,
+                     spec_cal, ntr, nf, coef);
   return hipGetLastError();
 }
 

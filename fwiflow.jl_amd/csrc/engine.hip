@@ -48,7 +48,7 @@ namespace fwiflow {
 namespace {
 
 constexpr int kBZ = 64;  // threads along z (one wave64 row)
-constexpr int kCondBlocks = 64;  // misfit partial sums per shot of the conditioned residual
+constexpr int kCondBlocks = 512;  // misfit partial sums per shot of the conditioned residual
 constexpr float kC1 = 9.0f / 8.0f;
 constexpr float kC2 = 1.0f / 24.0f;
 

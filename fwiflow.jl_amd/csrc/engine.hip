@@ -1520,6 +1520,19 @@ __global__ void k_res_inject(Geo g, float* __restrict__ szz, float* __restrict__
 // without PML gates in the reference; they are read only through a*dpsi terms,
 // i.e. where a != 0 (the C-PML layers) plus the 2-cell reach of the stencil,
 // so they are kept in that band (`zlive`/`xlive`) and nowhere else.
+// Residual injection folded into the adjoint stress update (res_injection,
+// utilities.cu:569-580): at a receiver's cell szz_adj += res and sxx_adj +=
+// 3 res before the update reads them (the update reads only its own cell's
+// stresses, so the sum is the same as a separate injection pass).  Used when
+// every receiver lies in one row z and no two receivers of a shot share a
+// cell: at[shot][x] is then the receiver of column x (index into the
+// residual row) or -1.
+struct Inj {
+  const float* row;  // residual row of the step, all receivers; null: no injection
+  const int* at;     // [ns][nx]
+  int z;
+};
+
 struct AdjStressIn {
   float vz0, vzpx, vzp2x, vzmx, vzm1, vzp1, vzm2;
   float vx0, vxp1, vxp2, vxm1, vxmx, vxpx, vxm2x;
@@ -1529,7 +1542,8 @@ struct AdjStressIn {
 };
 
 template <int U>
-__device__ __forceinline__ void k_adj_stress_body(int bid, Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p) {
+__device__ __forceinline__ void k_adj_stress_body(int bid, Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p,
+                                                  Inj inj) {
   int tz, tx;
   const int chunk = tile_of(tl, bid, tz, tx);
   const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
@@ -1548,12 +1562,24 @@ __device__ __forceinline__ void k_adj_stress_body(int bid, Tiling tl, Geo g, Mdl
   const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
   const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
   const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
+  const bool inj_row = inj.row && z == inj.z;
   for (int s0 = sb; s0 < se; s0 += U) {
     AdjStressIn r[U];
+    float resv[U];
+    bool has[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
+      resv[u] = 0.f;
+      has[u] = false;
       if (s >= se) break;
+      if (inj_row) {
+        const int k = inj.at[(size_t)(g.s0 + s) * g.nx + x];
+        if (k >= 0) {
+          resv[u] = inj.row[k];
+          has[u] = true;
+        }
+      }
       const long long so = s * g.N;
       const float* vz = a.vz + so;
       const float* vx = a.vx + so;
@@ -1598,9 +1624,17 @@ __device__ __forceinline__ void k_adj_stress_body(int bid, Tiling tl, Geo g, Mdl
       if (zlive) sto(p.vxz + so, bo, bzh * q.m_vxz + sxz * amu * dt);
       const float dvx_dx = (-kC1 * (q.vx0 - q.vxmx) + kC2 * (q.vxpx - q.vxm2x)) * rdx;
       const float dvz_dz = (-kC1 * (q.vz0 - q.vzm1) + kC2 * (q.vzp1 - q.vzm2)) * rdz;
-      float sxx = q.sxx + bb * dvx_dx * rkxh * dt;
+      float sxx0 = q.sxx, szz0 = q.szz;
+      if (has[u]) {
+        // 3 res rounded before the add, as the separate injection adds it
+#pragma clang fp contract(off)
+        const float v3 = 3.f * resv[u];
+        szz0 += resv[u];
+        sxx0 += v3;
+      }
+      float sxx = sxx0 + bb * dvx_dx * rkxh * dt;
       if (axh != 0.f) sxx += axh * ((-kC1 * (q.q3[0] - q.q3[1]) + kC2 * (q.q3[2] - q.q3[3])) * rdx);
-      float szz = q.szz + ba * dvz_dz * rkzh * dt;
+      float szz = szz0 + ba * dvz_dz * rkzh * dt;
       if (azh != 0.f) szz += azh * ((-kC1 * (q.q4[0] - q.q4[1]) + kC2 * (q.q4[2] - q.q4[3])) * rdz);
       sto(a.sxx + so, bo, sxx);
       sto(a.szz + so, bo, szz);
@@ -1611,8 +1645,9 @@ __device__ __forceinline__ void k_adj_stress_body(int bid, Tiling tl, Geo g, Mdl
 }
 
 template <int U>
-__global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p) {
-  k_adj_stress_body<U>(blockIdx.x, tl, g, m, c, a, p);
+__global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p,
+                                                   Inj inj) {
+  k_adj_stress_body<U>(blockIdx.x, tl, g, m, c, a, p, inj);
 }
 
 // Interior checkpoints of the reverse sweep (no reference counterpart): the 5
@@ -1694,10 +1729,11 @@ template <int U>
 __global__ __launch_bounds__(256) void k_bwd_pair_b(int nA, int nB, int mix, Tiling tU, Tiling tR,
                                                    Geo g, Mdl m, Pml c, Wave w, Wave a, Mem p,
                                                    float* __restrict__ acc,
-                                                   const float* __restrict__ ring, int it_rev) {
+                                                   const float* __restrict__ ring, int it_rev,
+                                                   Inj inj) {
   int b;
   if (pair_part(blockIdx.x, nA, nB, mix, b)) {
-    if (b < tU.n * tU.chunks) k_adj_stress_body<U>(b, tU, g, m, c, a, p);
+    if (b < tU.n * tU.chunks) k_adj_stress_body<U>(b, tU, g, m, c, a, p, inj);
   } else if (b < tR.n) {
     k_rev_vel_body<U>(b, tR, g, m, w, a, acc + 3 * g.N, acc + 4 * g.N, ring, it_rev);
   }
@@ -1897,6 +1933,9 @@ struct DeviceState {
   double* cpart = nullptr;                 // [ns][kCondBlocks] misfit partial sums
   int nrec_max = 0;
   bool src_updated = false;                // the last run updated the source rows
+  bool fold_inject = true;                 // residual injection inside the adjoint stress update
+  int* rec_at = nullptr;                   // [ns][nx] receiver of each column (one-row surveys)
+  int rec_row = -1;                        // the row of those receivers
   std::vector<float> keep_obs, keep_syn;   // conditioned data [R][nSteps] for the scratch files
   float* illum = nullptr; // [NX][NZ]
   std::vector<int> roff;  // first receiver of each shot (+ total)
@@ -2135,6 +2174,24 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
     }
   }
   d.roff[g.ns] = (int)r;
+  d.fold_inject = env_int("FWIFLOW_FOLD_INJECT", 1) != 0;
+  {
+    // one-row surveys with at most one receiver per cell get the injection
+    // inside the adjoint stress update
+    bool one_row = d.R > 0;
+    const int row = d.R > 0 ? rec[0] : -1;
+    std::vector<int> at((size_t)g.ns * g.nx, -1);
+    for (int k = 0; k < d.R && one_row; k++) {
+      int& slot = at[(size_t)rec[3 * k + 2] * g.nx + rec[3 * k + 1]];
+      if (rec[3 * k] != row || slot >= 0) one_row = false;
+      slot = k;
+    }
+    if (one_row) {
+      TRY(d.alloc(&d.rec_at, at.size()));
+      HIP_TRY(hipMemcpy(d.rec_at, at.data(), at.size() * sizeof(int), hipMemcpyHostToDevice));
+      d.rec_row = row;
+    }
+  }
   TRY(d.alloc(&d.src, src.size()));
   HIP_TRY(hipMemcpy(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
   TRY(d.alloc(&d.rec, rec.size()));
@@ -2724,36 +2781,43 @@ Status Plan::run(int calc_id) {
             hipLaunchKernelGGL(k_res_inject, rgrid, dim3(256), 0, st, g, a_all.szz, a_all.sxx,
                                d.rec + 3 * r0, Rg, d.res + (size_t)t * d.R + r0);
         };
+        // with backward pairs the residual of step t is injected by the
+        // adjoint stress update of step t itself (k_bwd_pair_b, k_adj_stress)
+        const bool fold = d.fold_inject && d.rec_at && d.pairs && g.nSteps >= 2;
+        auto inj_at = [&](int t) {
+          return Inj{fold ? d.res + (size_t)t * d.R : nullptr, d.rec_at, d.rec_row};
+        };
+        const Inj none{nullptr, d.rec_at, -1};
         // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
         LAUNCH_U(k_adj_vel, grdUg, blk, tUg, gg, m, c, a, pg, sg, sgrad, -1);
-        inject(g.nSteps - 1);
+        if (!fold) inject(g.nSteps - 1);
         if (d.pairs && g.nSteps >= 2) {
           const int nU8 = (tUg.n * tUg.chunks + 7) / 8 * 8, nR8 = (tR.n + 7) / 8 * 8;
           const dim3 grdP(nR8 + nU8);
           const int mix = d.mix ? 1 : 0, mix_b = d.mix_b ? 1 : 0;
           // adj_stress(nSteps-1) with rev_vel(nSteps-2)
           LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix_b, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
-                   g.nSteps - 2);
+                   g.nSteps - 2, inj_at(g.nSteps - 1));
           for (int it = g.nSteps - 2; it >= 0; it--) {
             const bool smp = (it % sample_every) == 0;
             d.launch("bwd_pair_a", smp, [&] {
               LAUNCH_U(k_bwd_pair_a, grdP, blk, nR8, nU8, mix, tR, tUg, gg, m, c, wg, a, pg, sg, acc,
                        ring, sgrad, it);
             });
-            inject(it);
+            if (!fold) inject(it);
             reset(it);
             if (it >= 1) {
               d.launch("bwd_pair_b", smp, [&] {
                 LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix_b, tUg, tR, gg, m, c, wg, a, pg, acc,
-                         ring, it - 1);
+                         ring, it - 1, inj_at(it));
               });
             } else {
-              LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg);
+              LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg, inj_at(0));
             }
           }
           continue;
         }
-        LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg);
+        LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg, none);
         for (int it = g.nSteps - 2; it >= 0; it--) {
           const bool smp = (it % sample_every) == 0;
           d.launch("rev_vel", smp, [&] {
@@ -2769,7 +2833,7 @@ Status Plan::run(int calc_id) {
           });
           inject(it);
           d.launch("adj_stress", smp, [&] {
-            LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg);
+            LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg, none);
           });
         }
       }

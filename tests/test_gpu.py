@@ -405,10 +405,12 @@ def _run_plan(para, ids, lam, mu, den, stf, env):
                                      {"FWIFLOW_SHOTS_PER_THREAD": "2"},
                                      {"FWIFLOW_BWD_PAIRS": "0"},
                                      {"FWIFLOW_XCD_BALANCE": "0", "FWIFLOW_TILE_X": "2"},
-                                     {"FWIFLOW_PAIR_MIX": "0", "FWIFLOW_PAIR_MIX_B": "1"}],
+                                     {"FWIFLOW_PAIR_MIX": "0", "FWIFLOW_PAIR_MIX_B": "1"},
+                                     {"FWIFLOW_FOLD_INJECT": "0"}],
                          ids=["column7", "column32", "tiled", "group_all_unroll2",
                               "group2_unroll4", "group1_tiled", "chunks_of_2", "unpaired",
-                              "unbalanced_tile_x2", "pair_halves_in_order"])
+                              "unbalanced_tile_x2", "pair_halves_in_order",
+                              "separate_injection"])
 def test_fused_kernels_match_split_kernels(workspace, variant):
     """The fused kernels (column-marching fused forward/reverse/adjoint steps
     with odd or even segment widths, or the tiled fused forward step; all
@@ -417,7 +419,9 @@ def test_fused_kernels_match_split_kernels(workspace, variant):
     (all shots in one launch, ragged groups of 2, single shots), shot
     batchings of the split kernels' loads (2 or 4 shots, partial batches)
     shot ranges spread over workgroups (ragged chunks of 2) and the backward
-    kernels launched one by one instead of in pairs."""
+    kernels launched one by one instead of in pairs.  The residual injected
+    by its own launch instead of inside the adjoint stress update gives the
+    same numbers bit for bit."""
     c = models.case_small(nz=40, nx=61, nSteps=240, nshots=3)
     para, stf = models.write_case(c, workspace)
     lam, mu, den = models.moduli(c, "true")
@@ -428,8 +432,11 @@ def test_fused_kernels_match_split_kernels(workspace, variant):
                            {"FWIFLOW_FUSED": "0", "FWIFLOW_TILED": "0", "FWIFLOW_SHOT_GROUP": "4",
                             "FWIFLOW_SHOT_UNROLL": "1", "FWIFLOW_SHOTS_PER_THREAD": "0",
                             "FWIFLOW_BWD_PAIRS": "1", "FWIFLOW_XCD_BALANCE": "1",
-                            "FWIFLOW_PAIR_MIX": "1", "FWIFLOW_PAIR_MIX_B": "0"})
+                            "FWIFLOW_PAIR_MIX": "1", "FWIFLOW_PAIR_MIX_B": "0",
+                            "FWIFLOW_FOLD_INJECT": "1"})
     d1, g1, J1 = _run_plan(para, ids, lam0, mu0, den0, stf, variant)
+    if variant.get("FWIFLOW_FOLD_INJECT") == "0":
+        assert J1 == J0 and all(np.array_equal(a, b) for a, b in zip(g1, g0))
     assert rel(d1, d0) < 1e-6
     assert abs(J1 - J0) <= 1e-6 * abs(J0)
     for a, b in zip(g1, g0):

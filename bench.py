@@ -50,8 +50,11 @@ KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_str
 # scaled by a 1 GiB copy's counters) at this workload with 4 shots per launch:
 # profiles/stencil_kernels_608x1780_4shot_groups_counters.md.  Reported as
 # roofline.traffic when the dominant kernel and shots per launch match.
-MEASURED_TRAFFIC_4SHOTS = {"bwd_pair_a": 361.6e6, "bwd_pair_b": 335.6e6,
-                           "stress_fwd": 167.8e6, "vel_fwd": 147.6e6}
+MEASURED_TRAFFIC = {("bwd_pair_a", 4): 361.6e6, ("bwd_pair_b", 4): 335.6e6,
+                    ("stress_fwd", 4): 167.8e6, ("vel_fwd", 4): 147.6e6}
+# kernels of the backward sweep (their launches cover the backward shot groups)
+BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "rev_step", "adj_step",
+            "bwd_pair_a", "bwd_pair_b")
 # C-PML memory-variable traffic per PML cell-direction (2 variables RW = 16 B)
 KERNEL_PML = {"fwd_tile": 32.0, "fwd_step": 32.0, "adj_step": 32.0, "stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
 
@@ -192,8 +195,9 @@ def main():
         if n:
             kt[name] = (ms / n, n)
     # every kernel runs once per time step per shot group
-    spl = P.shots_per_launch()
+    spl_f, spl_b = P.shots_per_launch()
     dominant = max(kt, key=lambda k: kt[k][0])
+    spl = spl_b if dominant in BACKWARD else spl_f
     avg_ms = kt[dominant][0]
     lb = kernel_launch_bytes(dominant, c, spl)
     achieved = lb / (avg_ms * 1e-3) / 1e9
@@ -229,8 +233,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (MEASURED_TRAFFIC_4SHOTS.get(dominant)
-                        if spl == 4 and c.NZ == 608 and c.NX == 1780 else None),
+            "traffic": (MEASURED_TRAFFIC.get((dominant, spl))
+                        if c.NZ == 608 and c.NX == 1780 else None),
             "avg_launch_ms": round(avg_ms, 4),
             "bytes_per_launch": lb,
             "shots_per_launch": spl,

@@ -212,9 +212,10 @@ int fwiflow_plan_set_illumination(fwiflow_plan* plan, int enable) {
 int fwiflow_plan_illumination(fwiflow_plan* plan, double* illum) {
   PLAN_CALL(plan->impl.illumination(illum));
 }
-int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* shots) {
-  if (!plan || !shots) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
-  *shots = plan->impl.shots_per_launch();
+int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* forward, int* backward) {
+  if (!plan) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
+  if (forward) *forward = plan->impl.shots_per_launch();
+  if (backward) *backward = plan->impl.shots_per_launch_bwd();
   return report(Status::Ok());
 }
 

@@ -1907,6 +1907,7 @@ struct DeviceState {
   bool fused_rev = true;  // fused reverse step (needs its LDS to fit)
   int unroll = 1;         // shots whose loads a split-kernel thread issues together
   int shot_group = -1;    // shots per launch of the split/tiled kernels (0: all, -1: auto)
+  int shot_group_bwd = -1;  // the same for the backward sweep (0: all, -1: auto)
   int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
   bool pairs = true;      // backward kernel pairs share a launch
   bool mix = true;        // ... with their workgroups interleaved round by round (pair a;
@@ -2081,7 +2082,22 @@ struct DeviceState {
   int group_size() const {
     if (fused || shot_group == 0) return g.ns;
     if (shot_group > 0) return std::min(shot_group, g.ns);
-    const double per_shot = 48.0 * (double)g.N;
+    // ~40 B per cell and shot stay cache-resident from step to step in the
+    // forward sweep (5 at 608x1780: 4.9 groups-of-5 measured 7.8 us per
+    // shot-step against 8.1 for 4 and 9.3 for 8)
+    const double per_shot = 40.0 * (double)g.N;
+    const int n = (int)std::lround(224e6 / per_shot);
+    return std::max(1, std::min(n, g.ns));
+  }
+  // The backward sweep holds forward and adjoint state of its group: fewer
+  // shots fit the 256 MB Infinity Cache (3 at 608x1780, where 4 shots'
+  // 4 x 43 MB of state plus model and accumulators overflow it).
+  int group_size_bwd() const {
+    if (fused || (shot_group_bwd < 0 && shot_group == 0)) return g.ns;
+    if (shot_group_bwd == 0) return g.ns;
+    if (shot_group_bwd > 0) return std::min(shot_group_bwd, g.ns);
+    if (shot_group > 0) return std::min(shot_group, g.ns);
+    const double per_shot = 64.0 * (double)g.N;
     const int n = (int)std::lround(224e6 / per_shot);
     return std::max(1, std::min(n, g.ns));
   }
@@ -2118,6 +2134,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
     d.unroll = u >= 4 ? 4 : (u >= 2 ? 2 : 1);
   }
   d.shot_group = std::max(-1, env_int("FWIFLOW_SHOT_GROUP", -1));
+  d.shot_group_bwd = std::max(-1, env_int("FWIFLOW_SHOT_GROUP_BWD", -1));
   d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
   d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
   d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
@@ -2432,6 +2449,7 @@ Status Plan::timing(const std::string& name, double* ms, long long* count) {
 long long Plan::device_bytes() const { return d_ ? d_->bytes : 0; }
 
 int Plan::shots_per_launch() const { return d_ ? d_->group_size() : 0; }
+int Plan::shots_per_launch_bwd() const { return d_ ? d_->group_size_bwd() : 0; }
 
 // Residual with the optional conditioning stages, one shot at a time
 // (libCUFD.cu:254-313): window obs and syn (per-trace windows and weights, or
@@ -2613,8 +2631,11 @@ Status Plan::run(int calc_id) {
   } else {
     float* fa0 = fa;
     float* ma0 = ma;
-    for (int s0 = 0; s0 < g.ns; s0 += G) {
-      const int n = std::min(G, g.ns - s0);
+    // ceil(ns / G) groups of balanced sizes (64 shots by 3: 20 x 3 + 2 x 2)
+    const int NGf = (g.ns + G - 1) / G;
+    for (int k = 0; k < NGf; k++) {
+      const int s0 = (int)((long long)k * g.ns / NGf);
+      const int n = (int)((long long)(k + 1) * g.ns / NGf) - s0;
       const Geo gg = d.group_geo(s0, n);
       Tiling tUg = tU;  // forward/adjoint split kernels: shots spread over chunks
       // (one chunk when the illumination is summed: one writer per cell)
@@ -2754,8 +2775,11 @@ Status Plan::run(int calc_id) {
       acc = d.accp;
     } else {
       const Wave a_all = d.wave(d.adj);
-      for (int s0 = 0; s0 < g.ns; s0 += G) {
-        const int n = std::min(G, g.ns - s0);
+      const int GB = d.group_size_bwd();
+      const int NGb = (g.ns + GB - 1) / GB;
+      for (int k = 0; k < NGb; k++) {
+        const int s0 = (int)((long long)k * g.ns / NGb);
+        const int n = (int)((long long)(k + 1) * g.ns / NGb) - s0;
         const Geo gg = d.group_geo(s0, n);
         Tiling tUg = tU;  // forward/adjoint split kernels: shots spread over chunks
         tUg.spt = d.spt > 0 ? std::min(d.spt, n) : n;

@@ -33,7 +33,8 @@ class Plan {
   Status set_timing(bool on);
   Status timing(const std::string& name, double* ms, long long* count);
   long long device_bytes() const;
-  int shots_per_launch() const;
+  int shots_per_launch() const;      // forward sweep
+  int shots_per_launch_bwd() const;  // backward sweep
   Status set_illumination(bool on);
   Status illumination(double* out);
 

@@ -85,7 +85,7 @@ def load():
     L.fwiflow_plan_set_timing.argtypes = [vp, C.c_int]
     L.fwiflow_plan_timing.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_longlong)]
     L.fwiflow_plan_device_bytes.argtypes = [vp, C.POINTER(C.c_longlong)]
-    L.fwiflow_plan_shots_per_launch.argtypes = [vp, C.POINTER(C.c_int)]
+    L.fwiflow_plan_shots_per_launch.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fwiflow_plan_set_illumination.argtypes = [vp, C.c_int]
     L.fwiflow_plan_illumination.argtypes = [vp, dp]
     L.fwiflow_cnn_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint]

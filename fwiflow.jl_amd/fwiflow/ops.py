@@ -208,9 +208,10 @@ class Plan:
         return out
 
     def shots_per_launch(self):
-        n = _lib.C.c_int(0)
-        check(self._L.fwiflow_plan_shots_per_launch(self._h, _lib.C.byref(n)))
-        return int(n.value)
+        """(forward, backward) shots per launch of the time-step kernels."""
+        f, b = _lib.C.c_int(0), _lib.C.c_int(0)
+        check(self._L.fwiflow_plan_shots_per_launch(self._h, _lib.C.byref(f), _lib.C.byref(b)))
+        return int(f.value), int(b.value)
 
 
 try:  # torch is plumbing for autodiff callers; the op itself does not need it

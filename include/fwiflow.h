@@ -145,11 +145,12 @@ int fwiflow_plan_set_timing(fwiflow_plan* plan, int enable);
 int fwiflow_plan_timing(fwiflow_plan* plan, const char* name, double* total_ms, long long* count);
 /* Device bytes held by the plan. */
 int fwiflow_plan_device_bytes(fwiflow_plan* plan, long long* bytes);
-/* Shots covered by one launch of the time-step kernels (the engine runs the
- * plan's shots in groups of this size; each group runs every time step before
- * the next).  No reference counterpart: it sizes per-launch byte counts for
+/* Shots covered by one launch of the time-step kernels of the forward and of
+ * the backward sweep (the engine runs the plan's shots in groups of these
+ * sizes; each group runs every time step before the next).  Either pointer
+ * may be NULL.  No reference counterpart: it sizes per-launch byte counts for
  * kernel timing. */
-int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* shots);
+int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* forward, int* backward);
 /* Forward illumination: with enable != 0, later runs also sum, over the
  * plan's shots and the steps it = 0..nSteps-2, vz^2 + vx^2 after each
  * velocity update on the update region [2, NZ-nPad-3] x [2, NX-3] (zero

@@ -1908,6 +1908,12 @@ struct DeviceState {
   int unroll = 1;         // shots whose loads a split-kernel thread issues together
   int shot_group = -1;    // shots per launch of the split/tiled kernels (0: all, -1: auto)
   int shot_group_bwd = -1;  // the same for the backward sweep (0: all, -1: auto)
+  // Two streams (default priority both): shot groups alternate between them,
+  // so one stream's kernel fills the other's launch gap; the backward groups
+  // of the second stream accumulate into acc + 5N.
+  int streams = 1;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t join[2] = {nullptr, nullptr};
   int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
   bool pairs = true;      // backward kernel pairs share a launch
   bool mix = true;        // ... with their workgroups interleaved round by round (pair a;
@@ -1981,7 +1987,10 @@ struct DeviceState {
       (void)hipEventDestroy(r.b);
     }
     for (auto e : pool) (void)hipEventDestroy(e);
+    for (auto e : join)
+      if (e) (void)hipEventDestroy(e);  // a null handle would leave a sticky error
     for (void* p : allocs) (void)hipFree(p);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -2135,6 +2144,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   }
   d.shot_group = std::max(-1, env_int("FWIFLOW_SHOT_GROUP", -1));
   d.shot_group_bwd = std::max(-1, env_int("FWIFLOW_SHOT_GROUP_BWD", -1));
+  d.streams = env_int("FWIFLOW_STREAMS", 1) == 2 ? 2 : 1;
   d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
   d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
   d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
@@ -2142,6 +2152,10 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
+  if (d.streams == 2) {
+    HIP_TRY(hipStreamCreate(&d.stream2));
+    for (auto& e : d.join) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   Geo& g = d.g;
   g.nz = para.nz;
   g.nx = para.nx;
@@ -2539,7 +2553,7 @@ Status Plan::run(int calc_id) {
     if (!d.adj) TRY(d.alloc(&d.adj, 5 * F));
     if (d.fused && !d.adj2) TRY(d.alloc(&d.adj2, 5 * F));
     if (!d.ring) TRY(d.alloc(&d.ring, (size_t)g.nSteps * 5 * g.ns * (size_t)g.ringlen));
-    if (!d.acc) TRY(d.alloc(&d.acc, 5 * N));
+    if (!d.acc) TRY(d.alloc(&d.acc, (size_t)(d.streams == 2 ? 10 : 5) * N));
     if (!d.grad) TRY(d.alloc(&d.grad, 3 * N));
     if (!d.stf_grad) TRY(d.alloc(&d.stf_grad, (size_t)g.ns * g.nSteps));
   }
@@ -2604,6 +2618,8 @@ Status Plan::run(int calc_id) {
   // one group stays resident in the last-level cache from step to step.
   const int G = d.group_size();
   const int U = d.unroll;
+  // two streams for the split kernels (the illumination has one writer per cell)
+  const bool two = d.streams == 2 && !d.fused && !d.tiled && !d.illum_on;
 #define LAUNCH_U(K, grid, block, ...)                                        \
   do {                                                                       \
     if (U == 4)                                                              \
@@ -2633,7 +2649,13 @@ Status Plan::run(int calc_id) {
     float* ma0 = ma;
     // ceil(ns / G) groups of balanced sizes (64 shots by 3: 20 x 3 + 2 x 2)
     const int NGf = (g.ns + G - 1) / G;
+    if (two) {
+      HIP_TRY(hipEventRecord(d.join[0], st));
+      HIP_TRY(hipStreamWaitEvent(d.stream2, d.join[0], 0));
+    }
     for (int k = 0; k < NGf; k++) {
+      const bool second = two && (k & 1);
+      hipStream_t st = second ? d.stream2 : d.stream;  // the launches below use it
       const int s0 = (int)((long long)k * g.ns / NGf);
       const int n = (int)((long long)(k + 1) * g.ns / NGf) - s0;
       const Geo gg = d.group_geo(s0, n);
@@ -2649,7 +2671,7 @@ Status Plan::run(int calc_id) {
       fa = fa0, fb = d.fwd2, ma = ma0, mb = d.mem2;
       float* ckg = ck_k > 0 ? d.ck + (size_t)s0 * ck_slots * 5 * ck_cells : nullptr;
       for (int it = 0; it <= g.nSteps - 2; it++) {
-        const bool smp = (it % sample_every) == 0;
+        const bool smp = !second && (it % sample_every) == 0;  // events on the first stream
         float* out = fa;  // buffer that holds step it+1 after this step
         if (ckg && it > 0 && it % ck_k == 0)
           hipLaunchKernelGGL(k_ckpt, dim3(tI.n), blk, 0, st, tI, gg, d.wave(fa, s0), ckg,
@@ -2687,6 +2709,10 @@ Status Plan::run(int calc_id) {
         }
       }
     }
+    if (two) {
+      HIP_TRY(hipEventRecord(d.join[1], d.stream2));
+      HIP_TRY(hipStreamWaitEvent(st, d.join[1], 0));
+    }
   }
   HIP_TRY(hipGetLastError());
   const Wave w = d.wave(fa);  // final forward state
@@ -2710,7 +2736,7 @@ Status Plan::run(int calc_id) {
     // ------------------------------------------------------- backward
     HIP_TRY(hipMemsetAsync(d.adj, 0, 5 * F * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(ma, 0, 8 * F * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(d.acc, 0, 5 * N * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(d.acc, 0, (two ? 10 : 5) * N * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(d.stf_grad, 0, (size_t)g.ns * g.nSteps * sizeof(float), st));
     const Tiling tR = make_tiling(g.ringH, g.ringW, d.tile_x, d.xcd);
     const dim3 grdR(tR.n);
@@ -2777,7 +2803,15 @@ Status Plan::run(int calc_id) {
       const Wave a_all = d.wave(d.adj);
       const int GB = d.group_size_bwd();
       const int NGb = (g.ns + GB - 1) / GB;
+      if (two) {
+        HIP_TRY(hipEventRecord(d.join[0], st));
+        HIP_TRY(hipStreamWaitEvent(d.stream2, d.join[0], 0));
+      }
+      float* const acc0 = acc;
       for (int k = 0; k < NGb; k++) {
+        const bool second = two && (k & 1);
+        hipStream_t st = second ? d.stream2 : d.stream;  // the launches below use it
+        float* acc = second ? acc0 + 5 * N : acc0;        // and these accumulators
         const int s0 = (int)((long long)k * g.ns / NGb);
         const int n = (int)((long long)(k + 1) * g.ns / NGb) - s0;
         const Geo gg = d.group_geo(s0, n);
@@ -2823,7 +2857,7 @@ Status Plan::run(int calc_id) {
           LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix_b, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
                    g.nSteps - 2, inj_at(g.nSteps - 1));
           for (int it = g.nSteps - 2; it >= 0; it--) {
-            const bool smp = (it % sample_every) == 0;
+            const bool smp = !second && (it % sample_every) == 0;
             d.launch("bwd_pair_a", smp, [&] {
               LAUNCH_U(k_bwd_pair_a, grdP, blk, nR8, nU8, mix, tR, tUg, gg, m, c, wg, a, pg, sg, acc,
                        ring, sgrad, it);
@@ -2843,7 +2877,7 @@ Status Plan::run(int calc_id) {
         }
         LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg, none);
         for (int it = g.nSteps - 2; it >= 0; it--) {
-          const bool smp = (it % sample_every) == 0;
+          const bool smp = !second && (it % sample_every) == 0;
           d.launch("rev_vel", smp, [&] {
             LAUNCH_U(k_rev_vel, grdR, blk, tR, gg, m, wg, a, acc + 3 * N, acc + 4 * N, ring, it);
           });
@@ -2860,6 +2894,13 @@ Status Plan::run(int calc_id) {
             LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg, none);
           });
         }
+      }
+      if (two) {
+        HIP_TRY(hipEventRecord(d.join[1], d.stream2));
+        HIP_TRY(hipStreamWaitEvent(st, d.join[1], 0));
+        const long long n5 = 5 * (long long)N;
+        hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)((n5 + 255) / 256)), dim3(256), 0, st,
+                           acc0, n5, 2);
       }
     }
 #undef LAUNCH_U

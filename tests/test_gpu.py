@@ -407,11 +407,13 @@ def _run_plan(para, ids, lam, mu, den, stf, env):
                                      {"FWIFLOW_XCD_BALANCE": "0", "FWIFLOW_TILE_X": "2"},
                                      {"FWIFLOW_PAIR_MIX": "0", "FWIFLOW_PAIR_MIX_B": "1"},
                                      {"FWIFLOW_FOLD_INJECT": "0"},
-                                     {"FWIFLOW_SHOT_GROUP": "2", "FWIFLOW_SHOT_GROUP_BWD": "1"}],
+                                     {"FWIFLOW_SHOT_GROUP": "2", "FWIFLOW_SHOT_GROUP_BWD": "1"},
+                                     {"FWIFLOW_STREAMS": "2", "FWIFLOW_SHOT_GROUP": "1",
+                                      "FWIFLOW_SHOT_GROUP_BWD": "1"}],
                          ids=["column7", "column32", "tiled", "group_all_unroll2",
                               "group2_unroll4", "group1_tiled", "chunks_of_2", "unpaired",
                               "unbalanced_tile_x2", "pair_halves_in_order",
-                              "separate_injection", "groups_2_fwd_1_bwd"])
+                              "separate_injection", "groups_2_fwd_1_bwd", "two_streams"])
 def test_fused_kernels_match_split_kernels(workspace, variant):
     """The fused kernels (column-marching fused forward/reverse/adjoint steps
     with odd or even segment widths, or the tiled fused forward step; all
@@ -421,7 +423,8 @@ def test_fused_kernels_match_split_kernels(workspace, variant):
     batchings of the split kernels' loads (2 or 4 shots, partial batches)
     shot ranges spread over workgroups (ragged chunks of 2) and the backward
     kernels launched one by one instead of in pairs, and backward shot
-    groups that differ from the forward ones.  The residual injected
+    groups that differ from the forward ones, and single-shot groups
+    alternating between two streams.  The residual injected
     by its own launch instead of inside the adjoint stress update gives the
     same numbers bit for bit."""
     c = models.case_small(nz=40, nx=61, nSteps=240, nshots=3)

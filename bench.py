@@ -47,11 +47,13 @@ KERNEL_BYTES = {
 # kernels that run two of the above as one grid (disjoint workgroup ranges)
 KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_stress", "rev_vel")}
 # Measured HBM-side bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, each
-# scaled by a 1 GiB copy's counters) at this workload with 4 shots per launch:
-# profiles/stencil_kernels_608x1780_4shot_groups_counters.md.  Reported as
-# roofline.traffic when the dominant kernel and shots per launch match.
-MEASURED_TRAFFIC = {("bwd_pair_a", 4): 361.6e6, ("bwd_pair_b", 4): 335.6e6,
-                    ("stress_fwd", 4): 167.8e6, ("vel_fwd", 4): 147.6e6}
+# scaled by a 1 GiB copy's counters, averaged over the launches) at this
+# workload with forward groups of 5 and backward groups of 3 nominal shots
+# (64 shots: 13 groups of 4.92, 22 of 2.91 on average):
+# profiles/stencil_kernels_608x1780_5fwd_3bwd_groups_counters.md.  Reported
+# as roofline.traffic when the dominant kernel and group size match.
+MEASURED_TRAFFIC = {("bwd_pair_a", 3): 234.2e6, ("bwd_pair_b", 3): 256.3e6,
+                    ("stress_fwd", 5): 204.8e6, ("vel_fwd", 5): 181.1e6}
 # kernels of the backward sweep (their launches cover the backward shot groups)
 BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "rev_step", "adj_step",
             "bwd_pair_a", "bwd_pair_b")
@@ -198,8 +200,11 @@ def main():
     spl_f, spl_b = P.shots_per_launch()
     dominant = max(kt, key=lambda k: kt[k][0])
     spl = spl_b if dominant in BACKWARD else spl_f
+    # the engine splits the shots into ceil(ns / spl) groups of balanced
+    # sizes: a launch covers ns / groups shots on average
+    mean_spl = a.nshots / -(-a.nshots // spl)
     avg_ms = kt[dominant][0]
-    lb = kernel_launch_bytes(dominant, c, spl)
+    lb = kernel_launch_bytes(dominant, c, mean_spl)
     achieved = lb / (avg_ms * 1e-3) / 1e9
     grad_bytes = algorithmic_bytes_per_shot_gradient(c, c.nSteps)
     job_gbs = grad_bytes * value / 1e9
@@ -237,7 +242,7 @@ def main():
                         if c.NZ == 608 and c.NX == 1780 else None),
             "avg_launch_ms": round(avg_ms, 4),
             "bytes_per_launch": lb,
-            "shots_per_launch": spl,
+            "shots_per_launch": round(mean_spl, 3),
         },
         "job_roofline": {
             "algorithmic_bytes_per_shot_gradient": grad_bytes,

@@ -32,6 +32,8 @@ CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-param
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-ffp-contract=on", "-munsafe-fp-atomics"]
+# extra compiler definitions for A/B builds of engine switches (space separated)
+HIPFLAGS += os.environ.get("FWIFLOW_HIPFLAGS", "").split()
 
 
 def _run(cmd):

@@ -156,8 +156,9 @@ def main():
         copy_calibration()
         return
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(shots would shard one group per GPU, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {a.gpus} is not supported (shots are "
+              "independent: each GPU would run a plan for its own shot range and the "
+              "gradients would be summed once per evaluation)", file=sys.stderr)
         sys.exit(2)
     import fwiflow
     from fwiflow import models

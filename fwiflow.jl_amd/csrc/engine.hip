@@ -1916,8 +1916,8 @@ struct DeviceState {
   hipEvent_t join[2] = {nullptr, nullptr};
   int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
   bool pairs = true;      // backward kernel pairs share a launch
-  bool mix = true;        // ... with their workgroups interleaved round by round (pair a;
-  bool mix_b = false;     // pair b measured slower interleaved: 55.1 vs 53.4 us)
+  bool mix = true;        // ... with their workgroups interleaved round by round (pair a,
+  bool mix_b = true;      // and pair b: slower with 4-shot groups, faster with 3)
   bool balance = true;    // C-PML columns shared evenly by the XCDs (update-region launches)
   int* perm_u = nullptr;  // workgroup -> tile order of the update-region tiling
   bool illum_on = false;  // sum the forward illumination
@@ -2148,7 +2148,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
   d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
   d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
-  d.mix_b = env_int("FWIFLOW_PAIR_MIX_B", 0) != 0;
+  d.mix_b = env_int("FWIFLOW_PAIR_MIX_B", 1) != 0;
   d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));

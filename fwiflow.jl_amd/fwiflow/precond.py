@@ -222,3 +222,118 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
             break
     return {"x": x.reshape(shape), "misfit": J, "iterations": len(hist), "evaluations": nev,
             "seconds": time.perf_counter() - t0, "history": hist}
+
+
+def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, memory=5,
+                  target_misfit=None, first_step_rms=10.0, c_armijo=1e-4, max_backtracks=4,
+                  illum_eps=1e-3):
+    """L-BFGS on cp with the network as the initial inverse Hessian.
+
+    The two-loop recursion (Nocedal & Wright, Alg. 7.4) applies H0 to the
+    partly corrected gradient q: with a network, H0 q = gamma * RMS(q) *
+    (-net(features(q)))'s unit-RMS direction (the network predicts an update,
+    i.e. -H g); without one (``net=None``, the control), H0 q = gamma * q /
+    (illumination + eps), the classic pseudo-Hessian scaling.  gamma comes
+    from the latest curvature pair (s'y / y'y, scaled to H0's RMS response),
+    or makes the first step ``first_step_rms`` m/s RMS.  The line search is
+    backtracking Armijo from alpha = 1 with quadratic interpolation and no
+    curvature condition (the reference's gradient is approximate, so a strong
+    Wolfe search stalls); pairs with s'y <= 0 are skipped, and a direction
+    that is not a descent direction falls back to H0 g alone, then to the
+    illumination-scaled gradient.  Returns the dict of ``learned_descent``.
+    """
+    shape = np.asarray(x0).shape
+    m2 = np.asarray(mask)
+    mflat = m2.astype(np.float64).ravel()
+    x = np.clip(np.asarray(x0, dtype=np.float64).ravel(), *bounds)
+    lo, hi = bounds
+    t0 = time.perf_counter()
+    nev = 0
+
+    def evaluate(v):
+        nonlocal nev
+        nev += 1
+        J, g, ex = objective(v)
+        return float(J), np.asarray(g, dtype=np.float64).ravel() * mflat, ex
+
+    def rms(v):
+        return _rms(v.reshape(shape), m2)
+
+    J, g, ex = evaluate(x)
+    S, Y = [], []
+    hist = []
+
+    def h0(q, ex, gamma):
+        il = np.asarray(ex.get("illumination", np.ones(shape)), dtype=np.float64).ravel()
+        if net is None:
+            return gamma * q / (il + illum_eps * max(il.max(), 1e-30)) * mflat
+        grads = ex.get("gradients")
+        gq = (q.reshape(shape),) + tuple(grads[1:] if grads is not None
+                                         else (np.zeros(shape), np.zeros(shape)))
+        d = net.forward(features(gq, il.reshape(shape), m2)).astype(np.float64)[0].ravel()
+        d = d * mflat
+        d /= rms(d)
+        return -gamma * rms(q) * d  # H0 q: the network's update is -H q
+
+    def gamma_now(ex):
+        if S:
+            s, y = S[-1], Y[-1]
+            # scale H0 so that H0 y has the size of s along y
+            r0 = h0(y, ex, 1.0)
+            den = float(np.dot(y, r0))
+            return float(np.dot(s, y)) / den if den > 0 else 1.0
+        r0 = h0(g, ex, 1.0)
+        return first_step_rms / max(rms(r0), 1e-30)
+
+    for _ in range(maxiter):
+        gamma = gamma_now(ex)
+        q = g.copy()
+        alphas = []
+        for s, y in reversed(list(zip(S, Y))):
+            rho = 1.0 / float(np.dot(y, s))
+            a = rho * float(np.dot(s, q))
+            alphas.append((a, rho, s, y))
+            q -= a * y
+        r = h0(q, ex, gamma)
+        for a, rho, s, y in reversed(alphas):
+            b = rho * float(np.dot(y, r))
+            r += s * (a - b)
+        candidates = [(-r, "lbfgs")]
+        if S:
+            candidates.append((-h0(g, ex, gamma), "h0"))
+        il = np.asarray(ex.get("illumination", np.ones(shape)), dtype=np.float64).ravel()
+        dp = -g / (il + illum_eps * max(il.max(), 1e-30)) * mflat
+        candidates.append((dp * (first_step_rms / rms(dp)), "fallback"))
+        res = None
+        for d, used in candidates:
+            slope = float(np.dot(g, d))
+            if not slope < 0:
+                continue
+            alpha = 1.0
+            for _ in range(max_backtracks + 1):
+                xn = np.clip(x + alpha * d, lo, hi)
+                Jn, gn, exn = evaluate(xn)
+                if Jn < J and Jn <= J + c_armijo * float(np.dot(g, xn - x)):
+                    res = (xn, Jn, gn, exn, used)
+                    break
+                curv = Jn - J - slope * alpha
+                a_q = -slope * alpha * alpha / (2.0 * curv) if curv > 0 else 0.5 * alpha
+                alpha = min(max(a_q, 0.1 * alpha), 0.5 * alpha)
+            if res is not None:
+                break
+        if res is None:
+            break
+        xn, Jn, gn, exn, used = res
+        s, y = xn - x, gn - g
+        if float(np.dot(s, y)) > 1e-12 * float(np.dot(y, y)):
+            S.append(s)
+            Y.append(y)
+            if len(S) > memory:
+                S.pop(0)
+                Y.pop(0)
+        x, J, g, ex = xn, Jn, gn, exn
+        hist.append((time.perf_counter() - t0, J, used))
+        if target_misfit is not None and J <= target_misfit:
+            break
+    return {"x": x.reshape(shape), "misfit": J, "iterations": len(hist), "evaluations": nev,
+            "seconds": time.perf_counter() - t0, "history": hist}

@@ -97,3 +97,42 @@ def test_evaluation_cache_keeps_a_restored_point():
     c.put(pts[4], 4)
     c.put(pts[5], 5)        # evicts 2 then 0
     assert c.get(pts[0]) is None and c.get(pts[5]) == 5
+
+
+def test_learned_lbfgs_control_converges_on_bounded_quadratic():
+    """The two-loop recursion with the illumination-scaled H0 (no network)
+    reaches the projected minimum of a bounded quadratic."""
+    from fwiflow import precond
+    f, A, xs, _ = _quadratic(n=40, seed=3)
+    x0 = np.full(xs.size, 3000.0)
+    mask = np.ones(xs.size)
+    r = precond.learned_lbfgs(f, x0, mask, net=None, maxiter=300, memory=8)
+    x = r["x"]
+    assert np.all(x >= 1500.0) and np.all(x <= 5500.0)
+    g = A.T @ (A @ (x - xs))
+    free = (x > 1500.0 + 1e-3) & (x < 5500.0 - 1e-3)
+    assert np.max(np.abs(g[free])) < 1e-3 * np.max(np.abs(A.T @ A @ xs))
+    ref = inversion.lbfgs(f, x0, maxiter=500, ftol=1e-14, gtol=1e-10)["misfit"]
+    assert r["misfit"] <= ref * (1 + 1e-9)
+
+
+class _IdentityNet:
+    """Stands in for the network: returns the negated cp gradient plane."""
+
+    def forward(self, x):
+        return np.stack([-x[0], np.zeros_like(x[0]), np.zeros_like(x[0])])
+
+
+def test_learned_lbfgs_with_gradient_net_descends():
+    from fwiflow import precond
+    f, A, xs, _ = _quadratic(n=36, seed=4)
+    f2 = lambda v: (lambda J, g, ex: (J, g, {"illumination": np.ones(v.size),  # noqa: E731
+                                             "gradients": (g.reshape(6, 6),
+                                                           np.zeros((6, 6)),
+                                                           np.zeros((6, 6)))}))(*f(v))
+    x0 = np.full((6, 6), 3000.0)
+    r = precond.learned_lbfgs(f2, x0, np.ones((6, 6)), net=_IdentityNet(), maxiter=60)
+    ref = inversion.lbfgs(f, x0.ravel(), maxiter=500, ftol=1e-14, gtol=1e-10)["misfit"]
+    assert r["misfit"] <= ref * (1 + 1e-9)  # the bounded minimum, as scipy's L-BFGS-B finds it
+    used = [h[2] for h in r["history"]]
+    assert used[:10] == ["lbfgs"] * 10  # (at the bounded minimum, later steps fall back)

@@ -111,16 +111,22 @@ def main():
         fwi, obj, x0 = setup(seed, a.nt, ws)
         J0 = obj(x0)[0]
         rp = inversion.lbfgs(obj, x0, maxiter=a.maxiter)
+        log(f"eval {seed}: plain {rp['misfit']:.4g} in {rp['seconds']:.1f} s, "
+            f"{rp['evaluations']} evaluations")
+
+        def progress(k, J, used, alpha, nev):
+            if k % 5 == 0:
+                log(f"  iteration {k}: misfit {J:.4g} ({used}, alpha {alpha:.3g}, {nev} evaluations)")
         if a.driver == "lbfgs":
             rl = precond.learned_lbfgs(obj, x0, fwi.mask, net=net, maxiter=2 * a.maxiter,
-                                       target_misfit=rp["misfit"])
+                                       target_misfit=rp["misfit"], on_iter=progress)
         else:
             rl = precond.learned_descent(obj, x0, net, fwi.mask, maxiter=2 * a.maxiter,
                                          target_misfit=rp["misfit"], step0=a.step0)
         log(f"eval {seed}: plain {rp['misfit']:.4g} in {rp['seconds']:.1f} s; "
             f"learned {rl['misfit']:.4g} in {rl['seconds']:.1f} s")
         rc = precond.learned_lbfgs(obj, x0, fwi.mask, net=None, maxiter=2 * a.maxiter,
-                                   target_misfit=rp["misfit"])
+                                   target_misfit=rp["misfit"], on_iter=progress)
         log(f"eval {seed}: control {rc['misfit']:.4g} in {rc['seconds']:.1f} s")
         obj.close()
         reached = rl["misfit"] <= rp["misfit"]

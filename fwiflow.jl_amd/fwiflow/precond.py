@@ -226,7 +226,7 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
 
 def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, memory=5,
                   target_misfit=None, first_step_rms=10.0, c_armijo=1e-4, max_backtracks=4,
-                  illum_eps=1e-3):
+                  illum_eps=1e-3, on_iter=None):
     """L-BFGS on cp with the network as the initial inverse Hessian.
 
     The two-loop recursion (Nocedal & Wright, Alg. 7.4) applies H0 to the
@@ -240,7 +240,11 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
     curvature condition (the reference's gradient is approximate, so a strong
     Wolfe search stalls); pairs with s'y <= 0 are skipped, and a direction
     that is not a descent direction falls back to H0 g alone, then to the
-    illumination-scaled gradient.  Returns the dict of ``learned_descent``.
+    illumination-scaled gradient.  When every candidate fails, the curvature
+    memory is cleared and the illumination-scaled gradient is tried once more
+    at a tenth of the step; only then does the run stop.  ``on_iter(k, J,
+    used, alpha, evaluations)`` is called after each accepted step.  Returns
+    the dict of ``learned_descent``.
     """
     shape = np.asarray(x0).shape
     m2 = np.asarray(mask)
@@ -303,9 +307,13 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
             candidates.append((-h0(g, ex, gamma), "h0"))
         il = np.asarray(ex.get("illumination", np.ones(shape)), dtype=np.float64).ravel()
         dp = -g / (il + illum_eps * max(il.max(), 1e-30)) * mflat
-        candidates.append((dp * (first_step_rms / rms(dp)), "fallback"))
+        dp *= first_step_rms / rms(dp)
+        candidates += [(dp, "fallback"), (0.1 * dp, "restart")]
         res = None
         for d, used in candidates:
+            if used == "restart":
+                S.clear()
+                Y.clear()
             slope = float(np.dot(g, d))
             if not slope < 0:
                 continue
@@ -314,7 +322,7 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
                 xn = np.clip(x + alpha * d, lo, hi)
                 Jn, gn, exn = evaluate(xn)
                 if Jn < J and Jn <= J + c_armijo * float(np.dot(g, xn - x)):
-                    res = (xn, Jn, gn, exn, used)
+                    res = (xn, Jn, gn, exn, used, alpha)
                     break
                 curv = Jn - J - slope * alpha
                 a_q = -slope * alpha * alpha / (2.0 * curv) if curv > 0 else 0.5 * alpha
@@ -323,7 +331,7 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
                 break
         if res is None:
             break
-        xn, Jn, gn, exn, used = res
+        xn, Jn, gn, exn, used, alpha = res
         s, y = xn - x, gn - g
         if float(np.dot(s, y)) > 1e-12 * float(np.dot(y, y)):
             S.append(s)
@@ -333,6 +341,8 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
                 Y.pop(0)
         x, J, g, ex = xn, Jn, gn, exn
         hist.append((time.perf_counter() - t0, J, used))
+        if on_iter is not None:
+            on_iter(len(hist), J, used, alpha, nev)
         if target_misfit is not None and J <= target_misfit:
             break
     return {"x": x.reshape(shape), "misfit": J, "iterations": len(hist), "evaluations": nev,

@@ -191,3 +191,28 @@ def test_paragen_surveygen_format(workspace):
     sv = json.loads(open(s).read())
     assert sv["nShots"] == 1 and sv["shot0"]["nrec"] == 2 and sv["shot0"]["x_rec"] == [3, 4]
     assert os.path.isdir(os.path.join(workspace, "D"))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says why,
+    before any GPU or library work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_bench_byte_model_matches_the_stated_per_unit_figure():
+    """766.3 GB per shot-gradient at the 500x1700 workload: 168 B per active
+    cell per step plus 96 B per C-PML cell-direction (bench.py's roofline
+    denominator), and the per-launch bytes scale with the shots of a launch."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    c = models.case3(nSteps=4000, nshots=2)
+    assert bench.algorithmic_bytes_per_shot_gradient(c, 4000) == pytest.approx(766.272e9, rel=1e-6)
+    one = bench.kernel_launch_bytes("bwd_pair_a", c, 1.0)
+    assert bench.kernel_launch_bytes("bwd_pair_a", c, 64 / 22) == pytest.approx(one * 64 / 22)

@@ -14,6 +14,9 @@ not a descent direction, so no optimiser can use it.
    reference's settings) on the training seeds log (gradient, illumination,
    accepted update) pairs in memory.
 2. The network (HIP kernels) trains on those pairs on the GPU.
+   With ``--teacher control`` the pairs come instead from the L-BFGS driver
+   with the illumination-scaled H0 (it does not stall on the approximate
+   gradient, so it logs more and later updates).
 3. On held-out seeds: the plain method runs to its end (maxiter); the learned
    driver then runs until it reaches the plain method's final misfit.
    value = median over held-out seeds of (learned wall time / plain wall time);
@@ -55,6 +58,8 @@ def parse():
     p.add_argument("--step0", type=float, default=20.0)
     p.add_argument("--seed0", type=int, default=100)
     p.add_argument("--driver", choices=["lbfgs", "descent"], default="lbfgs")
+    p.add_argument("--teacher", choices=["plain", "control"], default="plain",
+                   help="optimiser whose accepted updates are the training pairs")
     return p.parse_args()
 
 
@@ -91,7 +96,11 @@ def main():
             x = precond.features(extras["gradients"], extras["illumination"], mask)
             samples.append((x, precond.target_from_update(update, mask)))
 
-        r = inversion.lbfgs(obj, x0, maxiter=a.maxiter, on_pair=on_pair)
+        if a.teacher == "plain":
+            r = inversion.lbfgs(obj, x0, maxiter=a.maxiter, on_pair=on_pair)
+        else:
+            r = precond.learned_lbfgs(obj, x0, mask, net=None, maxiter=a.maxiter,
+                                      on_pair=on_pair)
         obj.close()
         plain_train.append((seed, r["iterations"], r["evaluations"], r["seconds"]))
         log(f"train model {seed}: {r['iterations']} iterations, {r['evaluations']} evaluations, "
@@ -155,7 +164,8 @@ def main():
         "control_value": (lambda v: v if np.isfinite(v) else None)(float(np.median(
             [e["control_ratio"] if e["control_ratio"] is not None else np.inf
              for e in evals]))) if evals else None,
-        "config": {"driver": a.driver, "grid": [int(H), int(W)], "nt": a.nt, "shots": 30,
+        "config": {"driver": a.driver, "teacher": a.teacher, "grid": [int(H), int(W)],
+                   "nt": a.nt, "shots": 30,
                    "train_models":
                    a.train_models, "eval_models": a.eval_models, "maxiter": a.maxiter,
                    "pairs": len(samples), "epochs": a.epochs, "lr": a.lr},

@@ -226,7 +226,7 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
 
 def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, memory=5,
                   target_misfit=None, first_step_rms=10.0, c_armijo=1e-4, max_backtracks=4,
-                  illum_eps=1e-3, on_iter=None):
+                  illum_eps=1e-3, on_iter=None, on_pair=None):
     """L-BFGS on cp with the network as the initial inverse Hessian.
 
     The two-loop recursion (Nocedal & Wright, Alg. 7.4) applies H0 to the
@@ -243,8 +243,10 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
     illumination-scaled gradient.  When every candidate fails, the curvature
     memory is cleared and the illumination-scaled gradient is tried once more
     at a tenth of the step; only then does the run stop.  ``on_iter(k, J,
-    used, alpha, evaluations)`` is called after each accepted step.  Returns
-    the dict of ``learned_descent``.
+    used, alpha, evaluations)`` is called after each accepted step and
+    ``on_pair(gradient, extras, update, misfit)`` with the training pair of
+    that step, as ``inversion.lbfgs`` logs them.  Returns the dict of
+    ``learned_descent``.
     """
     shape = np.asarray(x0).shape
     m2 = np.asarray(mask)
@@ -332,6 +334,8 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
         if res is None:
             break
         xn, Jn, gn, exn, used, alpha = res
+        if on_pair is not None:
+            on_pair(g, ex, xn - x, J)
         s, y = xn - x, gn - g
         if float(np.dot(s, y)) > 1e-12 * float(np.dot(y, y)):
             S.append(s)

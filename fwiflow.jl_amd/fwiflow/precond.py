@@ -226,7 +226,7 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
 
 def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, memory=5,
                   target_misfit=None, first_step_rms=10.0, c_armijo=1e-4, max_backtracks=4,
-                  quick_backtracks=2, illum_eps=1e-3, on_iter=None, on_pair=None):
+                  illum_eps=1e-3, on_iter=None, on_pair=None):
     """L-BFGS on cp with the network as the initial inverse Hessian.
 
     The two-loop recursion (Nocedal & Wright, Alg. 7.4) applies H0 to the
@@ -240,9 +240,7 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
     curvature condition (the reference's gradient is approximate, so a strong
     Wolfe search stalls); pairs with s'y <= 0 are skipped, and a direction
     that is not a descent direction falls back to H0 g alone, then to the
-    illumination-scaled gradient (the quasi-Newton candidates get
-    ``quick_backtracks`` trials each before the next candidate is tried, the
-    gradient candidates ``max_backtracks``).  When every candidate fails, the curvature
+    illumination-scaled gradient.  When every candidate fails, the curvature
     memory is cleared and the illumination-scaled gradient is tried once more
     at a tenth of the step; only then does the run stop.  ``on_iter(k, J,
     used, alpha, evaluations)`` is called after each accepted step and
@@ -322,8 +320,7 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
             if not slope < 0:
                 continue
             alpha = 1.0
-            tries = quick_backtracks if used in ("lbfgs", "h0") else max_backtracks
-            for _ in range(tries + 1):
+            for _ in range(max_backtracks + 1):
                 xn = np.clip(x + alpha * d, lo, hi)
                 Jn, gn, exn = evaluate(xn)
                 if Jn < J and Jn <= J + c_armijo * float(np.dot(g, xn - x)):

@@ -337,19 +337,23 @@ def test_plain_inversion_reduces_misfit(workspace):
                                rtol=1e-6, atol=1e-6 * np.abs(g0).max())
 
 
-@pytest.mark.parametrize("pairs", ["1", "0"])
-def test_interior_checkpoints_match_oracle(oracle_mod, workspace, pairs):
+@pytest.mark.parametrize("env", [{"FWIFLOW_BWD_PAIRS": "1"}, {"FWIFLOW_BWD_PAIRS": "0"},
+                                 {"FWIFLOW_SHOT_GROUP": "2", "FWIFLOW_SHOT_GROUP_BWD": "1"}],
+                         ids=["paired", "unpaired", "groups_2_fwd_1_bwd"])
+def test_interior_checkpoints_match_oracle(oracle_mod, workspace, env):
     """Reverse sweep reset to stored interior states every k steps (k = 7, and
     a ragged last interval): gradients against the oracle's scheme 2 in FP32
-    and FP64, with the backward kernels paired and unpaired."""
+    and FP64, with the backward kernels paired and unpaired, and with the
+    checkpoints written by forward groups of 2 and read by backward groups of
+    1."""
     c = models.case_small(nSteps=300, nshots=3)
     para, stf = models.write_case(c, workspace)
     lam, mu, den = models.moduli(c, "true")
     ids = [0, 1, 2]
     fwiflow.fwi_obs_op(lam, mu, den, stf, 0, ids, para)
     lam0, mu0, den0 = models.moduli(c, "start")
-    old = os.environ.get("FWIFLOW_BWD_PAIRS")
-    os.environ["FWIFLOW_BWD_PAIRS"] = pairs
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         with fwiflow.Plan(para, ids) as P:
             P.set_model(lam0, mu0, den0)
@@ -359,17 +363,18 @@ def test_interior_checkpoints_match_oracle(oracle_mod, workspace, pairs):
             P.run(1)
             g = P.gradients()
     finally:
-        if old is None:
-            os.environ.pop("FWIFLOW_BWD_PAIRS", None)
-        else:
-            os.environ["FWIFLOW_BWD_PAIRS"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     o32 = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=32, scheme=2,
                           ckpt_every=7)
     o64 = oracle_mod.cufd(lam0, mu0, den0, stf, 1, ids, para, precision=64, scheme=2,
                           ckpt_every=7)
     for name, a, b32, b64 in zip(("lam", "mu", "den"), g[:3], o32[1:4], o64[1:4]):
         e32, e64 = rel(a, b32), rel(a, b64)
-        say("checkpoint k=7 pairs", pairs, name, "rel32", e32, "rel64", e64)
+        say("checkpoint k=7", env, name, "rel32", e32, "rel64", e64)
         assert e32 < GRAD_TOL and e64 < GRAD_TOL
 
 

@@ -1906,6 +1906,7 @@ struct DeviceState {
   int rev_group = 8; // shots per workgroup of the fused reverse step
   bool fused_rev = true;  // fused reverse step (needs its LDS to fit)
   int unroll = 1;         // shots whose loads a split-kernel thread issues together
+  int unroll_fwd = 2;     // ... in the forward kernels
   int shot_group = -1;    // shots per launch of the split/tiled kernels (0: all, -1: auto)
   int shot_group_bwd = -1;  // the same for the backward sweep (0: all, -1: auto)
   // Two streams (default priority both): shot groups alternate between them,
@@ -2141,6 +2142,10 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   {
     const int u = env_int("FWIFLOW_SHOT_UNROLL", 1);
     d.unroll = u >= 4 ? 4 : (u >= 2 ? 2 : 1);
+    // the forward kernels keep occupancy 4-5 with two shots' loads in flight
+    // and gained (6.07-6.09 vs 6.05-6.06 shot-gradients/s)
+    const int uf = env_int("FWIFLOW_SHOT_UNROLL_FWD", 2);
+    d.unroll_fwd = uf >= 4 ? 4 : (uf >= 2 ? 2 : 1);
   }
   d.shot_group = std::max(-1, env_int("FWIFLOW_SHOT_GROUP", -1));
   d.shot_group_bwd = std::max(-1, env_int("FWIFLOW_SHOT_GROUP_BWD", -1));
@@ -2649,6 +2654,7 @@ Status Plan::run(int calc_id) {
     float* ma0 = ma;
     // ceil(ns / G) groups of balanced sizes (64 shots by 3: 20 x 3 + 2 x 2)
     const int NGf = (g.ns + G - 1) / G;
+    const int U = d.unroll_fwd;  // LAUNCH_U below
     if (two) {
       HIP_TRY(hipEventRecord(d.join[0], st));
       HIP_TRY(hipStreamWaitEvent(d.stream2, d.join[0], 0));

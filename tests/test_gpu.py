@@ -443,7 +443,7 @@ def test_fused_kernels_match_split_kernels(workspace, variant):
                             "FWIFLOW_SHOT_UNROLL": "1", "FWIFLOW_SHOTS_PER_THREAD": "0",
                             "FWIFLOW_BWD_PAIRS": "1", "FWIFLOW_XCD_BALANCE": "1",
                             "FWIFLOW_PAIR_MIX": "1", "FWIFLOW_PAIR_MIX_B": "0",
-                            "FWIFLOW_FOLD_INJECT": "1"})
+                            "FWIFLOW_FOLD_INJECT": "1", "FWIFLOW_SHOT_UNROLL_FWD": "1"})
     d1, g1, J1 = _run_plan(para, ids, lam0, mu0, den0, stf, variant)
     if variant.get("FWIFLOW_FOLD_INJECT") == "0":
         assert J1 == J0 and all(np.array_equal(a, b) for a, b in zip(g1, g0))

@@ -52,8 +52,8 @@ KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_str
 # (64 shots: 13 groups of 4.92, 22 of 2.91 on average):
 # profiles/stencil_kernels_608x1780_5fwd_3bwd_groups_counters.md.  Reported
 # as roofline.traffic when the dominant kernel and group size match.
-MEASURED_TRAFFIC = {("bwd_pair_a", 3): 234.2e6, ("bwd_pair_b", 3): 228.8e6,
-                    ("stress_fwd", 5): 205.0e6, ("vel_fwd", 5): 181.2e6}
+MEASURED_TRAFFIC = {("bwd_pair_a", 3): 234.1e6, ("bwd_pair_b", 3): 228.8e6,
+                    ("stress_fwd", 5): 204.3e6, ("vel_fwd", 5): 180.7e6}
 # kernels of the backward sweep (their launches cover the backward shot groups)
 BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "rev_step", "adj_step",
             "bwd_pair_a", "bwd_pair_b")

@@ -98,8 +98,8 @@ def test_config1_parity(oracle_mod, workspace):
     _assert(_parity(oracle_mod, models.case1(), workspace))
 
 
-@pytest.mark.parametrize("grid,nt", [((200, 600), 1000), ((300, 900), 300)],
-                         ids=["200x600", "300x900"])
+@pytest.mark.parametrize("grid,nt", [((200, 600), 1000), ((300, 900), 300), ((450, 1350), 800)],
+                         ids=["200x600", "300x900", "450x1350"])
 def test_config4_time_lapse_parity(oracle_mod, workspace, grid, nt):
     """Time-lapse surveys on the 12 m grids (32-cell C-PML, dt 1.5 ms, 10 Hz):
     baseline and monitor (-6% Vp / +2% density ellipse) each against the

@@ -11,6 +11,9 @@
 //   deps/CustomOps/FWI/Src/el_velocity_adj.cu:22-108, el_stress_adj.cu:22-104 (adjoint)
 //   deps/CustomOps/FWI/Src/utilities.cu:58-747 (init, averages, misfit, Courant,
 //                                               cpmlInit, ring store, source/receiver, window)
+//   deps/CustomOps/FWI/Src/utilities.cu:596-1112, libCUFD.cu:254-309 (optional data
+//                                               conditioning: per-trace windows, band-pass,
+//                                               source-signature update)
 //   deps/CustomOps/FWI/Src/Model.cu:15-93, Cpml.cu:7-117, Src_Rec.cu:19-245, Boundary.cu:5-102
 //
 // Two precisions:

@@ -11,7 +11,7 @@ from .utils import (moduli_to_velocity_grad, npad, padding, padding_adjoint, par
 from .fwi import (FWI, FWIExample, compute_misfit, compute_misfit_and_gradient,
                   compute_observation, make_fwi)
 from .inversion import PlanObjective, lbfgs
-from .precond import CNN, learned_descent
+from .precond import CNN, learned_descent, learned_lbfgs
 from ._lib import FwiflowError, version
 
 __all__ = [
@@ -19,5 +19,5 @@ __all__ = [
     "moduli_to_velocity_grad", "npad", "padding", "padding_adjoint", "paraGen", "sourceGene",
     "surveyGen", "velocity_to_moduli", "FWI", "FWIExample", "compute_misfit",
     "compute_misfit_and_gradient", "compute_observation", "make_fwi", "FwiflowError",
-    "version", "PlanObjective", "lbfgs", "CNN", "learned_descent",
+    "version", "PlanObjective", "lbfgs", "CNN", "learned_descent", "learned_lbfgs",
 ]

@@ -33,10 +33,6 @@ HBM_PEAK_GBS = 8000.0
 # shots of a launch and amortised to zero; C-PML memory variables add 8 B per
 # read-modify-write where the kernel updates them.
 KERNEL_BYTES = {
-    "fwd_tile": 40.0,     # tiled fused forward: read and write the five components
-    "fwd_step": 40.0,     # fused: read vz vx szz sxx sxz, write all five (ping-pong)
-    "rev_step": 60.0,     # fused reverse: read/write 5 forward components, read 5 adjoint
-    "adj_step": 40.0,     # fused adjoint: read and write the five adjoint components
     "stress_fwd": 32.0,   # read vz vx szz sxx sxz, write szz sxx sxz
     "vel_fwd": 28.0,      # read szz sxx sxz vz vx, write vz vx
     "rev_vel": 36.0,      # read 3 stress, 2 velocity, 2 adjoint velocity, write 2 velocity
@@ -48,17 +44,14 @@ KERNEL_BYTES = {
 KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_stress", "rev_vel")}
 # Measured HBM-side bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, each
 # scaled by a 1 GiB copy's counters, averaged over the launches) at this
-# workload with forward groups of 5 and backward groups of 3 nominal shots
-# (64 shots: 13 groups of 4.92, 22 of 2.91 on average):
-# profiles/stencil_kernels_608x1780_5fwd_3bwd_groups_counters.md.  Reported
-# as roofline.traffic when the dominant kernel and group size match.
-MEASURED_TRAFFIC = {("bwd_pair_a", 3): 234.1e6, ("bwd_pair_b", 3): 228.8e6,
-                    ("stress_fwd", 5): 204.3e6, ("vel_fwd", 5): 180.7e6}
+# workload's shot groups: profiles/stencil_kernels_608x1780_counters.md.
+# Reported as roofline.traffic when the dominant kernel and its mean shots
+# per launch match the profiled run.
+MEASURED_TRAFFIC = {}
 # kernels of the backward sweep (their launches cover the backward shot groups)
-BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "rev_step", "adj_step",
-            "bwd_pair_a", "bwd_pair_b")
+BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "bwd_pair_a", "bwd_pair_b")
 # C-PML memory-variable traffic per PML cell-direction (2 variables RW = 16 B)
-KERNEL_PML = {"fwd_tile": 32.0, "fwd_step": 32.0, "adj_step": 32.0, "stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
+KERNEL_PML = {"stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
 
 
 def parse():
@@ -70,6 +63,10 @@ def parse():
     p.add_argument("--nshots", type=int, default=64)
     p.add_argument("--cpu-sample-steps", type=int, default=1500)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--checkpoint", type=int, default=0,
+                   help="checkpoint interval of the backward sweep (0: automatic)")
+    p.add_argument("--memory-budget", type=float, default=None,
+                   help="device bytes for the boundary ring and checkpoints (default 32e9)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device-to-device copy, 4 times (the "
                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -129,12 +126,34 @@ def kernel_launch_bytes(name, c, ns):
     return b
 
 
+def host_cores():
+    """Threads the CPU baseline uses: the cores this process may run on
+    (what `nproc` reports), capped by OMP_NUM_THREADS when the host sets it
+    (a shared GPU host gives each GPU's jobs that share of its cores)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n or 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(c, ws, sample_steps):
     from fwiflow import models
     from oracle import oracle
 
     oracle.build()
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cores()
     cs = models.case3(nSteps=sample_steps, nshots=c.src_x.size)
     para, stf = models.write_case(cs, os.path.join(ws, "cpu"))
     lam, mu, den = models.moduli(cs, "true")
@@ -145,9 +164,10 @@ def cpu_baseline(c, ws, sample_steps):
     t = time.perf_counter() - t0
     per_grad = t * (c.nSteps / sample_steps)
     return dict(value=1.0 / per_grad, unit="shot-gradients/s", cores=cores, kind="port",
+                cpu=cpu_model(), nproc=os.cpu_count(),
                 sample=f"1 shot of the same grid, first {sample_steps} of {c.nSteps} steps "
-                       f"(calc_id 1, FP32 reference arithmetic, OpenMP), {t:.1f} s, "
-                       f"scaled linearly to {c.nSteps} steps")
+                       f"(calc_id 1, FP32 reference arithmetic, OpenMP on {cores} threads), "
+                       f"{t:.1f} s, scaled linearly to {c.nSteps} steps")
 
 
 def main():
@@ -169,6 +189,10 @@ def main():
     ids = list(range(a.nshots))
     log(f"bench: grid {c.NZ}x{c.NX} nt {c.nSteps} shots {a.nshots}")
     P = fwiflow.Plan(para, ids)
+    if a.checkpoint:
+        P.set_checkpoint(a.checkpoint)
+    if a.memory_budget:
+        P.set_memory_budget(a.memory_budget)
     lam, mu, den = models.moduli(c, "true")
     P.set_model(lam, mu, den)
     P.set_stf(stf)
@@ -197,13 +221,11 @@ def main():
         ms, n = P.timing(name)
         if n:
             kt[name] = (ms / n, n)
-    # every kernel runs once per time step per shot group
-    spl_f, spl_b = P.shots_per_launch()
+    # every kernel runs once per time step per shot group: a launch covers
+    # nshots / groups shots on average
+    groups_f, groups_b = P.launch_groups()
     dominant = max(kt, key=lambda k: kt[k][0])
-    spl = spl_b if dominant in BACKWARD else spl_f
-    # the engine splits the shots into ceil(ns / spl) groups of balanced
-    # sizes: a launch covers ns / groups shots on average
-    mean_spl = a.nshots / -(-a.nshots // spl)
+    mean_spl = a.nshots / (groups_b if dominant in BACKWARD else groups_f)
     avg_ms = kt[dominant][0]
     lb = kernel_launch_bytes(dominant, c, mean_spl)
     achieved = lb / (avg_ms * 1e-3) / 1e9
@@ -239,8 +261,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (MEASURED_TRAFFIC.get((dominant, spl))
-                        if c.NZ == 608 and c.NX == 1780 else None),
+            "traffic": (MEASURED_TRAFFIC.get((dominant, round(mean_spl, 2)))
+                        if c.NZ == 608 and c.NX == 1780 and c.nSteps == 4000 else None),
             "avg_launch_ms": round(avg_ms, 4),
             "bytes_per_launch": lb,
             "shots_per_launch": round(mean_spl, 3),
@@ -252,7 +274,10 @@ def main():
             "unit": "GB/s",
         },
         "kernel_avg_ms": {k: round(v[0], 4) for k, v in kt.items()},
+        "launch_groups": {"forward": groups_f, "backward": groups_b},
         "device_bytes": P.device_bytes(),
+        "memory_plan": dict(zip(("checkpoint_every", "shots_per_sweep", "ring_bytes",
+                                 "checkpoint_bytes"), P.memory_plan())),
     }
     P.close()
     if not a.no_cpu_baseline:

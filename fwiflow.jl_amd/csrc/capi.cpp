@@ -21,6 +21,10 @@ struct fwiflow_cnn {
   fwiflow::Cnn impl;
 };
 
+struct fwiflow_job {
+  fwiflow::Job impl;
+};
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -217,6 +221,37 @@ int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* forward, int* backwar
   if (forward) *forward = plan->impl.shots_per_launch();
   if (backward) *backward = plan->impl.shots_per_launch_bwd();
   return report(Status::Ok());
+}
+int fwiflow_plan_launch_groups(fwiflow_plan* plan, int* forward, int* backward) {
+  if (!plan) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
+  if (forward) *forward = plan->impl.launch_groups(0);
+  if (backward) *backward = plan->impl.launch_groups(1);
+  return report(Status::Ok());
+}
+int fwiflow_plan_set_memory_budget(fwiflow_plan* plan, double bytes) {
+  PLAN_CALL(plan->impl.set_memory_budget(bytes));
+}
+int fwiflow_plan_memory_plan(fwiflow_plan* plan, int* ckpt_every, int* shots_per_sweep,
+                             long long* ring_bytes, long long* checkpoint_bytes) {
+  PLAN_CALL(plan->impl.memory_plan(ckpt_every, shots_per_sweep, ring_bytes, checkpoint_bytes));
+}
+
+int fwiflow_job_create(fwiflow_job** job) {
+  if (!job) return report(Status::Err(FWIFLOW_ERR_ARG, "null output handle"));
+  *job = new fwiflow_job();
+  return report(Status::Ok());
+}
+int fwiflow_job_destroy(fwiflow_job* job) {
+  delete job;
+  return report(Status::Ok());
+}
+int fwiflow_job_add(fwiflow_job* job, fwiflow_plan* plan) {
+  if (!job || !plan) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
+  return report(job->impl.add(&plan->impl));
+}
+int fwiflow_job_run(fwiflow_job* job, int calc_id) {
+  if (!job) return report(Status::Err(FWIFLOW_ERR_ARG, "null job"));
+  return report(job->impl.run(calc_id));
 }
 
 #define CNN_CALL(expr)                                                      \

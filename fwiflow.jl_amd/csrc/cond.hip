@@ -288,12 +288,6 @@ __global__ __launch_bounds__(256) void k_shot_residual(int nt, int ntr, const fl
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ void k_source_increment(int nt, const float* __restrict__ src, float dt,
-                                   float* __restrict__ inj) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nt) inj[t] = (float)(2250000.0 * (double)src[t] * (double)dt);
-}
-
 int grid_for(size_t total) {
   const size_t b = (total + 255) / 256;
   return (int)(b < 4096 ? (b ? b : 1) : 4096);
@@ -407,11 +401,6 @@ hipError_t shot_residual(hipStream_t st, int nt, int ntr, const float* obs, cons
                          int in_stride, float* res, int res_stride, double* partial, int blocks) {
   hipLaunchKernelGGL(k_shot_residual, dim3(blocks), dim3(256), 0, st, nt, ntr, obs, syn, in_stride,
                      res, res_stride, partial);
-  return hipGetLastError();
-}
-
-hipError_t source_increment(hipStream_t st, int nt, const float* src, float dt, float* inj) {
-  hipLaunchKernelGGL(k_source_increment, dim3((nt + 255) / 256), dim3(256), 0, st, nt, src, dt, inj);
   return hipGetLastError();
 }
 

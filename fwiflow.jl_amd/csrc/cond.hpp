@@ -75,8 +75,4 @@ hipError_t window_traces(hipStream_t st, int nt, TraceSet ts, const float* w2,
 hipError_t shot_residual(hipStream_t st, int nt, int ntr, const float* obs, const float* syn,
                          int in_stride, float* res, int res_stride, double* partial, int blocks);
 
-// Injection increment 1500^2 * src * dt of the reverse re-materialisation
-// (add_source, utilities.cu:521-555).
-hipError_t source_increment(hipStream_t st, int nt, const float* src, float dt, float* inj);
-
 }  // namespace fwiflow

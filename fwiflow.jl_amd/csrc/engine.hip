@@ -1,34 +1,55 @@
-// gfx950 engine of the elastic FWI operator.
+// gfx950 engine of the elastic FWI operator: the time-step kernels, the plan
+// (device-resident model, data and wavefields of one parameter file) and the
+// job runner that drives one plan, or several plans on different grids,
+// through one set of launches per time step.
 //
 // Device data layout (all FP32, z fastest so a wave64 row is 64 consecutive
 // depth cells, one 256-byte coalesced access):
 //   model      lam, mu, ave_mu, byc_a, byc_b          [NX][NZ]
-//   wavefield  vz, vx, szz, sxx, sxz  per shot        [shot][NX][NZ]
-//   adjoint    same 5 components                      [shot][NX][NZ]
-//   C-PML memory variables (8), per shot              [shot][NX][NZ]
-//   boundary ring, 5 layers x 4 sides, 5 components   [shot][step][component][ring]
+//   wavefield  vz, vx, szz, sxx, sxz                  [component][shot][NX][NZ]
+//   adjoint    same 5 components                      [component][shot][NX][NZ]
+//   C-PML memory variables (8)                        [variable][shot][NX][NZ]
+//   boundary ring, 5 layers x 4 sides, 5 components   [shot][step][component][ring slot]
 //   seismograms syn/obs/res, time-major               [nSteps][all receivers]
 //   gradient accumulators (5, summed over shots)      [NX][NZ]
-// One thread owns one cell and loops over every shot of the group: the model
-// parameters and C-PML coefficients are read once per cell per step and the
-// gradient contributions of all shots are summed in registers before one
-// read-modify-write per step.
 //
-// Reference kernels restated here: el_stress.cu:24-131, el_velocity.cu:21-117,
+// Arithmetic.  Every wavefield update evaluates the reference's expression in
+// the reference's order and types, so that the fields follow its FP32
+// arithmetic rather than a re-association of it:
+//   * a stencil difference is DIVIDED by dz or dx, correctly rounded:
+//     q = x*y + (x - d*(x*y))*y with y = RN(1/d), one Newton correction on a
+//     correctly rounded reciprocal, which equals IEEE x/d for every normal x
+//     when d passes the exhaustive check at plan creation (check_spacing);
+//     C-PML K divisions take two corrections;
+//   * the reference's double literals promote exactly where its source says
+//     (2.0*mu, RSXXZZ*..., MEGA), so those terms are evaluated in FP64 and
+//     rounded once to FP32 on assignment, as C++ does;
+//   * no multiply-add contraction anywhere in this file.
+// Gradient accumulators are the one place that sums in another order: the
+// contributions of a launch's shots are added in registers, and the chain
+// rule to (lambda, mu, rho) is applied once at the end.
+//
+// Reference kernels restated here: el_stress.cu:50-131, el_velocity.cu:45-117,
 // el_velocity_adj.cu:22-108, el_stress_adj.cu:22-104, utilities.cu:125-205,
-// 361-424, 521-593, 653-747; driver sequence libCUFD.cu:157-463.
+// 361-424, 506-593, 653-747; driver sequence libCUFD.cu:157-463.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "cond.hpp"
 #include "engine.hpp"
+
+#pragma clang fp contract(off)
 
 namespace fwiflow {
 
@@ -45,19 +66,75 @@ namespace fwiflow {
     if (!s_.ok()) return s_; \
   } while (0)
 
+// 16-byte raw buffer load (the LLVM AMDGPU intrinsic, bound by name)
+typedef float f4 __attribute__((ext_vector_type(4)));
+__device__ f4 raw_load4(__amdgpu_buffer_rsrc_t r, int vo, int so, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+
 namespace {
 
-constexpr int kBZ = 64;  // threads along z (one wave64 row)
-constexpr int kCondBlocks = 512;  // misfit partial sums per shot of the conditioned residual
-constexpr float kC1 = 9.0f / 8.0f;
-constexpr float kC2 = 1.0f / 24.0f;
+constexpr int kBZ = 64;            // threads along z (one wave64 row)
+constexpr int kCondBlocks = 512;   // misfit partial sums per shot of the conditioned residual
+constexpr int kPartials = 1024;    // misfit partial sums of one residual launch
+// el_stress.cu:42-43: float c1 = 9.0/8.0, c2 = 1.0/24.0
+constexpr float kC1 = (float)(9.0 / 8.0);
+constexpr float kC2 = (float)(1.0 / 24.0);
+// add_source (utilities.cu:528): float scale = pow(1500.0, 2)
+constexpr float kSrcScale = 2250000.0f;
+// RSXXZZ (utilities.h:21) is the double literal 3.0
+constexpr double kRS = 3.0;
 
+// ------------------------------------------------------------ arithmetic
+// x / d for d with the correctly rounded reciprocal y: one Newton correction
+// (exact for every normal x when check_spacing(d) holds).
+__device__ __forceinline__ float div1(float x, float d, float y) {
+  const float q = x * y;
+  return __builtin_fmaf(__builtin_fmaf(-d, q, x), y, q);
+}
+// two corrections: correctly rounded for any normal divisor (the first makes
+// q faithful, the second rounds it)
+__device__ __forceinline__ float div2(float x, float d, float y) {
+  float q = x * y;
+  q = __builtin_fmaf(__builtin_fmaf(-d, q, x), y, q);
+  return __builtin_fmaf(__builtin_fmaf(-d, q, x), y, q);
+}
+__device__ __forceinline__ double ddiv2(double x, double d, double y) {
+  double q = x * y;
+  q = __builtin_fma(__builtin_fma(-d, q, x), y, q);
+  return __builtin_fma(__builtin_fma(-d, q, x), y, q);
+}
+
+// A C-PML stretching coefficient K of one cell and its reciprocals; x / K is
+// x itself where K == 1 (outside the C-PML).
+struct Kdiv {
+  float K, r;
+  double r64;
+  bool one;
+};
+__device__ __forceinline__ Kdiv kdiv_of(float K) {
+  Kdiv k;
+  k.K = K;
+  k.one = K == 1.f;
+  k.r = k.one ? 1.f : 1.f / K;
+  k.r64 = k.one ? 1.0 : 1.0 / (double)K;
+  return k;
+}
+__device__ __forceinline__ float kdiv(float x, const Kdiv& k) { return k.one ? x : div2(x, k.K, k.r); }
+__device__ __forceinline__ double kdiv(double x, const Kdiv& k) {
+  return k.one ? x : ddiv2(x, (double)k.K, k.r64);
+}
+
+// ------------------------------------------------------------ descriptors
 struct Geo {
-  int nz, nx, nzA, nPml, ns, nSteps;
-  long long N;  // cells per shot
-  float dt, rdz, rdx;
+  int nz, nx, nzA, nPml, nSteps;
+  int ns;          // shots of this part in the launch
+  int s0;          // plan index of its first shot
+  long long N;     // cells per shot
+  float dt, dz, dx, rdz, rdx;
   int ringW, ringH, ringlen;
-  int s0;  // plan index of the launch's first shot
+  int R;           // receivers of the plan: row length of the time-major seismograms
+  int rs0;         // ring: index of the part's first shot in the ring buffer
+  int rnt;         // ring: steps held
 };
 
 struct Mdl {
@@ -86,9 +163,73 @@ struct Mem {
 };
 
 struct Src {
-  const int* zx;     // [ns][2] grid indices of the source
-  const float* inj;  // [ns][nSteps] stress increment 1500^2 * stf_w(t) * dt
+  const int* zx;     // [shot][2] grid indices of the source
+  const float* amp;  // [shot][nSteps] windowed source rows (Src_Rec.cu:132-143)
 };
+
+// Residual injection folded into the adjoint stress update (one-row surveys
+// with one receiver per cell): at[plan shot][x] is the receiver of column x
+// in row z, or -1.
+struct Inj {
+  const float* res;  // time-major residual, row t at res + t * R; null: no injection
+  const int* at;
+  int z;
+};
+
+// One plan's shots in one launch.  Every per-shot pointer already points at
+// the part's first shot.
+struct Part {
+  Geo g;
+  Mdl m;
+  Pml c;
+  Wave w;           // forward state
+  Wave a;           // adjoint state
+  Mem p;            // C-PML memory variables (forward ones in forward launches,
+                    // adjoint ones in backward launches)
+  Src src;          // source rows: injected (forward) or removed (reversal)
+  Inj inj;
+  float* ring;      // boundary ring buffer (null: not saved)
+  float* acc;       // 5 gradient accumulators
+  float* stf_grad;  // [shot][nSteps]
+  float* illum;     // forward illumination (null: off)
+  const int* rec;   // [receiver][3] = (z, x, plan shot), from the part's first receiver
+  float* syn;       // seismograms + the part's first receiver (row t at + t * R)
+  int nrec;         // receivers of the part's shots
+  int gzU;          // tiles along z of the update-region tiling
+  int gzR;          // tiles along z of the ring-rectangle tiling
+};
+
+// Kernel argument: one part by value, or a table of parts in device memory.
+template <int NP>
+struct Parts;
+template <>
+struct Parts<1> {
+  Part p;
+  __device__ __forceinline__ const Part& get(int) const { return p; }
+};
+template <>
+struct Parts<0> {
+  const Part* p;
+  __device__ __forceinline__ const Part& get(int i) const { return p[i]; }
+};
+
+// Workgroup map entries: kind (3 bits), part (5 bits), tile or receiver
+// block (24 bits).  A launch's grid is 8 * L workgroups; workgroup b runs on
+// XCD b % 8, so the host lays out each XCD's list of work at b = 8 j + xcd.
+enum Kind : unsigned {
+  kNop = 0,
+  kStressFwd = 1,
+  kVelFwd = 2,
+  kRecord = 3,
+  kRevStress = 4,
+  kAdjVel = 5,
+  kAdjStress = 6,
+  kRevVel = 7,
+};
+constexpr int kMaxPartsTable = 31;
+__host__ __device__ inline unsigned wg_entry(unsigned kind, unsigned part, unsigned tile) {
+  return kind << 29 | part << 24 | tile;
+}
 
 // Boundary ring (Boundary.cu:17-41, utilities.cu:361-424): rows and columns
 // nPml-2 .. nPml+2 on each side of the interior, i.e. the band of width 5 of
@@ -107,63 +248,67 @@ __host__ __device__ inline int ring_slot(int z, int x, const Geo& g) {
   return -1;
 }
 
-// Ring layout [plan shot][step][component][slot]; `s` counts from the
-// launch's first shot.  (A step-major order, one step's slices of a shot
-// group contiguous, measured no faster: 5.73 vs 5.81 shot-gradients/s.)
+// Ring layout [ring shot][held step][component][slot]; `s` counts from the
+// part's first shot.  The ring holds rnt consecutive steps: step it is row
+// it % rnt (all steps of the sweep when rnt = nSteps - 1, one checkpoint
+// segment otherwise).
 __device__ inline float* ring_ptr(float* ring, const Geo& g, int it, int comp, int s, int slot) {
-  const size_t row = ((size_t)g.s0 + s) * g.nSteps + it;
+  const size_t row = ((size_t)g.rs0 + s) * g.rnt + (unsigned)it % (unsigned)g.rnt;
   return ring + (row * 5 + comp) * (size_t)g.ringlen + slot;
 }
 
-// 1-D launch over a 2-D tile grid.  Workgroups are dealt to the 8 XCDs
-// round-robin by linear id; with `xcd` set, the ids are permuted so that each
-// XCD walks one contiguous band of tiles (z fastest, then x), and tiles that
-// share x-halo columns share that XCD's L2.
-// A launch may also split its shots into `chunks` ranges of `spt` shots, one
-// range per copy of the tile grid (workgroup id = chunk * n + tile before
-// the XCD permutation); tile_of returns the chunk.
-// With `perm` set (one chunk only), the tile of workgroup b is perm[b]: a
-// host-built order that also balances the heavy C-PML columns over the XCDs.
-struct Tiling {
-  int gz, gx, n, xcd;
-  int chunks = 1, spt = 1 << 30;
-  const int* perm = nullptr;
+// Stencil loads and stores go through buffer resources: one scalar resource
+// per field of a part, the shot's byte offset in an SGPR (soffset) and the
+// cell's per-lane byte offset in a VGPR, with the z neighbours at immediate
+// offsets.  A wave's addresses then cost no per-lane 64-bit arithmetic.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc rsrc(const float* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)0xffffffff,
+                                           0x00020000);
+}
+// Per-lane byte offsets of a cell's stencil: z-2 (the z neighbours follow
+// 4 B apart) and the x neighbours x-2, x-1, x+1, x+2.
+struct Cell {
+  int zm2, xm2, xm1, xp1, xp2;
 };
+__device__ __forceinline__ Cell cell_of(unsigned bo, int nz) {
+  const int b = (int)bo, r = 4 * nz;
+  return Cell{b - 8, b - 2 * r, b - r, b + r, b + 2 * r};
+}
+__device__ __forceinline__ float ldb(Rsrc r, int so, int vo) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+// z neighbour k (-2..2) and x neighbour k (-2, -1, 1, 2) of the cell, shot offset so
+__device__ __forceinline__ float lz(Rsrc r, int so, const Cell& c, int k) {
+  return ldb(r, so, c.zm2 + 4 * (k + 2));
+}
+__device__ __forceinline__ float lx(Rsrc r, int so, const Cell& c, int k) {
+  return ldb(r, so, k == -2 ? c.xm2 : k == -1 ? c.xm1 : k == 1 ? c.xp1 : c.xp2);
+}
+// z neighbours k0 .. k0+3 of the cell in one 16-byte load
+__device__ __forceinline__ f4 lz4(Rsrc r, int so, const Cell& c, int k0) {
+  return raw_load4(r, c.zm2 + 4 * (k0 + 2), so, 0);
+}
+__device__ __forceinline__ void st(Rsrc r, int so, const Cell& c, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, c.zm2 + 8, so, 0);
+}
 
-__device__ inline int tile_of(const Tiling& t, int b, int& bz, int& bx) {
-  if (t.perm && t.chunks == 1) {
-    b = t.perm[b];
-  } else if (t.xcd) {
-    const int per = t.n * t.chunks / 8;
-    if (b < per * 8) b = (b % 8) * per + b / 8;
-  }
-  const int c = b / t.n;
-  b -= c * t.n;
-  bz = b % t.gz;
-  bx = b / t.gz;
-  return c;
+// Staggered 4th-order differences as the reference writes them:
+// forward / reverse (c1 (a - b) - c2 (c - d)) / h, adjoint (-c1 (a - b) + c2 (c - d)) / h.
+__device__ __forceinline__ float dfw(float a, float b, float c, float d, float h, float rh) {
+  return div1(kC1 * (a - b) - kC2 * (c - d), h, rh);
 }
-__device__ inline int tile_of(const Tiling& t, int& bz, int& bx) {
-  return tile_of(t, (int)blockIdx.x, bz, bx);
-}
-
-// Loads and stores at a byte offset from a wave-uniform base: the per-shot,
-// per-neighbour base is uniform across the wave and the per-lane part of
-// every address is one 32-bit offset shared by all fields and shots.
-__device__ __forceinline__ float ldo(const float* base, unsigned bo) {
-  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + bo);
-}
-__device__ __forceinline__ void sto(float* base, unsigned bo, float v) {
-  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + bo) = v;
+__device__ __forceinline__ float dad(float a, float b, float c, float d, float h, float rh) {
+  return div1(-kC1 * (a - b) + kC2 * (c - d), h, rh);
 }
 
 // ------------------------------------------------------------------ forward
-// The split kernels below walk all shots of the launch inside each thread, so
-// model parameters and C-PML coefficients are read once per cell.  Shots are
-// taken U at a time: every load of the U shots is issued before the first
-// store, so a wave keeps U shots' worth of reads in flight instead of one
-// (the stores of shot s may alias the loads of shot s+1 as far as the
-// compiler knows, so it cannot reorder them itself).
+// The kernels walk all shots of a part inside each thread, so model
+// parameters and C-PML coefficients are read once per cell.  Shots are taken
+// U at a time: every load of the U shots is issued before the first store,
+// so a wave keeps U shots' worth of reads in flight (the stores of shot s may
+// alias the loads of shot s+1 as far as the compiler knows, so it cannot
+// reorder them itself).
 
 // el_stress.cu:50-88 + add_source (utilities.cu:521-555) + from_bnd
 // (utilities.cu:361-391): the ring cells save sigma^it and v^it before the
@@ -177,104 +322,119 @@ struct StressIn {
 };
 
 template <int U>
-__global__ __launch_bounds__(256) void k_stress_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p, Src src,
-                                                    int it, float* __restrict__ ring) {
-  int tz, tx;
-  const int chunk = tile_of(tl, tz, tx);
-  const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
-  const int z = tz * kBZ + threadIdx.x;
-  const int x = tx * (int)blockDim.y + threadIdx.y;
+__device__ __forceinline__ void stress_fwd(const Part& P, int tile, int it) {
+  const Geo& g = P.g;
+  const int z = (tile % P.gzU) * kBZ + (int)threadIdx.x;
+  const int x = (tile / P.gzU) * (int)blockDim.y + (int)threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
-  const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i];
-  const float l2m = lam + 2.f * mu;
+  const Cell C = cell_of(bo, nz);
+  const int Nb = (int)(g.N * 4);  // bytes per shot
+  const Rsrc Rp_vxx = rsrc(P.p.vxx);
+  const Rsrc Rp_vxz = rsrc(P.p.vxz);
+  const Rsrc Rp_vzx = rsrc(P.p.vzx);
+  const Rsrc Rp_vzz = rsrc(P.p.vzz);
+  const Rsrc Rw_sxx = rsrc(P.w.sxx);
+  const Rsrc Rw_sxz = rsrc(P.w.sxz);
+  const Rsrc Rw_szz = rsrc(P.w.szz);
+  const Rsrc Rw_vx = rsrc(P.w.vx);
+  const Rsrc Rw_vz = rsrc(P.w.vz);
+  const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i];
+  const double l2m = (double)lam + 2.0 * (double)mu;  // (d_Lambda + 2.0*d_Mu)
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
   const bool px = x < g.nPml || x > g.nx - g.nPml - 1;
-  float bz = 0.f, az = 0.f, rkz = 1.f, bzh = 0.f, azh = 0.f, rkzh = 1.f;
-  float bx = 0.f, ax = 0.f, rkx = 1.f, bxh = 0.f, axh = 0.f, rkxh = 1.f;
+  float bz = 0.f, az = 0.f, bzh = 0.f, azh = 0.f, bx = 0.f, ax = 0.f, bxh = 0.f, axh = 0.f;
+  Kdiv kz{}, kzh{}, kx{}, kxh{};
   if (pz) {
-    bz = c.bz[z], az = c.az[z], rkz = 1.f / c.Kz[z];
-    bzh = c.bzh[z], azh = c.azh[z], rkzh = 1.f / c.Kzh[z];
+    bz = P.c.bz[z], az = P.c.az[z], kz = kdiv_of(P.c.Kz[z]);
+    bzh = P.c.bzh[z], azh = P.c.azh[z], kzh = kdiv_of(P.c.Kzh[z]);
   }
   if (px) {
-    bx = c.bx[x], ax = c.ax[x], rkx = 1.f / c.Kx[x];
-    bxh = c.bxh[x], axh = c.axh[x], rkxh = 1.f / c.Kxh[x];
+    bx = P.c.bx[x], ax = P.c.ax[x], kx = kdiv_of(P.c.Kx[x]);
+    bxh = P.c.bxh[x], axh = P.c.axh[x], kxh = kdiv_of(P.c.Kxh[x]);
   }
-  const int slot = ring ? ring_slot(z, x, g) : -1;
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s0 = sb; s0 < se; s0 += U) {
+  const int slot = P.ring ? ring_slot(z, x, g) : -1;
+  const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  const double dt64 = (double)dt;
+  const int ns = g.ns;
+  for (int s0 = 0; s0 < ns; s0 += U) {
     StressIn r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= se) break;
-      const long long so = s * g.N;
-      const float* vz = w.vz + so;
-      const float* vx = w.vx + so;
+      if (s >= ns) break;
+      const int so = s * Nb;
       StressIn& q = r[u];
-      q.vz0 = ldo(vz, bo), q.vzm1 = ldo(vz - 1, bo), q.vzp1 = ldo(vz + 1, bo), q.vzm2 = ldo(vz - 2, bo);
-      q.vzpx = ldo(vz + nz, bo), q.vzp2x = ldo(vz + 2 * nz, bo), q.vzmx = ldo(vz - nz, bo);
-      q.vx0 = ldo(vx, bo), q.vxmx = ldo(vx - nz, bo), q.vxpx = ldo(vx + nz, bo), q.vxm2x = ldo(vx - 2 * nz, bo);
-      q.vxp1 = ldo(vx + 1, bo), q.vxp2 = ldo(vx + 2, bo), q.vxm1 = ldo(vx - 1, bo);
-      q.szz = ldo(w.szz + so, bo), q.sxx = ldo(w.sxx + so, bo), q.sxz = ldo(w.sxz + so, bo);
-      if (pz) q.mvzz = ldo(p.vzz + so, bo), q.mvxz = ldo(p.vxz + so, bo);
-      if (px) q.mvxx = ldo(p.vxx + so, bo), q.mvzx = ldo(p.vzx + so, bo);
-      q.sz = src.zx[2 * s], q.sx = src.zx[2 * s + 1];
+      const f4 vzc = lz4(Rw_vz, so, C, -2), vxc = lz4(Rw_vx, so, C, -1);
+      q.vzm2 = vzc[0], q.vzm1 = vzc[1], q.vz0 = vzc[2], q.vzp1 = vzc[3];
+      q.vzpx = lx(Rw_vz, so, C, 1), q.vzp2x = lx(Rw_vz, so, C, 2), q.vzmx = lx(Rw_vz, so, C, -1);
+      q.vxm1 = vxc[0], q.vx0 = vxc[1], q.vxp1 = vxc[2], q.vxp2 = vxc[3];
+      q.vxmx = lx(Rw_vx, so, C, -1), q.vxpx = lx(Rw_vx, so, C, 1), q.vxm2x = lx(Rw_vx, so, C, -2);
+      q.szz = lz(Rw_szz, so, C, 0), q.sxx = lz(Rw_sxx, so, C, 0), q.sxz = lz(Rw_sxz, so, C, 0);
+      if (pz) q.mvzz = lz(Rp_vzz, so, C, 0), q.mvxz = lz(Rp_vxz, so, C, 0);
+      if (px) q.mvxx = lz(Rp_vxx, so, C, 0), q.mvzx = lz(Rp_vzx, so, C, 0);
+      q.sz = P.src.zx[2 * s], q.sx = P.src.zx[2 * s + 1];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= se) break;
-      const long long so = s * g.N;
+      if (s >= ns) break;
+      const int so = s * Nb;
       const StressIn& q = r[u];
-      float dvz_dz = (kC1 * (q.vz0 - q.vzm1) - kC2 * (q.vzp1 - q.vzm2)) * rdz;
-      float dvx_dx = (kC1 * (q.vx0 - q.vxmx) - kC2 * (q.vxpx - q.vxm2x)) * rdx;
-      float dvx_dz = (kC1 * (q.vxp1 - q.vx0) - kC2 * (q.vxp2 - q.vxm1)) * rdz;
-      float dvz_dx = (kC1 * (q.vzpx - q.vz0) - kC2 * (q.vzp2x - q.vzmx)) * rdx;
+      float dvz_dz = dfw(q.vz0, q.vzm1, q.vzp1, q.vzm2, dz, rdz);
+      float dvx_dx = dfw(q.vx0, q.vxmx, q.vxpx, q.vxm2x, dx, rdx);
       if (pz) {
-        const float a = bz * q.mvzz + az * dvz_dz;
-        sto(p.vzz + so, bo, a);
-        dvz_dz = dvz_dz * rkz + a;
-        const float b = bzh * q.mvxz + azh * dvx_dz;
-        sto(p.vxz + so, bo, b);
-        dvx_dz = dvx_dz * rkzh + b;
+        const float mm = bz * q.mvzz + az * dvz_dz;
+        st(Rp_vzz, so, C, mm);
+        dvz_dz = kdiv(dvz_dz, kz) + mm;
       }
       if (px) {
-        const float a = bx * q.mvxx + ax * dvx_dx;
-        sto(p.vxx + so, bo, a);
-        dvx_dx = dvx_dx * rkx + a;
-        const float b = bxh * q.mvzx + axh * dvz_dx;
-        sto(p.vzx + so, bo, b);
-        dvz_dx = dvz_dx * rkxh + b;
+        const float mm = bx * q.mvxx + ax * dvx_dx;
+        st(Rp_vxx, so, C, mm);
+        dvx_dx = kdiv(dvx_dx, kx) + mm;
       }
       float szz = q.szz, sxx = q.sxx, sxz = q.sxz;
       if (slot >= 0) {
-        *ring_ptr(ring, g, it, 0, s, slot) = szz;
-        *ring_ptr(ring, g, it, 1, s, slot) = sxz;
-        *ring_ptr(ring, g, it, 2, s, slot) = sxx;
-        *ring_ptr(ring, g, it, 3, s, slot) = q.vz0;
-        *ring_ptr(ring, g, it, 4, s, slot) = q.vx0;
+        *ring_ptr(P.ring, g, it, 0, s, slot) = szz;
+        *ring_ptr(P.ring, g, it, 1, s, slot) = sxz;
+        *ring_ptr(P.ring, g, it, 2, s, slot) = sxx;
+        *ring_ptr(P.ring, g, it, 3, s, slot) = q.vz0;
+        *ring_ptr(P.ring, g, it, 4, s, slot) = q.vx0;
       }
-      szz += (l2m * dvz_dz + lam * dvx_dx) * dt;
-      sxx += (lam * dvz_dz + l2m * dvx_dx) * dt;
-      sxz += amu * (dvx_dz + dvz_dx) * dt;
+      szz = (float)((double)szz + (l2m * (double)dvz_dz + (double)(lam * dvx_dx)) * dt64);
+      sxx = (float)((double)sxx + ((double)(lam * dvz_dz) + l2m * (double)dvx_dx) * dt64);
+      float dvx_dz = dfw(q.vxp1, q.vx0, q.vxp2, q.vxm1, dz, rdz);
+      float dvz_dx = dfw(q.vzpx, q.vz0, q.vzp2x, q.vzmx, dx, rdx);
+      if (pz) {
+        const float mm = bzh * q.mvxz + azh * dvx_dz;
+        st(Rp_vxz, so, C, mm);
+        dvx_dz = kdiv(dvx_dz, kzh) + mm;
+      }
+      if (px) {
+        const float mm = bxh * q.mvzx + axh * dvz_dx;
+        st(Rp_vzx, so, C, mm);
+        dvz_dx = kdiv(dvz_dx, kxh) + mm;
+      }
+      sxz = sxz + amu * (dvx_dz + dvz_dx) * dt;
       if (z == q.sz && x == q.sx) {
-        const float inj = src.inj[(size_t)s * g.nSteps + it];
-        szz += inj;
-        sxx += 3.f * inj;
+        // add_source: szz += scale*amp*dt*gauss (float), sxx += RSXXZZ*scale*amp*dt*gauss
+        // (double); the 9x9 Gaussian is 1 at the centre and 0 elsewhere in FP32
+        const float amp = P.src.amp[(size_t)s * g.nSteps + it];
+        szz = szz + kSrcScale * amp * dt;
+        sxx = (float)((double)sxx + kRS * (double)kSrcScale * (double)amp * dt64);
       }
-      sto(w.szz + so, bo, szz);
-      sto(w.sxx + so, bo, sxx);
-      sto(w.sxz + so, bo, sxz);
+      st(Rw_szz, so, C, szz);
+      st(Rw_sxx, so, C, sxx);
+      st(Rw_sxz, so, C, sxz);
     }
   }
 }
 
 // el_velocity.cu:45-82.  x-PML test is `x > nx - nPml` (el_velocity.cu:56,71).
-// With `illum` set, the launch's sum over shots of vz^2 + vx^2 is added to it
-// (forward illumination; no reference counterpart).
+// With an illumination buffer, the part's sum over shots of vz^2 + vx^2 is
+// added to it (forward illumination; no reference counterpart).
 struct VelIn {
   float szz0, szzp1, szzp2, szzm1;
   float sxz0, sxzmx, sxzpx, sxzm2x, sxzm1, sxzp1, sxzm2;
@@ -284,889 +444,671 @@ struct VelIn {
 };
 
 template <int U>
-__device__ __forceinline__ void k_vel_fwd_body(int bid, Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p,
-                                                float* __restrict__ illum) {
-  int tz, tx;
-  const int chunk = tile_of(tl, bid, tz, tx);
-  const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
-  const int z = tz * kBZ + threadIdx.x;
-  const int x = tx * (int)blockDim.y + threadIdx.y;
+__device__ __forceinline__ void vel_fwd(const Part& P, int tile) {
+  const Geo& g = P.g;
+  const int z = (tile % P.gzU) * kBZ + (int)threadIdx.x;
+  const int x = (tile / P.gzU) * (int)blockDim.y + (int)threadIdx.y;
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
-  const float ba = m.ba[i], bb = m.bb[i];
+  const Cell C = cell_of(bo, nz);
+  const int Nb = (int)(g.N * 4);  // bytes per shot
+  const Rsrc Rp_sxx = rsrc(P.p.sxx);
+  const Rsrc Rp_sxzx = rsrc(P.p.sxzx);
+  const Rsrc Rp_sxzz = rsrc(P.p.sxzz);
+  const Rsrc Rp_szz = rsrc(P.p.szz);
+  const Rsrc Rw_sxx = rsrc(P.w.sxx);
+  const Rsrc Rw_sxz = rsrc(P.w.sxz);
+  const Rsrc Rw_szz = rsrc(P.w.szz);
+  const Rsrc Rw_vx = rsrc(P.w.vx);
+  const Rsrc Rw_vz = rsrc(P.w.vz);
+  const float ba = P.m.ba[i], bb = P.m.bb[i];
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
   const bool px = x < g.nPml || x > g.nx - g.nPml;
-  float bz = 0.f, az = 0.f, rkz = 1.f, bzh = 0.f, azh = 0.f, rkzh = 1.f;
-  float bx = 0.f, ax = 0.f, rkx = 1.f, bxh = 0.f, axh = 0.f, rkxh = 1.f;
+  float bz = 0.f, az = 0.f, bzh = 0.f, azh = 0.f, bx = 0.f, ax = 0.f, bxh = 0.f, axh = 0.f;
+  Kdiv kz{}, kzh{}, kx{}, kxh{};
   if (pz) {
-    bz = c.bz[z], az = c.az[z], rkz = 1.f / c.Kz[z];
-    bzh = c.bzh[z], azh = c.azh[z], rkzh = 1.f / c.Kzh[z];
+    bz = P.c.bz[z], az = P.c.az[z], kz = kdiv_of(P.c.Kz[z]);
+    bzh = P.c.bzh[z], azh = P.c.azh[z], kzh = kdiv_of(P.c.Kzh[z]);
   }
   if (px) {
-    bx = c.bx[x], ax = c.ax[x], rkx = 1.f / c.Kx[x];
-    bxh = c.bxh[x], axh = c.axh[x], rkxh = 1.f / c.Kxh[x];
+    bx = P.c.bx[x], ax = P.c.ax[x], kx = kdiv_of(P.c.Kx[x]);
+    bxh = P.c.bxh[x], axh = P.c.axh[x], kxh = kdiv_of(P.c.Kxh[x]);
   }
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  float il = 0.f;  // illumination of this launch's shots
-  for (int s0 = sb; s0 < se; s0 += U) {
+  const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  const int ns = g.ns;
+  float il = 0.f;  // illumination of this part's shots
+  for (int s0 = 0; s0 < ns; s0 += U) {
     VelIn r[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= se) break;
-      const long long so = s * g.N;
-      const float* szz = w.szz + so;
-      const float* sxx = w.sxx + so;
-      const float* sxz = w.sxz + so;
+      if (s >= ns) break;
+      const int so = s * Nb;
       VelIn& q = r[u];
-      q.szz0 = ldo(szz, bo), q.szzp1 = ldo(szz + 1, bo), q.szzp2 = ldo(szz + 2, bo), q.szzm1 = ldo(szz - 1, bo);
-      q.sxz0 = ldo(sxz, bo), q.sxzmx = ldo(sxz - nz, bo), q.sxzpx = ldo(sxz + nz, bo), q.sxzm2x = ldo(sxz - 2 * nz, bo);
-      q.sxzm1 = ldo(sxz - 1, bo), q.sxzp1 = ldo(sxz + 1, bo), q.sxzm2 = ldo(sxz - 2, bo);
-      q.sxxpx = ldo(sxx + nz, bo), q.sxx0 = ldo(sxx, bo), q.sxxp2x = ldo(sxx + 2 * nz, bo), q.sxxmx = ldo(sxx - nz, bo);
-      if (pz) q.mszz = ldo(p.szz + so, bo), q.msxzz = ldo(p.sxzz + so, bo);
-      if (px) q.msxzx = ldo(p.sxzx + so, bo), q.msxx = ldo(p.sxx + so, bo);
-      q.vz = ldo(w.vz + so, bo), q.vx = ldo(w.vx + so, bo);
+      const f4 szc = lz4(Rw_szz, so, C, -1), sxc = lz4(Rw_sxz, so, C, -2);
+      q.szzm1 = szc[0], q.szz0 = szc[1], q.szzp1 = szc[2], q.szzp2 = szc[3];
+      q.sxzm2 = sxc[0], q.sxzm1 = sxc[1], q.sxz0 = sxc[2], q.sxzp1 = sxc[3];
+      q.sxzmx = lx(Rw_sxz, so, C, -1), q.sxzpx = lx(Rw_sxz, so, C, 1), q.sxzm2x = lx(Rw_sxz, so, C, -2);
+      q.sxxpx = lx(Rw_sxx, so, C, 1), q.sxx0 = lz(Rw_sxx, so, C, 0), q.sxxp2x = lx(Rw_sxx, so, C, 2), q.sxxmx = lx(Rw_sxx, so, C, -1);
+      if (pz) q.mszz = lz(Rp_szz, so, C, 0), q.msxzz = lz(Rp_sxzz, so, C, 0);
+      if (px) q.msxzx = lz(Rp_sxzx, so, C, 0), q.msxx = lz(Rp_sxx, so, C, 0);
+      q.vz = lz(Rw_vz, so, C, 0), q.vx = lz(Rw_vx, so, C, 0);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
-      if (s >= se) break;
-      const long long so = s * g.N;
+      if (s >= ns) break;
+      const int so = s * Nb;
       const VelIn& q = r[u];
-      float dszz_dz = (kC1 * (q.szzp1 - q.szz0) - kC2 * (q.szzp2 - q.szzm1)) * rdz;
-      float dsxz_dx = (kC1 * (q.sxz0 - q.sxzmx) - kC2 * (q.sxzpx - q.sxzm2x)) * rdx;
-      float dsxz_dz = (kC1 * (q.sxz0 - q.sxzm1) - kC2 * (q.sxzp1 - q.sxzm2)) * rdz;
-      float dsxx_dx = (kC1 * (q.sxxpx - q.sxx0) - kC2 * (q.sxxp2x - q.sxxmx)) * rdx;
+      float dszz_dz = dfw(q.szzp1, q.szz0, q.szzp2, q.szzm1, dz, rdz);
+      float dsxz_dx = dfw(q.sxz0, q.sxzmx, q.sxzpx, q.sxzm2x, dx, rdx);
       if (pz) {
-        const float a = bzh * q.mszz + azh * dszz_dz;
-        sto(p.szz + so, bo, a);
-        dszz_dz = dszz_dz * rkzh + a;
-        const float b = bz * q.msxzz + az * dsxz_dz;
-        sto(p.sxzz + so, bo, b);
-        dsxz_dz = dsxz_dz * rkz + b;
+        const float mm = bzh * q.mszz + azh * dszz_dz;
+        st(Rp_szz, so, C, mm);
+        dszz_dz = kdiv(dszz_dz, kzh) + mm;
       }
       if (px) {
-        const float a = bx * q.msxzx + ax * dsxz_dx;
-        sto(p.sxzx + so, bo, a);
-        dsxz_dx = dsxz_dx * rkx + a;
-        const float b = bxh * q.msxx + axh * dsxx_dx;
-        sto(p.sxx + so, bo, b);
-        dsxx_dx = dsxx_dx * rkxh + b;
+        const float mm = bx * q.msxzx + ax * dsxz_dx;
+        st(Rp_sxzx, so, C, mm);
+        dsxz_dx = kdiv(dsxz_dx, kx) + mm;
       }
-      float vz = q.vz, vx = q.vx;
-      vz += (dszz_dz + dsxz_dx) * ba * dt;
-      vx += (dsxz_dz + dsxx_dx) * bb * dt;
-      sto(w.vz + so, bo, vz);
-      sto(w.vx + so, bo, vx);
+      const float vz = q.vz + (dszz_dz + dsxz_dx) * ba * dt;
+      float dsxz_dz = dfw(q.sxz0, q.sxzm1, q.sxzp1, q.sxzm2, dz, rdz);
+      float dsxx_dx = dfw(q.sxxpx, q.sxx0, q.sxxp2x, q.sxxmx, dx, rdx);
+      if (pz) {
+        const float mm = bz * q.msxzz + az * dsxz_dz;
+        st(Rp_sxzz, so, C, mm);
+        dsxz_dz = kdiv(dsxz_dz, kz) + mm;
+      }
+      if (px) {
+        const float mm = bxh * q.msxx + axh * dsxx_dx;
+        st(Rp_sxx, so, C, mm);
+        dsxx_dx = kdiv(dsxx_dx, kxh) + mm;
+      }
+      const float vx = q.vx + (dsxz_dz + dsxx_dx) * bb * dt;
+      st(Rw_vz, so, C, vz);
+      st(Rw_vx, so, C, vx);
       il += vz * vz + vx * vx;
     }
   }
-  if (illum) illum[i] += il;
+  if (P.illum) P.illum[i] += il;
 }
 
-template <int U>
-__global__ __launch_bounds__(256) void k_vel_fwd(Tiling tl, Geo g, Mdl m, Pml c, Wave w, Mem p,
-                                                float* __restrict__ illum) {
-  k_vel_fwd_body<U>(blockIdx.x, tl, g, m, c, w, p, illum);
-}
-
-// ------------------------------------------------------------ fused forward
-// One time step (el_stress.cu:50-88 + add_source + el_velocity.cu:45-82 and
-// the ring save of from_bnd) in one pass.  A workgroup owns every row of one
-// shot for the x-segment [x0, x0+tx) and marches along x:
-//   column c  : sigma^{n+1}(., c) from v^n(., c-2..c+2)
-//   column c-2: v^{n+1}(., c-2) from sigma^{n+1}(., c-4..c)
-// x-neighbours are carried in per-thread register rings, z-neighbours are read
-// from LDS.  sigma^{n+1} is recomputed on 2 halo columns each side (same code,
-// same bits as the owner's value).  State and memory variables are read from
-// buffer A and written to buffer B, so halo reads never see a neighbour's new
-// values.
-struct Seg {
-  int tx, nseg, n, xcd;
-};
-
-__device__ inline void seg_of(const Seg& q, int ns, int& seg, int& s) {
-  int b = blockIdx.x;
-  if (q.xcd) {
-    const int per = q.n / 8;
-    if (b < per * 8) b = (b % 8) * per + b / 8;
-  }
-  seg = b / ns;
-  s = b % ns;
-}
-
-__device__ inline int mod3(int c) { return (c + 3) % 3; }
-
-__global__ __launch_bounds__(1024) void k_fwd_step(Seg q, Geo g, Mdl m, Pml c, Wave A, Wave B,
-                                                   Mem pA, Mem pB, Src src, int it,
-                                                   float* __restrict__ ring) {
-  extern __shared__ float lds[];
-  const int NT = blockDim.x;
-  float* lvz = lds;            // [2][NT] v^n at the current column
-  float* lvx = lds + 2 * NT;
-  float* lszz = lds + 4 * NT;  // [3][NT] sigma^{n+1} of the last 3 columns
-  float* lsxz = lds + 7 * NT;
-  const int z = threadIdx.x;
-  int seg, s;
-  seg_of(q, g.ns, seg, s);
-  const int x0 = seg * q.tx;
-  const int x1 = min(x0 + q.tx, g.nx);
-  const int nz = g.nz;
-  const long long so = (long long)s * g.N;
-  const bool zval = z < nz;
-  const bool zin = z >= 2 && z <= g.nzA - 3;
-  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
-  float bz = 0.f, az = 0.f, rkz = 1.f, bzh = 0.f, azh = 0.f, rkzh = 1.f;
-  if (zin && pz) {
-    bz = c.bz[z], az = c.az[z], rkz = 1.f / c.Kz[z];
-    bzh = c.bzh[z], azh = c.azh[z], rkzh = 1.f / c.Kzh[z];
-  }
-  const int zs = src.zx[2 * s], xs = src.zx[2 * s + 1];
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  auto ld = [&](const float* f, int col) -> float {
-    return (zval && col >= 0 && col < g.nx) ? f[so + (long long)col * nz + z] : 0.f;
-  };
-  // v rings: r*[k] holds column c-2+k
-  float vz0 = ld(A.vz, x0 - 4), vz1 = ld(A.vz, x0 - 3), vz2 = ld(A.vz, x0 - 2),
-        vz3 = ld(A.vz, x0 - 1), vz4;
-  float vx0 = ld(A.vx, x0 - 4), vx1 = ld(A.vx, x0 - 3), vx2 = ld(A.vx, x0 - 2),
-        vx3 = ld(A.vx, x0 - 1), vx4;
-  // sigma^{n+1} rings: sxz at columns c-4..c, sxx at columns c-3..c
-  float sxz0 = 0.f, sxz1 = 0.f, sxz2 = 0.f, sxz3 = 0.f, sxz4 = 0.f;
-  float sxx0 = 0.f, sxx1 = 0.f, sxx2 = 0.f, sxx3 = 0.f;
-  for (int col = x0 - 2; col <= x1 + 1; ++col) {
-    // ---- phase 1: bring v^n(., col+2) in, publish v^n(., col) for z-stencils
-    vz4 = ld(A.vz, col + 2);
-    vx4 = ld(A.vx, col + 2);
-    const int sv = (col & 1) * NT;
-    lvz[sv + z] = vz2;
-    lvx[sv + z] = vx2;
-    __syncthreads();
-    // ---- phase 2: sigma^{n+1}(z, col)
-    float nszz = 0.f, nsxx = 0.f, nsxz = 0.f;
-    if (zval && col >= 0 && col < g.nx) {
-      const long long i = (long long)col * nz + z;
-      const long long o = so + i;
-      float szz = A.szz[o], sxx = A.sxx[o], sxz = A.sxz[o];
-      const bool own = col >= x0 && col < x1;
-      if (zin && col >= 2 && col <= g.nx - 3) {
-        const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i];
-        const float l2m = lam + 2.f * mu;
-        const float* Vz = lvz + sv + z;
-        const float* Vx = lvx + sv + z;
-        float dvz_dz = (kC1 * (vz2 - Vz[-1]) - kC2 * (Vz[1] - Vz[-2])) * rdz;
-        float dvx_dx = (kC1 * (vx2 - vx1) - kC2 * (vx3 - vx0)) * rdx;
-        float dvx_dz = (kC1 * (Vx[1] - vx2) - kC2 * (Vx[2] - Vx[-1])) * rdz;
-        float dvz_dx = (kC1 * (vz3 - vz2) - kC2 * (vz4 - vz1)) * rdx;
-        if (pz) {
-          const float a = bz * pA.vzz[o] + az * dvz_dz;
-          dvz_dz = dvz_dz * rkz + a;
-          const float b = bzh * pA.vxz[o] + azh * dvx_dz;
-          dvx_dz = dvx_dz * rkzh + b;
-          if (own) {
-            pB.vzz[o] = a;
-            pB.vxz[o] = b;
-          }
-        }
-        if (col < g.nPml || col > g.nx - g.nPml - 1) {
-          const float a = c.bx[col] * pA.vxx[o] + c.ax[col] * dvx_dx;
-          dvx_dx = dvx_dx * (1.f / c.Kx[col]) + a;
-          const float b = c.bxh[col] * pA.vzx[o] + c.axh[col] * dvz_dx;
-          dvz_dx = dvz_dx * (1.f / c.Kxh[col]) + b;
-          if (own) {
-            pB.vxx[o] = a;
-            pB.vzx[o] = b;
-          }
-        }
-        if (own && ring) {
-          const int slot = ring_slot(z, col, g);
-          if (slot >= 0) {
-            *ring_ptr(ring, g, it, 0, s, slot) = szz;
-            *ring_ptr(ring, g, it, 1, s, slot) = sxz;
-            *ring_ptr(ring, g, it, 2, s, slot) = sxx;
-            *ring_ptr(ring, g, it, 3, s, slot) = vz2;
-            *ring_ptr(ring, g, it, 4, s, slot) = vx2;
-          }
-        }
-        szz += (l2m * dvz_dz + lam * dvx_dx) * dt;
-        sxx += (lam * dvz_dz + l2m * dvx_dx) * dt;
-        sxz += amu * (dvx_dz + dvz_dx) * dt;
-        if (z == zs && col == xs) {
-          const float inj = src.inj[(size_t)s * g.nSteps + it];
-          szz += inj;
-          sxx += 3.f * inj;
-        }
-        if (own) {
-          B.szz[o] = szz;
-          B.sxx[o] = sxx;
-          B.sxz[o] = sxz;
-        }
-      }
-      nszz = szz;
-      nsxx = sxx;
-      nsxz = sxz;
-    }
-    const int ss = mod3(col) * NT;
-    lszz[ss + z] = nszz;
-    lsxz[ss + z] = nsxz;
-    sxz0 = sxz1, sxz1 = sxz2, sxz2 = sxz3, sxz3 = sxz4, sxz4 = nsxz;
-    sxx0 = sxx1, sxx1 = sxx2, sxx2 = sxx3, sxx3 = nsxx;
-    __syncthreads();
-    // ---- phase 3: v^{n+1}(z, col-2)
-    const int cp = col - 2;
-    if (zin && cp >= x0 && cp < x1 && cp >= 2 && cp <= g.nx - 3) {
-      const long long i = (long long)cp * nz + z;
-      const long long o = so + i;
-      const float ba = m.ba[i], bb = m.bb[i];
-      const int sp = mod3(cp) * NT + z;
-      const float* Szz = lszz + sp;
-      const float* Sxz = lsxz + sp;
-      float dszz_dz = (kC1 * (Szz[1] - Szz[0]) - kC2 * (Szz[2] - Szz[-1])) * rdz;
-      float dsxz_dx = (kC1 * (sxz2 - sxz1) - kC2 * (sxz3 - sxz0)) * rdx;
-      float dsxz_dz = (kC1 * (Sxz[0] - Sxz[-1]) - kC2 * (Sxz[1] - Sxz[-2])) * rdz;
-      float dsxx_dx = (kC1 * (sxx2 - sxx1) - kC2 * (sxx3 - sxx0)) * rdx;
-      if (pz) {
-        const float a = bzh * pA.szz[o] + azh * dszz_dz;
-        pB.szz[o] = a;
-        dszz_dz = dszz_dz * rkzh + a;
-        const float b = bz * pA.sxzz[o] + az * dsxz_dz;
-        pB.sxzz[o] = b;
-        dsxz_dz = dsxz_dz * rkz + b;
-      }
-      if (cp < g.nPml || cp > g.nx - g.nPml) {
-        const float a = c.bx[cp] * pA.sxzx[o] + c.ax[cp] * dsxz_dx;
-        pB.sxzx[o] = a;
-        dsxz_dx = dsxz_dx * (1.f / c.Kx[cp]) + a;
-        const float b = c.bxh[cp] * pA.sxx[o] + c.axh[cp] * dsxx_dx;
-        pB.sxx[o] = b;
-        dsxx_dx = dsxx_dx * (1.f / c.Kxh[cp]) + b;
-      }
-      B.vz[o] = vz0 + (dszz_dz + dsxz_dx) * ba * dt;
-      B.vx[o] = vx0 + (dsxz_dz + dsxx_dx) * bb * dt;
-    }
-    vz0 = vz1, vz1 = vz2, vz2 = vz3, vz3 = vz4;
-    vx0 = vx1, vx1 = vx2, vx2 = vx3, vx3 = vx4;
-  }
-}
-
-// ------------------------------------------------------------ fused adjoint
-// One adjoint step: el_velocity_adj.cu:22-108, res_injection
-// (utilities.cu:569-580), el_stress_adj.cu:22-104 and source_grad
-// (utilities.cu:582-593) in one column-marching pass, same scheme as
-// k_fwd_step: adjoint velocity at column c, adjoint stress at column c-2,
-// state and memory variables read from A and written to B.  The residual is
-// injected into the owner's adjoint stress before the stress update, through
-// a per-shot, per-column receiver index.
-struct RecIdx {
-  const int* ptr;  // [ns][nx+1] start of each column's receivers
-  const int* zr;   // receiver row
-  const int* rg;   // receiver index in the time-major residual row
-};
-
-__global__ __launch_bounds__(1024) void k_adj_step(Seg q, Geo g, Mdl m, Pml c, Wave A, Wave B,
-                                                   Mem pA, Mem pB, Src src, RecIdx rx,
-                                                   const float* __restrict__ res_row, int R,
-                                                   float* __restrict__ stf_grad, int stf_it) {
-  extern __shared__ float lds[];
-  const int NT = blockDim.x;
-  float* lszz = lds;             // [2][NT] adjoint stress (old) at column c
-  float* lsxx = lds + 2 * NT;
-  float* lsxz = lds + 4 * NT;
-  float* lvz = lds + 6 * NT;     // [3][NT] adjoint velocity (new), last 3 columns
-  float* lvx = lds + 9 * NT;
-  float* lpzz = lds + 12 * NT;   // [3][NT] new mem_dsxz_dz
-  float* lpsz = lds + 15 * NT;   // [3][NT] new mem_dszz_dz
-  const int z = threadIdx.x;
-  int seg, s;
-  seg_of(q, g.ns, seg, s);
-  const int x0 = seg * q.tx;
-  const int x1 = min(x0 + q.tx, g.nx);
-  const int nz = g.nz;
-  const long long so = (long long)s * g.N;
-  const bool zval = z < nz;
-  const bool zin = z >= 2 && z <= g.nzA - 3;
-  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
-  const bool zlive = z <= g.nPml + 1 || z >= g.nzA - g.nPml - 2;
-  float az = 0.f, rkz = 1.f, bz = 0.f, azh = 0.f, rkzh = 1.f, bzh = 0.f;
-  if (zin) {
-    az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
-    azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
-  }
-  const int zs = src.zx[2 * s], xs = src.zx[2 * s + 1];
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  auto ld = [&](const float* f, int col) -> float {
-    return (zval && col >= 0 && col < g.nx) ? f[so + (long long)col * nz + z] : 0.f;
-  };
-  // adjoint stress (old) rings, slot k = column c-2+k
-  float zz0 = ld(A.szz, x0 - 4), zz1 = ld(A.szz, x0 - 3), zz2 = ld(A.szz, x0 - 2),
-        zz3 = ld(A.szz, x0 - 1), zz4;
-  float xx0 = ld(A.sxx, x0 - 4), xx1 = ld(A.sxx, x0 - 3), xx2 = ld(A.sxx, x0 - 2),
-        xx3 = ld(A.sxx, x0 - 1), xx4;
-  float xz0 = ld(A.sxz, x0 - 4), xz1 = ld(A.sxz, x0 - 3), xz2 = ld(A.sxz, x0 - 2),
-        xz3 = ld(A.sxz, x0 - 1), xz4;
-  // new adjoint velocity rings: vz at columns c-3..c, vx at c-4..c
-  float nvz0 = 0.f, nvz1 = 0.f, nvz2 = 0.f, nvz3 = 0.f;
-  float nvx0 = 0.f, nvx1 = 0.f, nvx2 = 0.f, nvx3 = 0.f, nvx4 = 0.f;
-  // new velocity-side memory variables: mem_dsxz_dx at c-3..c, mem_dsxx_dx at c-4..c
-  float pxz0 = 0.f, pxz1 = 0.f, pxz2 = 0.f, pxz3 = 0.f;
-  float pxx0 = 0.f, pxx1 = 0.f, pxx2 = 0.f, pxx3 = 0.f, pxx4 = 0.f;
-  for (int col = x0 - 2; col <= x1 + 1; ++col) {
-    // ---- phase 1
-    zz4 = ld(A.szz, col + 2);
-    xx4 = ld(A.sxx, col + 2);
-    xz4 = ld(A.sxz, col + 2);
-    const int sv = (col & 1) * NT;
-    lszz[sv + z] = zz2;
-    lsxx[sv + z] = xx2;
-    lsxz[sv + z] = xz2;
-    __syncthreads();
-    // ---- phase 2: adjoint velocity at (z, col)
-    float vzn = 0.f, vxn = 0.f, psxz = 0.f, psxx = 0.f, psxzz = 0.f, pszz = 0.f;
-    const bool own = col >= x0 && col < x1;
-    if (zval && col >= 0 && col < g.nx) {
-      const long long i = (long long)col * nz + z;
-      const long long o = so + i;
-      vzn = A.vz[o];
-      vxn = A.vx[o];
-      psxz = pA.sxzx[o];
-      psxx = pA.sxx[o];
-      psxzz = pA.sxzz[o];
-      pszz = pA.szz[o];
-      if (own && stf_it >= 0 && z == zs && col == xs)
-        stf_grad[(size_t)s * g.nSteps + stf_it] = -(zz2 + 3.f * xx2) * dt;
-      if (zin && col >= 2 && col <= g.nx - 3) {
-        const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
-        const float l2m = lam + 2.f * mu;
-        const float ax = c.ax[col], rkx = 1.f / c.Kx[col], bx = c.bx[col];
-        const float axh = c.axh[col], rkxh = 1.f / c.Kxh[col], bxh = c.bxh[col];
-        const bool px = col < g.nPml || col > g.nx - g.nPml - 1;
-        const float* Szz = lszz + sv + z;
-        const float* Sxx = lsxx + sv + z;
-        const float* Sxz = lsxz + sv + z;
-        const float dszz_dx = (-kC1 * (zz3 - zz2) + kC2 * (zz4 - zz1)) * rdx;
-        const float dsxx_dx = (-kC1 * (xx3 - xx2) + kC2 * (xx4 - xx1)) * rdx;
-        const float dsxz_dz = (-kC1 * (Sxz[0] - Sxz[-1]) + kC2 * (Sxz[1] - Sxz[-2])) * rdz;
-        vxn += (lam * dszz_dx * rkx * dt + l2m * dsxx_dx * rkx * dt + amu * rkzh * dsxz_dz * dt);
-        if (ax != 0.f) {
-          const float* qv = pA.vxx + o;
-          vxn += ax * ((-kC1 * (qv[nz] - qv[0]) + kC2 * (qv[2 * nz] - qv[-nz])) * rdx);
-        }
-        if (azh != 0.f) {
-          const float* qv = pA.vxz + o;
-          vxn += azh * ((-kC1 * (qv[0] - qv[-1]) + kC2 * (qv[1] - qv[-2])) * rdz);
-        }
-        if (px) psxx = bxh * psxx + bb * vxn * dt;
-        if (pz) psxzz = bz * psxzz + bb * vxn * dt;
-        const float dszz_dz = (-kC1 * (Szz[1] - Szz[0]) + kC2 * (Szz[2] - Szz[-1])) * rdz;
-        const float dsxx_dz = (-kC1 * (Sxx[1] - Sxx[0]) + kC2 * (Sxx[2] - Sxx[-1])) * rdz;
-        const float dsxz_dx = (-kC1 * (xz2 - xz1) + kC2 * (xz3 - xz0)) * rdx;
-        vzn += (l2m * dszz_dz * rkz * dt + lam * dsxx_dz * rkz * dt + amu * rkxh * dsxz_dx * dt);
-        if (az != 0.f) {
-          const float* qv = pA.vzz + o;
-          vzn += az * ((-kC1 * (qv[1] - qv[0]) + kC2 * (qv[2] - qv[-1])) * rdz);
-        }
-        if (axh != 0.f) {
-          const float* qv = pA.vzx + o;
-          vzn += axh * ((-kC1 * (qv[0] - qv[-nz]) + kC2 * (qv[nz] - qv[-2 * nz])) * rdx);
-        }
-        if (px) psxz = bx * psxz + ba * vzn * dt;
-        if (pz) pszz = bzh * pszz + ba * vzn * dt;
-        if (own) {
-          B.vx[o] = vxn;
-          B.vz[o] = vzn;
-          if (px) {
-            pB.sxx[o] = psxx;
-            pB.sxzx[o] = psxz;
-          }
-          if (pz) {
-            pB.sxzz[o] = psxzz;
-            pB.szz[o] = pszz;
-          }
-        }
-      }
-    }
-    const int sn = mod3(col) * NT;
-    lvz[sn + z] = vzn;
-    lvx[sn + z] = vxn;
-    lpzz[sn + z] = psxzz;
-    lpsz[sn + z] = pszz;
-    nvz0 = nvz1, nvz1 = nvz2, nvz2 = nvz3, nvz3 = vzn;
-    nvx0 = nvx1, nvx1 = nvx2, nvx2 = nvx3, nvx3 = nvx4, nvx4 = vxn;
-    pxz0 = pxz1, pxz1 = pxz2, pxz2 = pxz3, pxz3 = psxz;
-    pxx0 = pxx1, pxx1 = pxx2, pxx2 = pxx3, pxx3 = pxx4, pxx4 = psxx;
-    __syncthreads();
-    // ---- phase 3: residual injection + adjoint stress at (z, col-2)
-    const int cp = col - 2;
-    if (cp >= x0 && cp < x1 && zval) {
-      const long long i = (long long)cp * nz + z;
-      const long long o = so + i;
-      float szz = zz0, sxx = xx0, sxz = xz0;
-      const int* rp = rx.ptr + (size_t)s * (g.nx + 1);
-      for (int k = rp[cp]; k < rp[cp + 1]; ++k)
-        if (rx.zr[k] == z) {
-          const float r = res_row[rx.rg[k]];
-          szz += r;
-          sxx += 3.f * r;
-        }
-      if (zin && cp >= 2 && cp <= g.nx - 3) {
-        const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
-        const float l2m = lam + 2.f * mu;
-        const float ax = c.ax[cp], rkx = 1.f / c.Kx[cp], bx = c.bx[cp];
-        const float axh = c.axh[cp], rkxh = 1.f / c.Kxh[cp], bxh = c.bxh[cp];
-        const bool xlive = cp <= g.nPml + 1 || cp >= g.nx - g.nPml - 2;
-        const int sp = mod3(cp) * NT + z;
-        const float* Vz = lvz + sp;
-        const float* Vx = lvx + sp;
-        // nvz slot k = column col-3+k (cp = slot 1); nvx slot k = column col-4+k (cp = slot 2)
-        const float dvz_dx = (-kC1 * (nvz2 - nvz1) + kC2 * (nvz3 - nvz0)) * rdx;
-        const float dvx_dz = (-kC1 * (Vx[1] - Vx[0]) + kC2 * (Vx[2] - Vx[-1])) * rdz;
-        sxz += (dvz_dx * rkx * ba * dt + dvx_dz * rkz * bb * dt);
-        if (ax != 0.f)  // mem_dsxz_dx at cp-1..cp+2: pxz slot k = column col-3+k
-          sxz += ax * ((-kC1 * (pxz2 - pxz1) + kC2 * (pxz3 - pxz0)) * rdx);
-        if (az != 0.f) {
-          const float* qv = lpzz + sp;
-          sxz += az * ((-kC1 * (qv[1] - qv[0]) + kC2 * (qv[2] - qv[-1])) * rdz);
-        }
-        const float dvx_dx = (-kC1 * (nvx2 - nvx1) + kC2 * (nvx3 - nvx0)) * rdx;
-        const float dvz_dz = (-kC1 * (Vz[0] - Vz[-1]) + kC2 * (Vz[1] - Vz[-2])) * rdz;
-        sxx += bb * dvx_dx * rkxh * dt;
-        if (axh != 0.f)  // mem_dsxx_dx at cp-2..cp+1: pxx slot k = column col-4+k
-          sxx += axh * ((-kC1 * (pxx2 - pxx1) + kC2 * (pxx3 - pxx0)) * rdx);
-        szz += ba * dvz_dz * rkzh * dt;
-        if (azh != 0.f) {
-          const float* qv = lpsz + sp;
-          szz += azh * ((-kC1 * (qv[0] - qv[-1]) + kC2 * (qv[1] - qv[-2])) * rdz);
-        }
-        if (xlive) {
-          pB.vzx[o] = bxh * pA.vzx[o] + sxz * amu * dt;
-          pB.vxx[o] = bx * pA.vxx[o] + lam * szz * dt + l2m * sxx * dt;
-        }
-        if (zlive) {
-          pB.vxz[o] = bzh * pA.vxz[o] + sxz * amu * dt;
-          pB.vzz[o] = bz * pA.vzz[o] + l2m * szz * dt + lam * sxx * dt;
-        }
-      }
-      B.szz[o] = szz;
-      B.sxx[o] = sxx;
-      B.sxz[o] = sxz;
-    }
-    zz0 = zz1, zz1 = zz2, zz2 = zz3, zz3 = zz4;
-    xx0 = xx1, xx1 = xx2, xx2 = xx3, xx3 = xx4;
-    xz0 = xz1, xz1 = xz2, xz2 = xz3, xz3 = xz4;
-  }
-}
-
-// ------------------------------------------------------------ fused reverse
-// One reverse step: el_velocity.cu:84-117 and el_stress.cu:90-131 with
-// isFor = false, the ring restores of to_bnd (utilities.cu:394-424) and the
-// source removal of add_source, plus all gradient terms, in one column-marching
-// pass: v^{it}(c) from sigma^{it+1}(c-2..c+2), then sigma^{it}(c-2) from
-// v^{it}(c-4..c).  A workgroup owns one x-segment of TXR columns for a group of
-// shots; it walks the group's shots one after another and keeps the gradient
-// terms of its cells in LDS (each thread only touches its own row), then adds
-// them once to the group's slice of
-// the partial accumulators (one slice per group, so no two workgroups write
-// the same word).  State is read from A (step it+1) and written to B (step it).
-constexpr int TXR = 8;
-
-struct SegG {
-  int nseg, ngrp, gsz, n, xcd;
-};
-
-__global__ __launch_bounds__(1024) void k_rev_step(SegG q, Geo g, Mdl m, Wave A, Wave B, Wave adj,
-                                                   Src src, const float* __restrict__ ring,
-                                                   int it, float* __restrict__ accp) {
-  extern __shared__ float lds[];
-  const int NT = blockDim.x;
-  float* lszz = lds;          // [2][NT] sigma^{it+1} at column c
-  float* lsxz = lds + 2 * NT;
-  float* lvz = lds + 4 * NT;  // [3][NT] v^{it} of the last 3 columns
-  float* lvx = lds + 7 * NT;
-  float* lacc = lds + 10 * NT;  // [5][TXR][NT] gradient terms of the owned cells
-  const int z = threadIdx.x;
-  int b = blockIdx.x;
-  if (q.xcd) {
-    const int per = q.n / 8;
-    if (b < per * 8) b = (b % 8) * per + b / 8;
-  }
-  const int seg = b / q.ngrp, grp = b % q.ngrp;
-  const int x0 = seg * TXR;
-  const int x1 = min(x0 + TXR, g.nx);
-  const int nz = g.nz;
-  const bool zval = z < nz;
-  const bool zI = z >= g.nPml && z <= g.nzA - 1 - g.nPml;
-  const int xI0 = g.nPml, xI1 = g.nx - 1 - g.nPml;
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int k = 0; k < 5 * TXR; ++k) lacc[k * NT + z] = 0.f;
-  auto acc = [&](int term, int j) -> float& { return lacc[(term * TXR + j) * NT + z]; };
-  const int s0 = grp * q.gsz, s1 = min(s0 + q.gsz, g.ns);
-  for (int s = s0; s < s1; ++s) {
-    const long long so = (long long)s * g.N;
-    auto ld = [&](const float* f, int col) -> float {
-      return (zval && col >= 0 && col < g.nx) ? f[so + (long long)col * nz + z] : 0.f;
-    };
-    const int zs = src.zx[2 * s], xs = src.zx[2 * s + 1];
-    // sigma^{it+1} rings, slot k = column c-2+k
-    float zz0 = ld(A.szz, x0 - 4), zz1 = ld(A.szz, x0 - 3), zz2 = ld(A.szz, x0 - 2),
-          zz3 = ld(A.szz, x0 - 1), zz4;
-    float xx0 = ld(A.sxx, x0 - 4), xx1 = ld(A.sxx, x0 - 3), xx2 = ld(A.sxx, x0 - 2),
-          xx3 = ld(A.sxx, x0 - 1), xx4;
-    float xz0 = ld(A.sxz, x0 - 4), xz1 = ld(A.sxz, x0 - 3), xz2 = ld(A.sxz, x0 - 2),
-          xz3 = ld(A.sxz, x0 - 1), xz4;
-    // v^{it} rings: vz at columns c-3..c, vx at c-4..c
-    float wz0 = 0.f, wz1 = 0.f, wz2 = 0.f, wz3 = 0.f;
-    float wx0 = 0.f, wx1 = 0.f, wx2 = 0.f, wx3 = 0.f, wx4 = 0.f;
-#pragma unroll 1
-    for (int j = 0; j < TXR + 4; ++j) {
-      const int col = x0 - 2 + j;
-      // ---- phase 1
-      zz4 = ld(A.szz, col + 2);
-      xx4 = ld(A.sxx, col + 2);
-      xz4 = ld(A.sxz, col + 2);
-      const int sv = (j & 1) * NT;
-      lszz[sv + z] = zz2;
-      lsxz[sv + z] = xz2;
-      __syncthreads();
-      // ---- phase 2: v^{it}(z, col)
-      float nvz = 0.f, nvx = 0.f;
-      const bool jown = j >= 2 && j < TXR + 2;  // col in [x0, x0+TXR)
-      if (zval && col >= 0 && col < g.nx) {
-        const long long i = (long long)col * nz + z;
-        const long long o = so + i;
-        nvz = A.vz[o];
-        nvx = A.vx[o];
-        const bool own = jown && col < x1;
-        const bool inI = zI && col >= xI0 && col <= xI1;
-        const int slot = ring_slot(z, col, g);
-        if (inI) {
-          const float* Szz = lszz + sv + z;
-          const float* Sxz = lsxz + sv + z;
-          const float dszz_dz = (kC1 * (Szz[1] - zz2) - kC2 * (Szz[2] - Szz[-1])) * rdz;
-          const float dsxz_dx = (kC1 * (xz2 - xz1) - kC2 * (xz3 - xz0)) * rdx;
-          const float dsxz_dz = (kC1 * (xz2 - Sxz[-1]) - kC2 * (Sxz[1] - Sxz[-2])) * rdz;
-          const float dsxx_dx = (kC1 * (xx3 - xx2) - kC2 * (xx4 - xx1)) * rdx;
-          const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
-          if (own) {
-            acc(3, j - 2) += adj.vz[o] * fz;
-            acc(4, j - 2) += adj.vx[o] * fx;
-          }
-          if (slot < 0) {
-            nvz -= fz * m.ba[i] * dt;
-            nvx -= fx * m.bb[i] * dt;
-          }
-        }
-        if (slot >= 0) {
-          nvz = *ring_ptr(const_cast<float*>(ring), g, it, 3, s, slot);
-          nvx = *ring_ptr(const_cast<float*>(ring), g, it, 4, s, slot);
-        }
-        if (own) {
-          B.vz[o] = nvz;
-          B.vx[o] = nvx;
-        }
-      }
-      const int sn = (j % 3) * NT;
-      lvz[sn + z] = nvz;
-      lvx[sn + z] = nvx;
-      wz0 = wz1, wz1 = wz2, wz2 = wz3, wz3 = nvz;
-      wx0 = wx1, wx1 = wx2, wx2 = wx3, wx3 = wx4, wx4 = nvx;
-      __syncthreads();
-      // ---- phase 3: sigma^{it}(z, col-2)
-      const int cp = col - 2;
-      if (j >= 4 && zval && cp < x1) {
-        const long long i = (long long)cp * nz + z;
-        const long long o = so + i;
-        float szz = zz0, sxx = xx0, sxz = xz0;
-        const bool inI = zI && cp >= xI0 && cp <= xI1;
-        const int slot = ring_slot(z, cp, g);
-        if (inI) {
-          const int sp = ((j - 2) % 3) * NT + z;
-          const float* Vz = lvz + sp;
-          const float* Vx = lvx + sp;
-          // wz slot k = column col-3+k (cp = slot 1); wx slot k = column col-4+k (cp = slot 2)
-          const float dvz_dz = (kC1 * (wz1 - Vz[-1]) - kC2 * (Vz[1] - Vz[-2])) * rdz;
-          const float dvx_dx = (kC1 * (wx2 - wx1) - kC2 * (wx3 - wx0)) * rdx;
-          const float dvx_dz = (kC1 * (Vx[1] - wx2) - kC2 * (Vx[2] - Vx[-1])) * rdz;
-          const float dvz_dx = (kC1 * (wz2 - wz1) - kC2 * (wz3 - wz0)) * rdx;
-          const float szza = adj.szz[o], sxxa = adj.sxx[o];
-          const int k = j - 4;
-          acc(0, k) += (szza + sxxa) * (dvz_dz + dvx_dx);
-          acc(1, k) += szza * dvz_dz + sxxa * dvx_dx;
-          acc(2, k) += adj.sxz[o] * (dvx_dz + dvz_dx);
-          if (slot < 0) {
-            const float lam = m.lam[i], mu = m.mu[i];
-            const float l2m = lam + 2.f * mu;
-            if (z == zs && cp == xs) {
-              const float inj = src.inj[(size_t)s * g.nSteps + it];
-              szz -= inj;
-              sxx -= 3.f * inj;
-            }
-            szz -= (l2m * dvz_dz + lam * dvx_dx) * dt;
-            sxx -= (lam * dvz_dz + l2m * dvx_dx) * dt;
-            sxz -= m.amu[i] * (dvx_dz + dvz_dx) * dt;
-          }
-        }
-        if (slot >= 0) {
-          szz = *ring_ptr(const_cast<float*>(ring), g, it, 0, s, slot);
-          sxz = *ring_ptr(const_cast<float*>(ring), g, it, 1, s, slot);
-          sxx = *ring_ptr(const_cast<float*>(ring), g, it, 2, s, slot);
-        }
-        B.szz[o] = szz;
-        B.sxx[o] = sxx;
-        B.sxz[o] = sxz;
-      }
-      zz0 = zz1, zz1 = zz2, zz2 = zz3, zz3 = zz4;
-      xx0 = xx1, xx1 = xx2, xx2 = xx3, xx3 = xx4;
-      xz0 = xz1, xz1 = xz2, xz2 = xz3, xz3 = xz4;
-    }
-  }
-  if (zI) {
-    const long long N = g.N;
-    float* P = accp + (size_t)grp * 5 * N;
-    for (int j = 0; j < TXR; ++j) {
-      const int col = x0 + j;
-      if (col < x1 && col >= xI0 && col <= xI1) {
-        const long long i = (long long)col * nz + z;
-        for (int t = 0; t < 5; ++t) P[t * N + i] += acc(t, j);
-      }
-    }
-  }
-}
-
-// Sum the per-group partial accumulators into slice 0.
-__global__ void k_sum_groups(float* __restrict__ accp, long long n5, int ngrp) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n5) return;
-  float t = accp[k];
-  for (int gi = 1; gi < ngrp; ++gi) t += accp[(size_t)gi * n5 + k];
-  accp[k] = t;
-}
-
-// ------------------------------------------------------------ tiled fused forward
-// One time step on a 32x32 output tile, every shot of the group walked inside
-// the block.  Per shot: v^n on the tile + 4-cell halo goes to LDS, sigma^{n+1}
-// is computed on the tile + 2-cell halo (recomputed by neighbours, same bits)
-// into LDS, then v^{n+1} on the tile.  The model parameters of the cells a
-// thread handles stay in registers for all shots.  State and memory variables
-// are read from A and written to B.
-constexpr int TT = 32;
-constexpr int TR4 = TT + 8;  // 40
-constexpr int TR2 = TT + 4;  // 36
-constexpr int NR4 = TR4 * TR4;
-constexpr int NR2 = TR2 * TR2;
-
-__global__ __launch_bounds__(1024) void k_fwd_tile(Tiling tl, Geo g, Mdl m, Pml c, Wave A, Wave B,
-                                                   Mem pA, Mem pB, Src src, int it,
-                                                   float* __restrict__ ring) {
-  __shared__ float svz[NR4], svx[NR4];
-  __shared__ float sszz[NR2], ssxx[NR2], ssxz[NR2];
-  const int t = threadIdx.x;
-  int tz, tx;
-  tile_of(tl, tz, tx);
-  const int z0 = tz * TT, x0 = tx * TT;
-  const int nz = g.nz, nx = g.nx;
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  const int oz = t % TT, ox = t / TT;
-  const int zo = z0 + oz, xo = x0 + ox;
-  const bool updo = zo >= 2 && zo <= g.nzA - 3 && xo >= 2 && xo <= nx - 3;
-  const long long io = (long long)xo * nz + zo;
-  for (int s = 0; s < g.ns; ++s) {
-    const long long so = (long long)s * g.N;
-    // ---- v^n on the tile + 4-cell halo, sigma^n on the 2-cell halo region
-    float lvz0 = 0.f, lvx0 = 0.f, lvz1 = 0.f, lvx1 = 0.f;
-    {
-      const int k = t;
-      const int zz = z0 - 4 + k % TR4, xx = x0 - 4 + k / TR4;
-      if (zz >= 0 && zz < nz && xx >= 0 && xx < nx) {
-        const long long o = so + (long long)xx * nz + zz;
-        lvz0 = A.vz[o];
-        lvx0 = A.vx[o];
-      }
-    }
-    if (t + 1024 < NR4) {
-      const int k = t + 1024;
-      const int zz = z0 - 4 + k % TR4, xx = x0 - 4 + k / TR4;
-      if (zz >= 0 && zz < nz && xx >= 0 && xx < nx) {
-        const long long o = so + (long long)xx * nz + zz;
-        lvz1 = A.vz[o];
-        lvx1 = A.vx[o];
-      }
-    }
-    __syncthreads();  // previous shot's readers are done with the LDS tiles
-    svz[t] = lvz0;
-    svx[t] = lvx0;
-    if (t + 1024 < NR4) {
-      svz[t + 1024] = lvz1;
-      svx[t + 1024] = lvx1;
-    }
-    __syncthreads();
-    // ---- sigma^{n+1} on the tile + 2-cell halo
-    for (int k = t; k < NR2; k += 1024) {
-      const int zz = z0 - 2 + k % TR2, xx = x0 - 2 + k / TR2;
-      float szz = 0.f, sxx = 0.f, sxz = 0.f;
-      if (zz >= 0 && zz < nz && xx >= 0 && xx < nx) {
-        const long long i = (long long)xx * nz + zz;
-        const long long o = so + i;
-        szz = A.szz[o];
-        sxx = A.sxx[o];
-        sxz = A.sxz[o];
-        if (zz >= 2 && zz <= g.nzA - 3 && xx >= 2 && xx <= nx - 3) {
-          const bool own = zz >= z0 && zz < z0 + TT && xx >= x0 && xx < x0 + TT;
-          const int q = ((xx - x0 + 4) * TR4) + (zz - z0 + 4);
-          const float vz0 = svz[q], vx0 = svx[q];
-          float dvz_dz = (kC1 * (vz0 - svz[q - 1]) - kC2 * (svz[q + 1] - svz[q - 2])) * rdz;
-          float dvx_dx = (kC1 * (vx0 - svx[q - TR4]) - kC2 * (svx[q + TR4] - svx[q - 2 * TR4])) * rdx;
-          float dvx_dz = (kC1 * (svx[q + 1] - vx0) - kC2 * (svx[q + 2] - svx[q - 1])) * rdz;
-          float dvz_dx = (kC1 * (svz[q + TR4] - vz0) - kC2 * (svz[q + 2 * TR4] - svz[q - TR4])) * rdx;
-          if (zz < g.nPml || zz > g.nzA - g.nPml - 1) {
-            const float a = c.bz[zz] * pA.vzz[o] + c.az[zz] * dvz_dz;
-            dvz_dz = dvz_dz * (1.f / c.Kz[zz]) + a;
-            const float b = c.bzh[zz] * pA.vxz[o] + c.azh[zz] * dvx_dz;
-            dvx_dz = dvx_dz * (1.f / c.Kzh[zz]) + b;
-            if (own) {
-              pB.vzz[o] = a;
-              pB.vxz[o] = b;
-            }
-          }
-          if (xx < g.nPml || xx > nx - g.nPml - 1) {
-            const float a = c.bx[xx] * pA.vxx[o] + c.ax[xx] * dvx_dx;
-            dvx_dx = dvx_dx * (1.f / c.Kx[xx]) + a;
-            const float b = c.bxh[xx] * pA.vzx[o] + c.axh[xx] * dvz_dx;
-            dvz_dx = dvz_dx * (1.f / c.Kxh[xx]) + b;
-            if (own) {
-              pB.vxx[o] = a;
-              pB.vzx[o] = b;
-            }
-          }
-          if (own && ring) {
-            const int slot = ring_slot(zz, xx, g);
-            if (slot >= 0) {
-              *ring_ptr(ring, g, it, 0, s, slot) = szz;
-              *ring_ptr(ring, g, it, 1, s, slot) = sxz;
-              *ring_ptr(ring, g, it, 2, s, slot) = sxx;
-              *ring_ptr(ring, g, it, 3, s, slot) = vz0;
-              *ring_ptr(ring, g, it, 4, s, slot) = vx0;
-            }
-          }
-          const float lam = m.lam[i], l2m = lam + 2.f * m.mu[i];
-          szz += (l2m * dvz_dz + lam * dvx_dx) * dt;
-          sxx += (lam * dvz_dz + l2m * dvx_dx) * dt;
-          sxz += m.amu[i] * (dvx_dz + dvz_dx) * dt;
-          if (zz == src.zx[2 * s] && xx == src.zx[2 * s + 1]) {
-            const float inj = src.inj[(size_t)s * g.nSteps + it];
-            szz += inj;
-            sxx += 3.f * inj;
-          }
-          if (own) {
-            B.szz[o] = szz;
-            B.sxx[o] = sxx;
-            B.sxz[o] = sxz;
-          }
-        }
-      }
-      sszz[k] = szz;
-      ssxx[k] = sxx;
-      ssxz[k] = sxz;
-    }
-    __syncthreads();
-    // ---- v^{n+1} on the tile
-    if (updo) {
-      const long long o = so + io;
-      const int q = (ox + 2) * TR2 + (oz + 2);
-      const int q4 = (ox + 4) * TR4 + (oz + 4);
-      float dszz_dz = (kC1 * (sszz[q + 1] - sszz[q]) - kC2 * (sszz[q + 2] - sszz[q - 1])) * rdz;
-      float dsxz_dx = (kC1 * (ssxz[q] - ssxz[q - TR2]) - kC2 * (ssxz[q + TR2] - ssxz[q - 2 * TR2])) * rdx;
-      float dsxz_dz = (kC1 * (ssxz[q] - ssxz[q - 1]) - kC2 * (ssxz[q + 1] - ssxz[q - 2])) * rdz;
-      float dsxx_dx = (kC1 * (ssxx[q + TR2] - ssxx[q]) - kC2 * (ssxx[q + 2 * TR2] - ssxx[q - TR2])) * rdx;
-      if (zo < g.nPml || zo > g.nzA - g.nPml - 1) {
-        const float a = c.bzh[zo] * pA.szz[o] + c.azh[zo] * dszz_dz;
-        pB.szz[o] = a;
-        dszz_dz = dszz_dz * (1.f / c.Kzh[zo]) + a;
-        const float b = c.bz[zo] * pA.sxzz[o] + c.az[zo] * dsxz_dz;
-        pB.sxzz[o] = b;
-        dsxz_dz = dsxz_dz * (1.f / c.Kz[zo]) + b;
-      }
-      if (xo < g.nPml || xo > nx - g.nPml) {  // el_velocity.cu:56,71
-        const float a = c.bx[xo] * pA.sxzx[o] + c.ax[xo] * dsxz_dx;
-        pB.sxzx[o] = a;
-        dsxz_dx = dsxz_dx * (1.f / c.Kx[xo]) + a;
-        const float b = c.bxh[xo] * pA.sxx[o] + c.axh[xo] * dsxx_dx;
-        pB.sxx[o] = b;
-        dsxx_dx = dsxx_dx * (1.f / c.Kxh[xo]) + b;
-      }
-      B.vz[o] = svz[q4] + (dszz_dz + dsxz_dx) * m.ba[io] * dt;
-      B.vx[o] = svx[q4] + (dsxz_dz + dsxx_dx) * m.bb[io] * dt;
-    }
-  }
-}
-
-// recording (utilities.cu:557-567): szz + 3 sxx at each receiver, written to
-// row `t` of the time-major seismogram array.
-__device__ inline void record_at(const Geo& g, const float* __restrict__ szz,
-                                 const float* __restrict__ sxx, const int* __restrict__ rec,
-                                 int R, float* __restrict__ row, int r) {
-  if (r >= R) return;
-  const int z = rec[3 * r], x = rec[3 * r + 1], s = rec[3 * r + 2];
+// recording (utilities.cu:557-567): szz + RSXXZZ sxx (a double sum rounded to
+// FP32) at each receiver of the part, written to row it+1 of the seismograms;
+// one receiver per thread of the workgroup.
+__device__ __forceinline__ void record(const Part& P, int blk, int it) {
+  const int per = (int)(blockDim.x * blockDim.y);
+  const int r = blk * per + (int)(threadIdx.y * blockDim.x + threadIdx.x);
+  if (r >= P.nrec) return;
+  const Geo& g = P.g;
+  const int z = P.rec[3 * r], x = P.rec[3 * r + 1], s = P.rec[3 * r + 2] - g.s0;
   const long long o = s * g.N + (long long)x * g.nz + z;
-  row[r] = szz[o] + 3.f * sxx[o];
+  P.syn[(size_t)(it + 1) * g.R + r] = (float)((double)P.w.szz[o] + kRS * (double)P.w.sxx[o]);
 }
 
-__global__ void k_record(Geo g, const float* __restrict__ szz, const float* __restrict__ sxx,
-                         const int* __restrict__ rec, int R, float* __restrict__ row) {
-  record_at(g, szz, sxx, rec, R, row, blockIdx.x * blockDim.x + threadIdx.x);
-}
+// ------------------------------------------------------------------ backward
+// Reverse-time re-materialisation (el_velocity.cu:84-117 with isFor = false)
+// fused with the ring restore of v (to_bnd, utilities.cu:394-424) and the
+// density-gradient accumulation.  The thread domain is the ring rectangle;
+// interior cells update, ring cells take the stored values.
+// acc_a += vz_adj (dszz/dz + dsxz/dx), acc_b += vx_adj (dsxz/dz + dsxx/dx).
+struct RevVelIn {
+  float szzp1, szz0, szzp2, szzm1;
+  float sxz0, sxzmx, sxzpx, sxzm2x, sxzm1, sxzp1, sxzm2;
+  float sxxpx, sxx0, sxxp2x, sxxmx;
+  float avz, avx, vz, vx;
+};
 
-// The velocity update of one step with the recording of that step appended as
-// extra workgroups of the same grid: recording reads the stresses, which the
-// velocity update does not touch, so both run in one launch.  Workgroups
-// [0, nV) update velocities (nV a multiple of 8), the rest record one
-// receiver per thread.
 template <int U>
-__global__ __launch_bounds__(256) void k_vel_fwd_rec(int nV, Tiling tl, Geo g, Mdl m, Pml c, Wave w,
-                                                    Mem p, float* __restrict__ illum, Geo gall,
-                                                    const float* __restrict__ szz,
-                                                    const float* __restrict__ sxx,
-                                                    const int* __restrict__ rec, int R,
-                                                    float* __restrict__ row) {
-  const int b = blockIdx.x;
-  if (b < nV) {
-    if (b < tl.n * tl.chunks) k_vel_fwd_body<U>(b, tl, g, m, c, w, p, illum);
-  } else {
-    const int per = (int)(blockDim.x * blockDim.y);
-    record_at(gall, szz, sxx, rec, R, row,
-              (b - nV) * per + (int)(threadIdx.y * blockDim.x + threadIdx.x));
+__device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
+  const Geo& g = P.g;
+  const int z = (tile % P.gzR) * kBZ + (int)threadIdx.x + (g.nPml - 2);
+  const int x = (tile / P.gzR) * (int)blockDim.y + (int)threadIdx.y + (g.nPml - 2);
+  if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
+  const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
+  const int slot = ring_slot(z, x, g);
+  if (!in && slot < 0) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
+  const Cell C = cell_of(bo, nz);
+  const int Nb = (int)(g.N * 4);  // bytes per shot
+  const Rsrc Ra_vx = rsrc(P.a.vx);
+  const Rsrc Ra_vz = rsrc(P.a.vz);
+  const Rsrc Rw_sxx = rsrc(P.w.sxx);
+  const Rsrc Rw_sxz = rsrc(P.w.sxz);
+  const Rsrc Rw_szz = rsrc(P.w.szz);
+  const Rsrc Rw_vx = rsrc(P.w.vx);
+  const Rsrc Rw_vz = rsrc(P.w.vz);
+  const float ba = P.m.ba[i], bb = P.m.bb[i];
+  const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  const int ns = g.ns;
+  float ga = 0.f, gb = 0.f;
+  for (int s0 = 0; s0 < ns; s0 += U) {
+    RevVelIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= ns) break;
+      const int so = s * Nb;
+      RevVelIn& q = r[u];
+      if (in) {
+        const f4 szc = lz4(Rw_szz, so, C, -1), sxc = lz4(Rw_sxz, so, C, -2);
+        q.szzm1 = szc[0], q.szz0 = szc[1], q.szzp1 = szc[2], q.szzp2 = szc[3];
+        q.sxzm2 = sxc[0], q.sxzm1 = sxc[1], q.sxz0 = sxc[2], q.sxzp1 = sxc[3];
+        q.sxzmx = lx(Rw_sxz, so, C, -1), q.sxzpx = lx(Rw_sxz, so, C, 1), q.sxzm2x = lx(Rw_sxz, so, C, -2);
+        q.sxxpx = lx(Rw_sxx, so, C, 1), q.sxx0 = lz(Rw_sxx, so, C, 0), q.sxxp2x = lx(Rw_sxx, so, C, 2), q.sxxmx = lx(Rw_sxx, so, C, -1);
+        q.avz = lz(Ra_vz, so, C, 0), q.avx = lz(Ra_vx, so, C, 0);
+      }
+      if (slot < 0) {
+        q.vz = lz(Rw_vz, so, C, 0), q.vx = lz(Rw_vx, so, C, 0);
+      } else {
+        q.vz = *ring_ptr(P.ring, g, it, 3, s, slot);
+        q.vx = *ring_ptr(P.ring, g, it, 4, s, slot);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= ns) break;
+      const int so = s * Nb;
+      const RevVelIn& q = r[u];
+      float vz = q.vz, vx = q.vx;
+      if (in) {
+        const float dszz_dz = dfw(q.szzp1, q.szz0, q.szzp2, q.szzm1, dz, rdz);
+        const float dsxz_dx = dfw(q.sxz0, q.sxzmx, q.sxzpx, q.sxzm2x, dx, rdx);
+        const float dsxz_dz = dfw(q.sxz0, q.sxzm1, q.sxzp1, q.sxzm2, dz, rdz);
+        const float dsxx_dx = dfw(q.sxxpx, q.sxx0, q.sxxp2x, q.sxxmx, dx, rdx);
+        const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
+        ga += q.avz * fz;
+        gb += q.avx * fx;
+        if (slot < 0) {
+          vz = vz - fz * ba * dt;
+          vx = vx - fx * bb * dt;
+        }
+      }
+      st(Rw_vz, so, C, vz);
+      st(Rw_vx, so, C, vx);
+    }
+  }
+  if (in) {
+    P.acc[3 * g.N + i] += ga;
+    P.acc[4 * g.N + i] += gb;
   }
 }
 
-// Residual (libCUFD.cu:258-313 without the optional filters): window obs and
-// syn with w^2, res = obs_w - syn_w (first sample 0, gpuMinus utilities.cu:154-167),
-// sum res^2 (cuda_cal_objective, here in FP64), then window res again.
-__global__ __launch_bounds__(256) void k_residual(int nSteps, int R, const float* __restrict__ w2,
+// el_stress.cu:90-131 (isFor = false) preceded by add_source(-) and followed
+// by the ring restore of sigma.  acc_l += (szz_adj + sxx_adj)(dvz/dz + dvx/dx),
+// acc_m += szz_adj dvz/dz + sxx_adj dvx/dx, acc_s += sxz_adj (dvx/dz + dvz/dx).
+struct RevStressIn {
+  float vz0, vzm1, vzp1, vzm2, vzpx, vzp2x, vzmx;
+  float vx0, vxmx, vxpx, vxm2x, vxp1, vxp2, vxm1;
+  float aszz, asxx, asxz;
+  float szz, sxx, sxz;
+  int sz, sx;
+};
+
+template <int U>
+__device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
+  const Geo& g = P.g;
+  const int z = (tile % P.gzR) * kBZ + (int)threadIdx.x + (g.nPml - 2);
+  const int x = (tile / P.gzR) * (int)blockDim.y + (int)threadIdx.y + (g.nPml - 2);
+  if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
+  const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
+  const int slot = ring_slot(z, x, g);
+  if (!in && slot < 0) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
+  const Cell C = cell_of(bo, nz);
+  const int Nb = (int)(g.N * 4);  // bytes per shot
+  const Rsrc Ra_sxx = rsrc(P.a.sxx);
+  const Rsrc Ra_sxz = rsrc(P.a.sxz);
+  const Rsrc Ra_szz = rsrc(P.a.szz);
+  const Rsrc Rw_sxx = rsrc(P.w.sxx);
+  const Rsrc Rw_sxz = rsrc(P.w.sxz);
+  const Rsrc Rw_szz = rsrc(P.w.szz);
+  const Rsrc Rw_vx = rsrc(P.w.vx);
+  const Rsrc Rw_vz = rsrc(P.w.vz);
+  const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i];
+  const double l2m = (double)lam + 2.0 * (double)mu;
+  const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  const double dt64 = (double)dt;
+  const int ns = g.ns;
+  float gl = 0.f, gm = 0.f, gs = 0.f;
+  for (int s0 = 0; s0 < ns; s0 += U) {
+    RevStressIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= ns) break;
+      const int so = s * Nb;
+      RevStressIn& q = r[u];
+      if (in) {
+        const f4 vzc = lz4(Rw_vz, so, C, -2), vxc = lz4(Rw_vx, so, C, -1);
+        q.vzm2 = vzc[0], q.vzm1 = vzc[1], q.vz0 = vzc[2], q.vzp1 = vzc[3];
+        q.vzpx = lx(Rw_vz, so, C, 1), q.vzp2x = lx(Rw_vz, so, C, 2), q.vzmx = lx(Rw_vz, so, C, -1);
+        q.vxm1 = vxc[0], q.vx0 = vxc[1], q.vxp1 = vxc[2], q.vxp2 = vxc[3];
+        q.vxmx = lx(Rw_vx, so, C, -1), q.vxpx = lx(Rw_vx, so, C, 1), q.vxm2x = lx(Rw_vx, so, C, -2);
+        q.aszz = lz(Ra_szz, so, C, 0), q.asxx = lz(Ra_sxx, so, C, 0), q.asxz = lz(Ra_sxz, so, C, 0);
+      }
+      if (slot < 0) {
+        q.szz = lz(Rw_szz, so, C, 0), q.sxx = lz(Rw_sxx, so, C, 0), q.sxz = lz(Rw_sxz, so, C, 0);
+        q.sz = P.src.zx[2 * s], q.sx = P.src.zx[2 * s + 1];
+      } else {
+        q.szz = *ring_ptr(P.ring, g, it, 0, s, slot);
+        q.sxz = *ring_ptr(P.ring, g, it, 1, s, slot);
+        q.sxx = *ring_ptr(P.ring, g, it, 2, s, slot);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= ns) break;
+      const int so = s * Nb;
+      const RevStressIn& q = r[u];
+      float szz = q.szz, sxx = q.sxx, sxz = q.sxz;
+      if (in) {
+        const float dvz_dz = dfw(q.vz0, q.vzm1, q.vzp1, q.vzm2, dz, rdz);
+        const float dvx_dx = dfw(q.vx0, q.vxmx, q.vxpx, q.vxm2x, dx, rdx);
+        const float dvx_dz = dfw(q.vxp1, q.vx0, q.vxp2, q.vxm1, dz, rdz);
+        const float dvz_dx = dfw(q.vzpx, q.vz0, q.vzp2x, q.vzmx, dx, rdx);
+        gl += (q.aszz + q.asxx) * (dvz_dz + dvx_dx);
+        gm += q.aszz * dvz_dz + q.asxx * dvx_dx;
+        gs += q.asxz * (dvx_dz + dvz_dx);
+        if (slot < 0) {
+          if (z == q.sz && x == q.sx) {
+            const float amp = P.src.amp[(size_t)s * g.nSteps + it];
+            szz = szz - kSrcScale * amp * dt;
+            sxx = (float)((double)sxx - kRS * (double)kSrcScale * (double)amp * dt64);
+          }
+          szz = (float)((double)szz - (l2m * (double)dvz_dz + (double)(lam * dvx_dx)) * dt64);
+          sxx = (float)((double)sxx - ((double)(lam * dvz_dz) + l2m * (double)dvx_dx) * dt64);
+          sxz = sxz - amu * (dvx_dz + dvz_dx) * dt;
+        }
+      }
+      st(Rw_szz, so, C, szz);
+      st(Rw_sxx, so, C, sxx);
+      st(Rw_sxz, so, C, sxz);
+    }
+  }
+  if (in) {
+    P.acc[i] += gl;
+    P.acc[g.N + i] += gm;
+    P.acc[2 * g.N + i] += gs;
+  }
+}
+
+// el_velocity_adj.cu:22-108, plus source_grad (utilities.cu:582-593) for step
+// `it` < nSteps-1 (the adjoint stress it reads is not touched by this update).
+// The C-PML memory terms a*dpsi are evaluated only where a != 0; elsewhere
+// they are exactly zero in the reference as well, and adding a zero leaves
+// the sum unchanged.
+struct AdjVelIn {
+  float szz0, szzpx, szzp2x, szzmx, szzp1, szzp2, szzm1;
+  float sxx0, sxxpx, sxxp2x, sxxmx, sxxp1, sxxp2, sxxm1;
+  float sxz0, sxzm1, sxzp1, sxzm2, sxzmx, sxzpx, sxzm2x;
+  float q1[4], q2[4], q3[4], q4[4];  // vxx (x), vxz (z), vzz (z), vzx (x) stencils
+  float m_sxx, m_sxzz, m_sxzx, m_szz;
+  float vx, vz;
+  int sz, sx;
+};
+
+template <int U>
+__device__ __forceinline__ void adj_vel(const Part& P, int tile, int it) {
+  const Geo& g = P.g;
+  const int z = (tile % P.gzU) * kBZ + (int)threadIdx.x;
+  const int x = (tile / P.gzU) * (int)blockDim.y + (int)threadIdx.y;
+  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
+  const Cell C = cell_of(bo, nz);
+  const int Nb = (int)(g.N * 4);  // bytes per shot
+  const Rsrc Ra_sxx = rsrc(P.a.sxx);
+  const Rsrc Ra_sxz = rsrc(P.a.sxz);
+  const Rsrc Ra_szz = rsrc(P.a.szz);
+  const Rsrc Ra_vx = rsrc(P.a.vx);
+  const Rsrc Ra_vz = rsrc(P.a.vz);
+  const Rsrc Rp_sxx = rsrc(P.p.sxx);
+  const Rsrc Rp_sxzx = rsrc(P.p.sxzx);
+  const Rsrc Rp_sxzz = rsrc(P.p.sxzz);
+  const Rsrc Rp_szz = rsrc(P.p.szz);
+  const Rsrc Rp_vxx = rsrc(P.p.vxx);
+  const Rsrc Rp_vxz = rsrc(P.p.vxz);
+  const Rsrc Rp_vzz = rsrc(P.p.vzz);
+  const Rsrc Rp_vzx = rsrc(P.p.vzx);
+  const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i], ba = P.m.ba[i], bb = P.m.bb[i];
+  const double l2m = (double)lam + 2.0 * (double)mu;
+  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
+  const bool px = x < g.nPml || x > g.nx - g.nPml - 1;
+  const float ax = P.c.ax[x], bx = P.c.bx[x], axh = P.c.axh[x], bxh = P.c.bxh[x];
+  const float az = P.c.az[z], bz = P.c.bz[z], azh = P.c.azh[z], bzh = P.c.bzh[z];
+  const Kdiv kx = kdiv_of(P.c.Kx[x]), kz = kdiv_of(P.c.Kz[z]);
+  const float amu_kzh = kdiv(amu, kdiv_of(P.c.Kzh[z]));  // d_ave_Mu / d_K_z_half
+  const float amu_kxh = kdiv(amu, kdiv_of(P.c.Kxh[x]));  // d_ave_Mu / d_K_x_half
+  const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  const double dt64 = (double)dt;
+  const int ns = g.ns;
+  const bool stf = it < g.nSteps - 1 && P.stf_grad;
+  for (int s0 = 0; s0 < ns; s0 += U) {
+    AdjVelIn r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= ns) break;
+      const int so = s * Nb;
+      AdjVelIn& q = r[u];
+      const f4 szc = lz4(Ra_szz, so, C, -1), sxc = lz4(Ra_sxx, so, C, -1), sxzc = lz4(Ra_sxz, so, C, -2);
+      q.szzm1 = szc[0], q.szz0 = szc[1], q.szzp1 = szc[2], q.szzp2 = szc[3];
+      q.szzpx = lx(Ra_szz, so, C, 1), q.szzp2x = lx(Ra_szz, so, C, 2), q.szzmx = lx(Ra_szz, so, C, -1);
+      q.sxxm1 = sxc[0], q.sxx0 = sxc[1], q.sxxp1 = sxc[2], q.sxxp2 = sxc[3];
+      q.sxxpx = lx(Ra_sxx, so, C, 1), q.sxxp2x = lx(Ra_sxx, so, C, 2), q.sxxmx = lx(Ra_sxx, so, C, -1);
+      q.sxzm2 = sxzc[0], q.sxzm1 = sxzc[1], q.sxz0 = sxzc[2], q.sxzp1 = sxzc[3];
+      q.sxzmx = lx(Ra_sxz, so, C, -1), q.sxzpx = lx(Ra_sxz, so, C, 1), q.sxzm2x = lx(Ra_sxz, so, C, -2);
+      if (ax != 0.f) {  // psi_vxx at x+1, x, x+2, x-1
+        q.q1[0] = lx(Rp_vxx, so, C, 1), q.q1[1] = lz(Rp_vxx, so, C, 0), q.q1[2] = lx(Rp_vxx, so, C, 2), q.q1[3] = lx(Rp_vxx, so, C, -1);
+      }
+      if (azh != 0.f) {  // psi_vxz at z, z-1, z+1, z-2
+        const f4 t = lz4(Rp_vxz, so, C, -2);
+        q.q2[0] = t[2], q.q2[1] = t[1], q.q2[2] = t[3], q.q2[3] = t[0];
+      }
+      if (az != 0.f) {  // psi_vzz at z+1, z, z+2, z-1
+        const f4 t = lz4(Rp_vzz, so, C, -1);
+        q.q3[0] = t[2], q.q3[1] = t[1], q.q3[2] = t[3], q.q3[3] = t[0];
+      }
+      if (axh != 0.f) {  // psi_vzx at x, x-1, x+1, x-2
+        q.q4[0] = lz(Rp_vzx, so, C, 0), q.q4[1] = lx(Rp_vzx, so, C, -1), q.q4[2] = lx(Rp_vzx, so, C, 1), q.q4[3] = lx(Rp_vzx, so, C, -2);
+      }
+      if (px) q.m_sxx = lz(Rp_sxx, so, C, 0), q.m_sxzx = lz(Rp_sxzx, so, C, 0);
+      if (pz) q.m_sxzz = lz(Rp_sxzz, so, C, 0), q.m_szz = lz(Rp_szz, so, C, 0);
+      q.vx = lz(Ra_vx, so, C, 0), q.vz = lz(Ra_vz, so, C, 0);
+      q.sz = P.src.zx[2 * s], q.sx = P.src.zx[2 * s + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= ns) break;
+      const int so = s * Nb;
+      const AdjVelIn& q = r[u];
+      if (stf && z == q.sz && x == q.sx)  // -(szz_adj + RSXXZZ sxx_adj) dt, double, stored as float
+        P.stf_grad[(size_t)s * g.nSteps + it] = (float)(-((double)q.szz0 + kRS * (double)q.sxx0) * dt64);
+      // update vx: a_x dpsi_xx/dx + lambda dszz/dx / K_x dt + (lambda + 2.0 mu) dsxx/dx / K_x dt
+      //            + a_z_half dpsi_xz/dz + ave_mu / K_z_half dsxz/dz dt
+      const float dszz_dx = dad(q.szzpx, q.szz0, q.szzp2x, q.szzmx, dx, rdx);
+      const float dsxx_dx = dad(q.sxxpx, q.sxx0, q.sxxp2x, q.sxxmx, dx, rdx);
+      const float dsxz_dz = dad(q.sxz0, q.sxzm1, q.sxzp1, q.sxzm2, dz, rdz);
+      float t = kdiv(lam * dszz_dx, kx) * dt;
+      if (ax != 0.f) t = ax * dad(q.q1[0], q.q1[1], q.q1[2], q.q1[3], dx, rdx) + t;
+      double S = (double)t + kdiv(l2m * (double)dsxx_dx, kx) * dt64;
+      if (azh != 0.f) S = S + (double)(azh * dad(q.q2[0], q.q2[1], q.q2[2], q.q2[3], dz, rdz));
+      S = S + (double)(amu_kzh * dsxz_dz * dt);
+      const float vx = (float)((double)q.vx + S);
+      st(Ra_vx, so, C, vx);
+      if (px) st(Rp_sxx, so, C, bxh * q.m_sxx + bb * vx * dt);
+      if (pz) st(Rp_sxzz, so, C, bz * q.m_sxzz + bb * vx * dt);
+      // update vz: a_z dpsi_zz/dz + (lambda + 2.0 mu) dszz/dz / K_z dt + lambda dsxx/dz / K_z dt
+      //            + a_x_half dpsi_zx/dx + ave_mu / K_x_half dsxz/dx dt
+      const float dszz_dz = dad(q.szzp1, q.szz0, q.szzp2, q.szzm1, dz, rdz);
+      const float dsxx_dz = dad(q.sxxp1, q.sxx0, q.sxxp2, q.sxxm1, dz, rdz);
+      const float dsxz_dx = dad(q.sxz0, q.sxzmx, q.sxzpx, q.sxzm2x, dx, rdx);
+      S = kdiv(l2m * (double)dszz_dz, kz) * dt64;
+      if (az != 0.f) S = (double)(az * dad(q.q3[0], q.q3[1], q.q3[2], q.q3[3], dz, rdz)) + S;
+      S = S + (double)(kdiv(lam * dsxx_dz, kz) * dt);
+      if (axh != 0.f) S = S + (double)(axh * dad(q.q4[0], q.q4[1], q.q4[2], q.q4[3], dx, rdx));
+      S = S + (double)(amu_kxh * dsxz_dx * dt);
+      const float vz = (float)((double)q.vz + S);
+      st(Ra_vz, so, C, vz);
+      if (px) st(Rp_sxzx, so, C, bx * q.m_sxzx + ba * vz * dt);
+      if (pz) st(Rp_szz, so, C, bzh * q.m_szz + ba * vz * dt);
+    }
+  }
+}
+
+// el_stress_adj.cu:22-104.  The 4 stress-side memory variables are updated
+// without PML gates in the reference; they are read only through a*dpsi
+// terms, i.e. where a != 0 (the C-PML layers) plus the 2-cell reach of the
+// stencil, so they are kept in that band (`zlive`/`xlive`) and nowhere else.
+// Residual injection folded into the update (res_injection,
+// utilities.cu:569-580): at a receiver's cell szz_adj += res and sxx_adj +=
+// RSXXZZ res before the update reads them (the update reads only its own
+// cell's stresses, so the sums are those of a separate injection pass).
+struct AdjStressIn {
+  float vz0, vzpx, vzp2x, vzmx, vzm1, vzp1, vzm2;
+  float vx0, vxp1, vxp2, vxm1, vxmx, vxpx, vxm2x;
+  float q1[4], q2[4], q3[4], q4[4];  // sxzx (x), sxzz (z), sxx (x), szz (z) stencils
+  float m_vzx, m_vxz, m_vxx, m_vzz;
+  float sxz, sxx, szz;
+};
+
+template <int U>
+__device__ __forceinline__ void adj_stress(const Part& P, int tile, int it) {
+  const Geo& g = P.g;
+  const int z = (tile % P.gzU) * kBZ + (int)threadIdx.x;
+  const int x = (tile / P.gzU) * (int)blockDim.y + (int)threadIdx.y;
+  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  const int nz = g.nz;
+  const long long i = (long long)x * nz + z;
+  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
+  const Cell C = cell_of(bo, nz);
+  const int Nb = (int)(g.N * 4);  // bytes per shot
+  const Rsrc Ra_sxx = rsrc(P.a.sxx);
+  const Rsrc Ra_sxz = rsrc(P.a.sxz);
+  const Rsrc Ra_szz = rsrc(P.a.szz);
+  const Rsrc Ra_vx = rsrc(P.a.vx);
+  const Rsrc Ra_vz = rsrc(P.a.vz);
+  const Rsrc Rp_sxx = rsrc(P.p.sxx);
+  const Rsrc Rp_sxzx = rsrc(P.p.sxzx);
+  const Rsrc Rp_sxzz = rsrc(P.p.sxzz);
+  const Rsrc Rp_szz = rsrc(P.p.szz);
+  const Rsrc Rp_vxx = rsrc(P.p.vxx);
+  const Rsrc Rp_vxz = rsrc(P.p.vxz);
+  const Rsrc Rp_vzx = rsrc(P.p.vzx);
+  const Rsrc Rp_vzz = rsrc(P.p.vzz);
+  const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i], ba = P.m.ba[i], bb = P.m.bb[i];
+  const double l2m = (double)lam + 2.0 * (double)mu;
+  const bool zlive = z <= g.nPml + 1 || z >= g.nzA - g.nPml - 2;
+  const bool xlive = x <= g.nPml + 1 || x >= g.nx - g.nPml - 2;
+  const float ax = P.c.ax[x], bx = P.c.bx[x], axh = P.c.axh[x], bxh = P.c.bxh[x];
+  const float az = P.c.az[z], bz = P.c.bz[z], azh = P.c.azh[z], bzh = P.c.bzh[z];
+  const Kdiv kx = kdiv_of(P.c.Kx[x]), kz = kdiv_of(P.c.Kz[z]);
+  const Kdiv kxh = kdiv_of(P.c.Kxh[x]), kzh = kdiv_of(P.c.Kzh[z]);
+  const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  const double dt64 = (double)dt;
+  const int ns = g.ns;
+  const bool inj_row = P.inj.res && z == P.inj.z;
+  const float* res_row = P.inj.res ? P.inj.res + (size_t)it * g.R : nullptr;
+  for (int s0 = 0; s0 < ns; s0 += U) {
+    AdjStressIn r[U];
+    float resv[U];
+    bool has[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      resv[u] = 0.f;
+      has[u] = false;
+      if (s >= ns) break;
+      if (inj_row) {
+        const int k = P.inj.at[(size_t)(g.s0 + s) * g.nx + x];
+        if (k >= 0) {
+          resv[u] = res_row[k];
+          has[u] = true;
+        }
+      }
+      const int so = s * Nb;
+      AdjStressIn& q = r[u];
+      const f4 vzc = lz4(Ra_vz, so, C, -2), vxc = lz4(Ra_vx, so, C, -1);
+      q.vzm2 = vzc[0], q.vzm1 = vzc[1], q.vz0 = vzc[2], q.vzp1 = vzc[3];
+      q.vzpx = lx(Ra_vz, so, C, 1), q.vzp2x = lx(Ra_vz, so, C, 2), q.vzmx = lx(Ra_vz, so, C, -1);
+      q.vxm1 = vxc[0], q.vx0 = vxc[1], q.vxp1 = vxc[2], q.vxp2 = vxc[3];
+      q.vxmx = lx(Ra_vx, so, C, -1), q.vxpx = lx(Ra_vx, so, C, 1), q.vxm2x = lx(Ra_vx, so, C, -2);
+      if (ax != 0.f) {  // psi_sxzx at x+1, x, x+2, x-1
+        q.q1[0] = lx(Rp_sxzx, so, C, 1), q.q1[1] = lz(Rp_sxzx, so, C, 0), q.q1[2] = lx(Rp_sxzx, so, C, 2), q.q1[3] = lx(Rp_sxzx, so, C, -1);
+      }
+      if (az != 0.f) {  // psi_sxzz at z+1, z, z+2, z-1
+        const f4 t = lz4(Rp_sxzz, so, C, -1);
+        q.q2[0] = t[2], q.q2[1] = t[1], q.q2[2] = t[3], q.q2[3] = t[0];
+      }
+      if (axh != 0.f) {  // psi_sxx at x, x-1, x+1, x-2
+        q.q3[0] = lz(Rp_sxx, so, C, 0), q.q3[1] = lx(Rp_sxx, so, C, -1), q.q3[2] = lx(Rp_sxx, so, C, 1), q.q3[3] = lx(Rp_sxx, so, C, -2);
+      }
+      if (azh != 0.f) {  // psi_szz at z, z-1, z+1, z-2
+        const f4 t = lz4(Rp_szz, so, C, -2);
+        q.q4[0] = t[2], q.q4[1] = t[1], q.q4[2] = t[3], q.q4[3] = t[0];
+      }
+      if (xlive) q.m_vzx = lz(Rp_vzx, so, C, 0), q.m_vxx = lz(Rp_vxx, so, C, 0);
+      if (zlive) q.m_vxz = lz(Rp_vxz, so, C, 0), q.m_vzz = lz(Rp_vzz, so, C, 0);
+      q.sxz = lz(Ra_sxz, so, C, 0), q.sxx = lz(Ra_sxx, so, C, 0), q.szz = lz(Ra_szz, so, C, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = s0 + u;
+      if (s >= ns) break;
+      const int so = s * Nb;
+      const AdjStressIn& q = r[u];
+      // sxz += a_x dphi_xz_x/dx + dvz/dx / K_x ba dt + a_z dphi_xz_z/dz + dvx/dz / K_z bb dt
+      const float dvz_dx = dad(q.vzpx, q.vz0, q.vzp2x, q.vzmx, dx, rdx);
+      const float dvx_dz = dad(q.vxp1, q.vx0, q.vxp2, q.vxm1, dz, rdz);
+      float e = kdiv(dvz_dx, kx) * ba * dt;
+      if (ax != 0.f) e = ax * dad(q.q1[0], q.q1[1], q.q1[2], q.q1[3], dx, rdx) + e;
+      if (az != 0.f) e = e + az * dad(q.q2[0], q.q2[1], q.q2[2], q.q2[3], dz, rdz);
+      e = e + kdiv(dvx_dz, kz) * bb * dt;
+      const float sxz = q.sxz + e;
+      st(Ra_sxz, so, C, sxz);
+      if (xlive) st(Rp_vzx, so, C, bxh * q.m_vzx + sxz * amu * dt);
+      if (zlive) st(Rp_vxz, so, C, bzh * q.m_vxz + sxz * amu * dt);
+      const float dvx_dx = dad(q.vx0, q.vxmx, q.vxpx, q.vxm2x, dx, rdx);
+      const float dvz_dz = dad(q.vz0, q.vzm1, q.vzp1, q.vzm2, dz, rdz);
+      float sxx0 = q.sxx, szz0 = q.szz;
+      if (has[u]) {
+        szz0 = szz0 + resv[u];
+        sxx0 = (float)((double)sxx0 + kRS * (double)resv[u]);
+      }
+      // sxx += a_x_half dphi_xx/dx + bb dvx/dx / K_x_half dt; szz likewise in z
+      float t = kdiv(bb * dvx_dx, kxh) * dt;
+      if (axh != 0.f) t = axh * dad(q.q3[0], q.q3[1], q.q3[2], q.q3[3], dx, rdx) + t;
+      const float sxx = sxx0 + t;
+      t = kdiv(ba * dvz_dz, kzh) * dt;
+      if (azh != 0.f) t = azh * dad(q.q4[0], q.q4[1], q.q4[2], q.q4[3], dz, rdz) + t;
+      const float szz = szz0 + t;
+      st(Ra_sxx, so, C, sxx);
+      st(Ra_szz, so, C, szz);
+      // psi_xx = b_x psi + lambda szz dt + (lambda + 2.0 mu) sxx dt;
+      // psi_zz = b_z psi + (lambda + 2.0 mu) szz dt + lambda sxx dt
+      if (xlive)
+        st(Rp_vxx, so, C, (float)((double)(bx * q.m_vxx + lam * szz * dt) + l2m * (double)sxx * dt64));
+      if (zlive)
+        st(Rp_vzz, so, C, (float)((double)(bz * q.m_vzz) + l2m * (double)szz * dt64 + (double)(lam * sxx * dt)));
+    }
+  }
+}
+
+// ------------------------------------------------------------ step kernels
+template <int NP>
+__device__ __forceinline__ const Part& part_of(const Parts<NP>& ps, unsigned e) {
+  return ps.get((int)((e >> 24) & 31u));
+}
+
+template <int NP, int U>
+__global__ __launch_bounds__(256) void k_fwd_stress(Parts<NP> ps, const unsigned* __restrict__ map,
+                                                    int it) {
+  const unsigned e = map[blockIdx.x];
+  if ((e >> 29) == kNop) return;
+  stress_fwd<U>(part_of(ps, e), (int)(e & 0xffffffu), it);
+}
+
+// the velocity update of step it and the recording of the stresses it read
+// (recording reads only stresses, which the velocity update does not touch)
+template <int NP, int U>
+__global__ __launch_bounds__(256) void k_fwd_vel(Parts<NP> ps, const unsigned* __restrict__ map,
+                                                 int it) {
+  const unsigned e = map[blockIdx.x];
+  const unsigned kind = e >> 29;
+  if (kind == kVelFwd)
+    vel_fwd<U>(part_of(ps, e), (int)(e & 0xffffffu));
+  else if (kind == kRecord)
+    record(part_of(ps, e), (int)(e & 0xffffffu), it);
+}
+
+// Backward pairs with disjoint reads and writes share a launch:
+//   a: rev_stress(it) + adj_vel(it)   (rev_stress reads the adjoint stress that
+//      adj_vel also only reads; adj_vel writes the adjoint velocity that
+//      rev_stress does not touch)
+//   b: adj_stress(it) + rev_vel(it-1) (rev_vel reads the adjoint velocity that
+//      adj_stress also only reads; adj_stress writes the adjoint stress that
+//      rev_vel does not touch)
+template <int NP, int U>
+__global__ __launch_bounds__(256) void k_bwd_a(Parts<NP> ps, const unsigned* __restrict__ map,
+                                               int it) {
+  const unsigned e = map[blockIdx.x];
+  const unsigned kind = e >> 29;
+  if (kind == kRevStress)
+    rev_stress<U>(part_of(ps, e), (int)(e & 0xffffffu), it);
+  else if (kind == kAdjVel)
+    adj_vel<U>(part_of(ps, e), (int)(e & 0xffffffu), it);
+}
+
+template <int NP, int U>
+__global__ __launch_bounds__(256) void k_bwd_b(Parts<NP> ps, const unsigned* __restrict__ map,
+                                               int it) {
+  const unsigned e = map[blockIdx.x];
+  const unsigned kind = e >> 29;
+  if (kind == kAdjStress)
+    adj_stress<U>(part_of(ps, e), (int)(e & 0xffffffu), it);
+  else if (kind == kRevVel)
+    rev_vel<U>(part_of(ps, e), (int)(e & 0xffffffu), it - 1);
+}
+
+// res_injection (utilities.cu:569-580) for surveys whose receivers are not one
+// per cell in one row: one thread per injection cell adds its receivers'
+// residuals in receiver order (szz_adj += res, sxx_adj += RSXXZZ res), the
+// order in which the reference's per-receiver += lands on a shared cell when
+// its threads run in index order.  cells[c] = (plan shot, z, x); receivers
+// of cell c are rid[ptr[c] .. ptr[c+1]).
+__global__ void k_res_inject(Geo g, Wave a_all, const int* __restrict__ cells,
+                             const int* __restrict__ ptr, const int* __restrict__ rid, int c0,
+                             int ncells, const float* __restrict__ row) {
+  const int c = c0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= c0 + ncells) return;
+  const long long o = cells[3 * c] * g.N + (long long)cells[3 * c + 2] * g.nz + cells[3 * c + 1];
+  float szz = a_all.szz[o], sxx = a_all.sxx[o];
+  for (int j = ptr[c]; j < ptr[c + 1]; ++j) {
+    const float v = row[rid[j]];
+    szz = szz + v;
+    sxx = (float)((double)sxx + kRS * (double)v);
+  }
+  a_all.szz[o] = szz;
+  a_all.sxx[o] = sxx;
+}
+
+// Residual (libCUFD.cu:258-313 without the optional filters) over receivers
+// [r0, r0 + nr) of the plan: window obs and syn with w^2, res = obs_w - syn_w
+// (first sample 0, gpuMinus utilities.cu:154-167), sum res^2
+// (cuda_cal_objective, here in FP64), then window res again.
+__global__ __launch_bounds__(256) void k_residual(int nSteps, int R, int r0, int nr,
+                                                  const float* __restrict__ w2,
                                                   const float* __restrict__ obs,
                                                   const float* __restrict__ syn,
                                                   float* __restrict__ res,
                                                   double* __restrict__ partial) {
   __shared__ double red[256];
-  const size_t total = (size_t)nSteps * R;
+  const size_t total = (size_t)nSteps * nr;
   double acc = 0.0;
   for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
        k += (size_t)gridDim.x * blockDim.x) {
-    const int t = (int)(k / (size_t)R);
-    float rv;
-    {
-      // obs and syn are windowed separately (two roundings, as the reference's
-      // separate cuda_window launches do); an FMA contraction of
-      // obs*ww - syn*ww would leave a rounding residue when obs == syn.
-#pragma clang fp contract(off)
-      const float ww = w2[t];
-      const float ow = obs[k] * ww;
-      const float sw = syn[k] * ww;
-      rv = t > 0 ? ow - sw : 0.f;
-    }
+    const int t = (int)(k / (size_t)nr);
+    const size_t j = (size_t)t * R + r0 + (k - (size_t)t * nr);
     const float ww = w2[t];
+    // obs and syn are windowed separately (two roundings, as the reference's
+    // separate cuda_window launches do)
+    const float ow = obs[j] * ww;
+    const float sw = syn[j] * ww;
+    const float rv = t > 0 ? ow - sw : 0.f;
     acc += (double)rv * (double)rv;
-    res[k] = rv * ww;
+    res[j] = rv * ww;
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -1190,566 +1132,15 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
   if (threadIdx.x == 0) *out = red[0];
 }
 
-// ------------------------------------------------------------------ backward
-// Reverse-time re-materialisation (el_velocity.cu:84-117 with isFor = false)
-// fused with the ring restore of v (to_bnd, utilities.cu:394-424) and the
-// density-gradient accumulation.  The thread domain is the ring rectangle;
-// interior cells update, ring cells take the stored values.
-// acc_a += vz_adj (dszz/dz + dsxz/dx), acc_b += vx_adj (dsxz/dz + dsxx/dx).
-// Shots are batched U at a time as in the forward kernels.
-struct RevVelIn {
-  float szzp1, szz0, szzp2, szzm1;
-  float sxz0, sxzmx, sxzpx, sxzm2x, sxzm1, sxzp1, sxzm2;
-  float sxxpx, sxx0, sxxp2x, sxxmx;
-  float avz, avx, vz, vx;
-};
-
-template <int U>
-__device__ __forceinline__ void k_rev_vel_body(int bid, Tiling tl, Geo g, Mdl m, Wave w, Wave a,
-                                                 float* __restrict__ acc_a,
-                                                 float* __restrict__ acc_b,
-                                                 const float* __restrict__ ring, int it) {
-  int tz, tx;
-  tile_of(tl, bid, tz, tx);
-  const int z = tz * kBZ + threadIdx.x + (g.nPml - 2);
-  const int x = tx * (int)blockDim.y + threadIdx.y + (g.nPml - 2);
-  if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
-  const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
-  const int slot = ring_slot(z, x, g);
-  if (!in && slot < 0) return;
-  const int nz = g.nz;
-  const long long i = (long long)x * nz + z;
-  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
-  const float ba = m.ba[i], bb = m.bb[i];
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  float ga = 0.f, gb = 0.f;
-  for (int s0 = 0; s0 < g.ns; s0 += U) {
-    RevVelIn r[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      if (s >= g.ns) break;
-      const long long so = s * g.N;
-      RevVelIn& q = r[u];
-      if (in) {
-        const float* szz = w.szz + so;
-        const float* sxx = w.sxx + so;
-        const float* sxz = w.sxz + so;
-        q.szzp1 = ldo(szz + 1, bo), q.szz0 = ldo(szz, bo), q.szzp2 = ldo(szz + 2, bo), q.szzm1 = ldo(szz - 1, bo);
-        q.sxz0 = ldo(sxz, bo), q.sxzmx = ldo(sxz - nz, bo), q.sxzpx = ldo(sxz + nz, bo), q.sxzm2x = ldo(sxz - 2 * nz, bo);
-        q.sxzm1 = ldo(sxz - 1, bo), q.sxzp1 = ldo(sxz + 1, bo), q.sxzm2 = ldo(sxz - 2, bo);
-        q.sxxpx = ldo(sxx + nz, bo), q.sxx0 = ldo(sxx, bo), q.sxxp2x = ldo(sxx + 2 * nz, bo), q.sxxmx = ldo(sxx - nz, bo);
-        q.avz = ldo(a.vz + so, bo), q.avx = ldo(a.vx + so, bo);
-      }
-      if (slot < 0) {
-        q.vz = ldo(w.vz + so, bo), q.vx = ldo(w.vx + so, bo);
-      } else {
-        q.vz = *ring_ptr(const_cast<float*>(ring), g, it, 3, s, slot);
-        q.vx = *ring_ptr(const_cast<float*>(ring), g, it, 4, s, slot);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      if (s >= g.ns) break;
-      const long long so = s * g.N;
-      const RevVelIn& q = r[u];
-      float vz = q.vz, vx = q.vx;
-      if (in) {
-        const float dszz_dz = (kC1 * (q.szzp1 - q.szz0) - kC2 * (q.szzp2 - q.szzm1)) * rdz;
-        const float dsxz_dx = (kC1 * (q.sxz0 - q.sxzmx) - kC2 * (q.sxzpx - q.sxzm2x)) * rdx;
-        const float dsxz_dz = (kC1 * (q.sxz0 - q.sxzm1) - kC2 * (q.sxzp1 - q.sxzm2)) * rdz;
-        const float dsxx_dx = (kC1 * (q.sxxpx - q.sxx0) - kC2 * (q.sxxp2x - q.sxxmx)) * rdx;
-        const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
-        ga += q.avz * fz;
-        gb += q.avx * fx;
-        if (slot < 0) {
-          vz -= fz * ba * dt;
-          vx -= fx * bb * dt;
-        }
-      }
-      sto(w.vz + so, bo, vz);
-      sto(w.vx + so, bo, vx);
-    }
-  }
-  if (in) {
-    acc_a[i] += ga;
-    acc_b[i] += gb;
-  }
-}
-
-template <int U>
-__global__ __launch_bounds__(256) void k_rev_vel(Tiling tl, Geo g, Mdl m, Wave w, Wave a,
-                                                 float* __restrict__ acc_a,
-                                                 float* __restrict__ acc_b,
-                                                 const float* __restrict__ ring, int it) {
-  k_rev_vel_body<U>(blockIdx.x, tl, g, m, w, a, acc_a, acc_b, ring, it);
-}
-
-// el_stress.cu:90-131 (isFor = false) fused with add_source(-) and the ring
-// restore of sigma.  acc_l += (szz_adj + sxx_adj)(dvz/dz + dvx/dx),
-// acc_m += szz_adj dvz/dz + sxx_adj dvx/dx, acc_s += sxz_adj (dvx/dz + dvz/dx).
-struct RevStressIn {
-  float vz0, vzm1, vzp1, vzm2, vzpx, vzp2x, vzmx;
-  float vx0, vxmx, vxpx, vxm2x, vxp1, vxp2, vxm1;
-  float aszz, asxx, asxz;
-  float szz, sxx, sxz;
-  int sz, sx;
-};
-
-template <int U>
-__device__ __forceinline__ void k_rev_stress_body(int bid, Tiling tl, Geo g, Mdl m, Wave w, Wave a, Src src,
-                                                    float* __restrict__ acc_l,
-                                                    float* __restrict__ acc_m,
-                                                    float* __restrict__ acc_s,
-                                                    const float* __restrict__ ring, int it) {
-  int tz, tx;
-  tile_of(tl, bid, tz, tx);
-  const int z = tz * kBZ + threadIdx.x + (g.nPml - 2);
-  const int x = tx * (int)blockDim.y + threadIdx.y + (g.nPml - 2);
-  if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
-  const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
-  const int slot = ring_slot(z, x, g);
-  if (!in && slot < 0) return;
-  const int nz = g.nz;
-  const long long i = (long long)x * nz + z;
-  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
-  const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i];
-  const float l2m = lam + 2.f * mu;
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  float gl = 0.f, gm = 0.f, gs = 0.f;
-  for (int s0 = 0; s0 < g.ns; s0 += U) {
-    RevStressIn r[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      if (s >= g.ns) break;
-      const long long so = s * g.N;
-      RevStressIn& q = r[u];
-      if (in) {
-        const float* vz = w.vz + so;
-        const float* vx = w.vx + so;
-        q.vz0 = ldo(vz, bo), q.vzm1 = ldo(vz - 1, bo), q.vzp1 = ldo(vz + 1, bo), q.vzm2 = ldo(vz - 2, bo);
-        q.vzpx = ldo(vz + nz, bo), q.vzp2x = ldo(vz + 2 * nz, bo), q.vzmx = ldo(vz - nz, bo);
-        q.vx0 = ldo(vx, bo), q.vxmx = ldo(vx - nz, bo), q.vxpx = ldo(vx + nz, bo), q.vxm2x = ldo(vx - 2 * nz, bo);
-        q.vxp1 = ldo(vx + 1, bo), q.vxp2 = ldo(vx + 2, bo), q.vxm1 = ldo(vx - 1, bo);
-        q.aszz = ldo(a.szz + so, bo), q.asxx = ldo(a.sxx + so, bo), q.asxz = ldo(a.sxz + so, bo);
-      }
-      if (slot < 0) {
-        q.szz = ldo(w.szz + so, bo), q.sxx = ldo(w.sxx + so, bo), q.sxz = ldo(w.sxz + so, bo);
-        q.sz = src.zx[2 * s], q.sx = src.zx[2 * s + 1];
-      } else {
-        q.szz = *ring_ptr(const_cast<float*>(ring), g, it, 0, s, slot);
-        q.sxz = *ring_ptr(const_cast<float*>(ring), g, it, 1, s, slot);
-        q.sxx = *ring_ptr(const_cast<float*>(ring), g, it, 2, s, slot);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      if (s >= g.ns) break;
-      const long long so = s * g.N;
-      const RevStressIn& q = r[u];
-      float szz = q.szz, sxx = q.sxx, sxz = q.sxz;
-      if (in) {
-        const float dvz_dz = (kC1 * (q.vz0 - q.vzm1) - kC2 * (q.vzp1 - q.vzm2)) * rdz;
-        const float dvx_dx = (kC1 * (q.vx0 - q.vxmx) - kC2 * (q.vxpx - q.vxm2x)) * rdx;
-        const float dvx_dz = (kC1 * (q.vxp1 - q.vx0) - kC2 * (q.vxp2 - q.vxm1)) * rdz;
-        const float dvz_dx = (kC1 * (q.vzpx - q.vz0) - kC2 * (q.vzp2x - q.vzmx)) * rdx;
-        gl += (q.aszz + q.asxx) * (dvz_dz + dvx_dx);
-        gm += q.aszz * dvz_dz + q.asxx * dvx_dx;
-        gs += q.asxz * (dvx_dz + dvz_dx);
-        if (slot < 0) {
-          if (z == q.sz && x == q.sx) {
-            const float inj = src.inj[(size_t)s * g.nSteps + it];
-            szz -= inj;
-            sxx -= 3.f * inj;
-          }
-          szz = szz - (l2m * dvz_dz + lam * dvx_dx) * dt;
-          sxx = sxx - (lam * dvz_dz + l2m * dvx_dx) * dt;
-          sxz -= amu * (dvx_dz + dvz_dx) * dt;
-        }
-      }
-      sto(w.szz + so, bo, szz);
-      sto(w.sxx + so, bo, sxx);
-      sto(w.sxz + so, bo, sxz);
-    }
-  }
-  if (in) {
-    acc_l[i] += gl;
-    acc_m[i] += gm;
-    acc_s[i] += gs;
-  }
-}
-
-template <int U>
-__global__ __launch_bounds__(256) void k_rev_stress(Tiling tl, Geo g, Mdl m, Wave w, Wave a, Src src,
-                                                    float* __restrict__ acc_l,
-                                                    float* __restrict__ acc_m,
-                                                    float* __restrict__ acc_s,
-                                                    const float* __restrict__ ring, int it) {
-  k_rev_stress_body<U>(blockIdx.x, tl, g, m, w, a, src, acc_l, acc_m, acc_s, ring, it);
-}
-
-// el_velocity_adj.cu:22-108, plus source_grad (utilities.cu:582-593) for step
-// `it` (the adjoint stress it reads is not touched by this kernel).  The
-// C-PML memory terms a*dpsi are evaluated only where a != 0; elsewhere they
-// are exactly zero in the reference as well.
-struct AdjVelIn {
-  float szz0, szzpx, szzp2x, szzmx, szzp1, szzp2, szzm1;
-  float sxx0, sxxpx, sxxp2x, sxxmx, sxxp1, sxxp2, sxxm1;
-  float sxz0, sxzm1, sxzp1, sxzm2, sxzmx, sxzpx, sxzm2x;
-  float q1[4], q2[4], q3[4], q4[4];  // vxx (x), vxz (z), vzz (z), vzx (x) stencils
-  float m_sxx, m_sxzz, m_sxzx, m_szz;
-  float vx, vz;
-  int sz, sx;
-};
-
-template <int U>
-__device__ __forceinline__ void k_adj_vel_body(int bid, Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
-                                                 float* __restrict__ stf_grad, int it) {
-  int tz, tx;
-  const int chunk = tile_of(tl, bid, tz, tx);
-  const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
-  const int z = tz * kBZ + threadIdx.x;
-  const int x = tx * (int)blockDim.y + threadIdx.y;
-  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
-  const int nz = g.nz;
-  const long long i = (long long)x * nz + z;
-  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
-  const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
-  const float l2m = lam + 2.f * mu;
-  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
-  const bool px = x < g.nPml || x > g.nx - g.nPml - 1;
-  const float ax = c.ax[x], rkx = 1.f / c.Kx[x], bx = c.bx[x];
-  const float axh = c.axh[x], rkxh = 1.f / c.Kxh[x], bxh = c.bxh[x];
-  const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
-  const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  for (int s0 = sb; s0 < se; s0 += U) {
-    AdjVelIn r[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      if (s >= se) break;
-      const long long so = s * g.N;
-      const float* szz = a.szz + so;
-      const float* sxx = a.sxx + so;
-      const float* sxz = a.sxz + so;
-      AdjVelIn& q = r[u];
-      q.szz0 = ldo(szz, bo), q.szzpx = ldo(szz + nz, bo), q.szzp2x = ldo(szz + 2 * nz, bo), q.szzmx = ldo(szz - nz, bo);
-      q.szzp1 = ldo(szz + 1, bo), q.szzp2 = ldo(szz + 2, bo), q.szzm1 = ldo(szz - 1, bo);
-      q.sxx0 = ldo(sxx, bo), q.sxxpx = ldo(sxx + nz, bo), q.sxxp2x = ldo(sxx + 2 * nz, bo), q.sxxmx = ldo(sxx - nz, bo);
-      q.sxxp1 = ldo(sxx + 1, bo), q.sxxp2 = ldo(sxx + 2, bo), q.sxxm1 = ldo(sxx - 1, bo);
-      q.sxz0 = ldo(sxz, bo), q.sxzm1 = ldo(sxz - 1, bo), q.sxzp1 = ldo(sxz + 1, bo), q.sxzm2 = ldo(sxz - 2, bo);
-      q.sxzmx = ldo(sxz - nz, bo), q.sxzpx = ldo(sxz + nz, bo), q.sxzm2x = ldo(sxz - 2 * nz, bo);
-      if (ax != 0.f) {
-        const float* v = p.vxx + so;
-        q.q1[0] = ldo(v + nz, bo), q.q1[1] = ldo(v, bo), q.q1[2] = ldo(v + 2 * nz, bo), q.q1[3] = ldo(v - nz, bo);
-      }
-      if (azh != 0.f) {
-        const float* v = p.vxz + so;
-        q.q2[0] = ldo(v, bo), q.q2[1] = ldo(v - 1, bo), q.q2[2] = ldo(v + 1, bo), q.q2[3] = ldo(v - 2, bo);
-      }
-      if (az != 0.f) {
-        const float* v = p.vzz + so;
-        q.q3[0] = ldo(v + 1, bo), q.q3[1] = ldo(v, bo), q.q3[2] = ldo(v + 2, bo), q.q3[3] = ldo(v - 1, bo);
-      }
-      if (axh != 0.f) {
-        const float* v = p.vzx + so;
-        q.q4[0] = ldo(v, bo), q.q4[1] = ldo(v - nz, bo), q.q4[2] = ldo(v + nz, bo), q.q4[3] = ldo(v - 2 * nz, bo);
-      }
-      if (px) q.m_sxx = ldo(p.sxx + so, bo), q.m_sxzx = ldo(p.sxzx + so, bo);
-      if (pz) q.m_sxzz = ldo(p.sxzz + so, bo), q.m_szz = ldo(p.szz + so, bo);
-      q.vx = ldo(a.vx + so, bo), q.vz = ldo(a.vz + so, bo);
-      q.sz = src.zx[2 * s], q.sx = src.zx[2 * s + 1];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      if (s >= se) break;
-      const long long so = s * g.N;
-      const AdjVelIn& q = r[u];
-      if (it >= 0 && z == q.sz && x == q.sx)
-        stf_grad[(size_t)s * g.nSteps + it] = -(q.szz0 + 3.f * q.sxx0) * dt;
-      // update vx
-      const float dszz_dx = (-kC1 * (q.szzpx - q.szz0) + kC2 * (q.szzp2x - q.szzmx)) * rdx;
-      const float dsxx_dx = (-kC1 * (q.sxxpx - q.sxx0) + kC2 * (q.sxxp2x - q.sxxmx)) * rdx;
-      const float dsxz_dz = (-kC1 * (q.sxz0 - q.sxzm1) + kC2 * (q.sxzp1 - q.sxzm2)) * rdz;
-      float vx = q.vx + (lam * dszz_dx * rkx * dt + l2m * dsxx_dx * rkx * dt +
-                         amu * rkzh * dsxz_dz * dt);
-      if (ax != 0.f) vx += ax * ((-kC1 * (q.q1[0] - q.q1[1]) + kC2 * (q.q1[2] - q.q1[3])) * rdx);
-      if (azh != 0.f) vx += azh * ((-kC1 * (q.q2[0] - q.q2[1]) + kC2 * (q.q2[2] - q.q2[3])) * rdz);
-      sto(a.vx + so, bo, vx);
-      if (px) sto(p.sxx + so, bo, bxh * q.m_sxx + bb * vx * dt);
-      if (pz) sto(p.sxzz + so, bo, bz * q.m_sxzz + bb * vx * dt);
-      // update vz
-      const float dszz_dz = (-kC1 * (q.szzp1 - q.szz0) + kC2 * (q.szzp2 - q.szzm1)) * rdz;
-      const float dsxx_dz = (-kC1 * (q.sxxp1 - q.sxx0) + kC2 * (q.sxxp2 - q.sxxm1)) * rdz;
-      const float dsxz_dx = (-kC1 * (q.sxz0 - q.sxzmx) + kC2 * (q.sxzpx - q.sxzm2x)) * rdx;
-      float vz = q.vz + (l2m * dszz_dz * rkz * dt + lam * dsxx_dz * rkz * dt +
-                         amu * rkxh * dsxz_dx * dt);
-      if (az != 0.f) vz += az * ((-kC1 * (q.q3[0] - q.q3[1]) + kC2 * (q.q3[2] - q.q3[3])) * rdz);
-      if (axh != 0.f) vz += axh * ((-kC1 * (q.q4[0] - q.q4[1]) + kC2 * (q.q4[2] - q.q4[3])) * rdx);
-      sto(a.vz + so, bo, vz);
-      if (px) sto(p.sxzx + so, bo, bx * q.m_sxzx + ba * vz * dt);
-      if (pz) sto(p.szz + so, bo, bzh * q.m_szz + ba * vz * dt);
-    }
-  }
-}
-
-template <int U>
-__global__ __launch_bounds__(256) void k_adj_vel(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p, Src src,
-                                                 float* __restrict__ stf_grad, int it) {
-  k_adj_vel_body<U>(blockIdx.x, tl, g, m, c, a, p, src, stf_grad, it);
-}
-
-// res_injection (utilities.cu:569-580): szz_adj += res, sxx_adj += 3 res.
-__global__ void k_res_inject(Geo g, float* __restrict__ szz, float* __restrict__ sxx,
-                             const int* __restrict__ rec, int R, const float* __restrict__ row) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R) return;
-  const int z = rec[3 * r], x = rec[3 * r + 1], s = rec[3 * r + 2];
-  const long long o = s * g.N + (long long)x * g.nz + z;
-  const float v = row[r];
-  atomicAdd(szz + o, v);
-  atomicAdd(sxx + o, 3.f * v);
-}
-
-// el_stress_adj.cu:22-104.  The 4 stress-side memory variables are updated
-// without PML gates in the reference; they are read only through a*dpsi terms,
-// i.e. where a != 0 (the C-PML layers) plus the 2-cell reach of the stencil,
-// so they are kept in that band (`zlive`/`xlive`) and nowhere else.
-// Residual injection folded into the adjoint stress update (res_injection,
-// utilities.cu:569-580): at a receiver's cell szz_adj += res and sxx_adj +=
-// 3 res before the update reads them (the update reads only its own cell's
-// stresses, so the sum is the same as a separate injection pass).  Used when
-// every receiver lies in one row z and no two receivers of a shot share a
-// cell: at[shot][x] is then the receiver of column x (index into the
-// residual row) or -1.
-struct Inj {
-  const float* row;  // residual row of the step, all receivers; null: no injection
-  const int* at;     // [ns][nx]
-  int z;
-};
-
-struct AdjStressIn {
-  float vz0, vzpx, vzp2x, vzmx, vzm1, vzp1, vzm2;
-  float vx0, vxp1, vxp2, vxm1, vxmx, vxpx, vxm2x;
-  float q1[4], q2[4], q3[4], q4[4];  // sxzx (x), sxzz (z), sxx (x), szz (z) stencils
-  float m_vzx, m_vxz, m_vxx, m_vzz;
-  float sxz, sxx, szz;
-};
-
-template <int U>
-__device__ __forceinline__ void k_adj_stress_body(int bid, Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p,
-                                                  Inj inj) {
-  int tz, tx;
-  const int chunk = tile_of(tl, bid, tz, tx);
-  const int sb = chunk * tl.spt, se = min(g.ns, sb + tl.spt);  // shots of this workgroup
-  const int z = tz * kBZ + threadIdx.x;
-  const int x = tx * (int)blockDim.y + threadIdx.y;
-  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
-  const int nz = g.nz;
-  const long long i = (long long)x * nz + z;
-  const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
-  const float lam = m.lam[i], mu = m.mu[i], amu = m.amu[i], ba = m.ba[i], bb = m.bb[i];
-  const float l2m = lam + 2.f * mu;
-  const bool zlive = z <= g.nPml + 1 || z >= g.nzA - g.nPml - 2;
-  const bool xlive = x <= g.nPml + 1 || x >= g.nx - g.nPml - 2;
-  const float ax = c.ax[x], rkx = 1.f / c.Kx[x], bx = c.bx[x];
-  const float axh = c.axh[x], rkxh = 1.f / c.Kxh[x], bxh = c.bxh[x];
-  const float az = c.az[z], rkz = 1.f / c.Kz[z], bz = c.bz[z];
-  const float azh = c.azh[z], rkzh = 1.f / c.Kzh[z], bzh = c.bzh[z];
-  const float dt = g.dt, rdz = g.rdz, rdx = g.rdx;
-  const bool inj_row = inj.row && z == inj.z;
-  for (int s0 = sb; s0 < se; s0 += U) {
-    AdjStressIn r[U];
-    float resv[U];
-    bool has[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      resv[u] = 0.f;
-      has[u] = false;
-      if (s >= se) break;
-      if (inj_row) {
-        const int k = inj.at[(size_t)(g.s0 + s) * g.nx + x];
-        if (k >= 0) {
-          resv[u] = inj.row[k];
-          has[u] = true;
-        }
-      }
-      const long long so = s * g.N;
-      const float* vz = a.vz + so;
-      const float* vx = a.vx + so;
-      AdjStressIn& q = r[u];
-      q.vz0 = ldo(vz, bo), q.vzpx = ldo(vz + nz, bo), q.vzp2x = ldo(vz + 2 * nz, bo), q.vzmx = ldo(vz - nz, bo);
-      q.vzm1 = ldo(vz - 1, bo), q.vzp1 = ldo(vz + 1, bo), q.vzm2 = ldo(vz - 2, bo);
-      q.vx0 = ldo(vx, bo), q.vxp1 = ldo(vx + 1, bo), q.vxp2 = ldo(vx + 2, bo), q.vxm1 = ldo(vx - 1, bo);
-      q.vxmx = ldo(vx - nz, bo), q.vxpx = ldo(vx + nz, bo), q.vxm2x = ldo(vx - 2 * nz, bo);
-      if (ax != 0.f) {
-        const float* v = p.sxzx + so;
-        q.q1[0] = ldo(v + nz, bo), q.q1[1] = ldo(v, bo), q.q1[2] = ldo(v + 2 * nz, bo), q.q1[3] = ldo(v - nz, bo);
-      }
-      if (az != 0.f) {
-        const float* v = p.sxzz + so;
-        q.q2[0] = ldo(v + 1, bo), q.q2[1] = ldo(v, bo), q.q2[2] = ldo(v + 2, bo), q.q2[3] = ldo(v - 1, bo);
-      }
-      if (axh != 0.f) {
-        const float* v = p.sxx + so;
-        q.q3[0] = ldo(v, bo), q.q3[1] = ldo(v - nz, bo), q.q3[2] = ldo(v + nz, bo), q.q3[3] = ldo(v - 2 * nz, bo);
-      }
-      if (azh != 0.f) {
-        const float* v = p.szz + so;
-        q.q4[0] = ldo(v, bo), q.q4[1] = ldo(v - 1, bo), q.q4[2] = ldo(v + 1, bo), q.q4[3] = ldo(v - 2, bo);
-      }
-      if (xlive) q.m_vzx = ldo(p.vzx + so, bo), q.m_vxx = ldo(p.vxx + so, bo);
-      if (zlive) q.m_vxz = ldo(p.vxz + so, bo), q.m_vzz = ldo(p.vzz + so, bo);
-      q.sxz = ldo(a.sxz + so, bo), q.sxx = ldo(a.sxx + so, bo), q.szz = ldo(a.szz + so, bo);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int s = s0 + u;
-      if (s >= se) break;
-      const long long so = s * g.N;
-      const AdjStressIn& q = r[u];
-      const float dvz_dx = (-kC1 * (q.vzpx - q.vz0) + kC2 * (q.vzp2x - q.vzmx)) * rdx;
-      const float dvx_dz = (-kC1 * (q.vxp1 - q.vx0) + kC2 * (q.vxp2 - q.vxm1)) * rdz;
-      float sxz = q.sxz + (dvz_dx * rkx * ba * dt + dvx_dz * rkz * bb * dt);
-      if (ax != 0.f) sxz += ax * ((-kC1 * (q.q1[0] - q.q1[1]) + kC2 * (q.q1[2] - q.q1[3])) * rdx);
-      if (az != 0.f) sxz += az * ((-kC1 * (q.q2[0] - q.q2[1]) + kC2 * (q.q2[2] - q.q2[3])) * rdz);
-      sto(a.sxz + so, bo, sxz);
-      if (xlive) sto(p.vzx + so, bo, bxh * q.m_vzx + sxz * amu * dt);
-      if (zlive) sto(p.vxz + so, bo, bzh * q.m_vxz + sxz * amu * dt);
-      const float dvx_dx = (-kC1 * (q.vx0 - q.vxmx) + kC2 * (q.vxpx - q.vxm2x)) * rdx;
-      const float dvz_dz = (-kC1 * (q.vz0 - q.vzm1) + kC2 * (q.vzp1 - q.vzm2)) * rdz;
-      float sxx0 = q.sxx, szz0 = q.szz;
-      if (has[u]) {
-        // 3 res rounded before the add, as the separate injection adds it
-#pragma clang fp contract(off)
-        const float v3 = 3.f * resv[u];
-        szz0 += resv[u];
-        sxx0 += v3;
-      }
-      float sxx = sxx0 + bb * dvx_dx * rkxh * dt;
-      if (axh != 0.f) sxx += axh * ((-kC1 * (q.q3[0] - q.q3[1]) + kC2 * (q.q3[2] - q.q3[3])) * rdx);
-      float szz = szz0 + ba * dvz_dz * rkzh * dt;
-      if (azh != 0.f) szz += azh * ((-kC1 * (q.q4[0] - q.q4[1]) + kC2 * (q.q4[2] - q.q4[3])) * rdz);
-      sto(a.sxx + so, bo, sxx);
-      sto(a.szz + so, bo, szz);
-      if (xlive) sto(p.vxx + so, bo, bx * q.m_vxx + lam * szz * dt + l2m * sxx * dt);
-      if (zlive) sto(p.vzz + so, bo, bz * q.m_vzz + l2m * szz * dt + lam * sxx * dt);
-    }
-  }
-}
-
-template <int U>
-__global__ __launch_bounds__(256) void k_adj_stress(Tiling tl, Geo g, Mdl m, Pml c, Wave a, Mem p,
-                                                   Inj inj) {
-  k_adj_stress_body<U>(blockIdx.x, tl, g, m, c, a, p, inj);
-}
-
-// Interior checkpoints of the reverse sweep (no reference counterpart): the 5
-// components on the interior rectangle [nPml, nzA-1-nPml] x [nPml, nx-1-nPml]
-// are saved before forward step it and written back after reverse step it,
-// for it a positive multiple of the interval, which bounds the drift of the
-// time-reversed re-materialisation.  Layout [shot][checkpoint][component]
-// [interior cell, z fastest].  Thread domain: the interior rectangle.
-__global__ __launch_bounds__(256) void k_ckpt(Tiling tl, Geo g, Wave w, float* __restrict__ ck,
-                                             int slot, int nslots, int save) {
-  int tz, tx;
-  tile_of(tl, tz, tx);
-  const int Ih = g.nzA - 2 * g.nPml, Iw = g.nx - 2 * g.nPml;
-  const int zi = tz * kBZ + threadIdx.x;
-  const int xi = tx * (int)blockDim.y + threadIdx.y;
-  if (zi >= Ih || xi >= Iw) return;
-  const long long i = (long long)(xi + g.nPml) * g.nz + (zi + g.nPml);
-  const size_t Ic = (size_t)Ih * Iw, j = (size_t)xi * Ih + zi;
-  float* f[5] = {w.vz, w.vx, w.szz, w.sxx, w.sxz};
-  for (int s = 0; s < g.ns; ++s) {
-    float* b = ck + ((size_t)s * nslots + slot) * 5 * Ic + j;
-    const long long o = s * g.N + i;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      if (save)
-        b[q * Ic] = f[q][o];
-      else
-        f[q][o] = b[q * Ic];
-    }
-  }
-}
-
-// Backward kernel pairs with disjoint reads and writes, launched as one grid
-// so they share a launch and its ramp-up and drain.  The grid holds nA
-// workgroups of the first kernel and nB of the second (both multiples of 8,
-// so both parts see the same workgroup-to-XCD dealing).  With `mix` set,
-// rounds of 8 workgroups (one per XCD) alternate between the parts in
-// proportion to their sizes, so both sweep their tiles at the same pace and
-// the model parameters and shared adjoint values that both read are fetched
-// close together in time, one L2 fill serving both; otherwise the first nA
-// workgroups run the first kernel.
-//   pair a: rev_stress(it) + adj_vel(it) (rev_stress reads the adjoint stress
-//           that adj_vel also only reads; adj_vel writes the adjoint velocity
-//           that rev_stress does not touch)
-//   pair b: adj_stress(it) + rev_vel(it-1) (rev_vel reads the adjoint velocity
-//           that adj_stress also only reads; adj_stress writes the adjoint
-//           stress that rev_vel does not touch)
-__device__ inline bool pair_part(int b, int nA, int nB, int mix, int& local) {
-  if (!mix) {
-    local = b < nA ? b : b - nA;
-    return b < nA;
-  }
-  const long long RA = nA / 8, R = RA + nB / 8, r = b / 8;
-  const int before = (int)(r * RA / R);  // first-part rounds before round r
-  if ((int)((r + 1) * RA / R) > before) {
-    local = before * 8 + b % 8;
-    return true;
-  }
-  local = (int)(r - before) * 8 + b % 8;
-  return false;
-}
-
-template <int U>
-__global__ __launch_bounds__(256) void k_bwd_pair_a(int nA, int nB, int mix, Tiling tR, Tiling tU,
-                                                   Geo g, Mdl m, Pml c, Wave w, Wave a, Mem p,
-                                                   Src src, float* __restrict__ acc,
-                                                   const float* __restrict__ ring,
-                                                   float* __restrict__ stf_grad, int it) {
-  int b;
-  if (pair_part(blockIdx.x, nA, nB, mix, b)) {
-    if (b < tR.n)
-      k_rev_stress_body<U>(b, tR, g, m, w, a, src, acc, acc + g.N, acc + 2 * g.N, ring, it);
-  } else if (b < tU.n * tU.chunks) {
-    k_adj_vel_body<U>(b, tU, g, m, c, a, p, src, stf_grad, it);
-  }
-}
-
-template <int U>
-__global__ __launch_bounds__(256) void k_bwd_pair_b(int nA, int nB, int mix, Tiling tU, Tiling tR,
-                                                   Geo g, Mdl m, Pml c, Wave w, Wave a, Mem p,
-                                                   float* __restrict__ acc,
-                                                   const float* __restrict__ ring, int it_rev,
-                                                   Inj inj) {
-  int b;
-  if (pair_part(blockIdx.x, nA, nB, mix, b)) {
-    if (b < tU.n * tU.chunks) k_adj_stress_body<U>(b, tU, g, m, c, a, p, inj);
-  } else if (b < tR.n) {
-    k_rev_vel_body<U>(b, tR, g, m, w, a, acc + 3 * g.N, acc + 4 * g.N, ring, it_rev);
-  }
-}
-
 // Chain rule from the accumulated derivatives to (lambda, mu, rho), once per
 // run.  Deterministic gather form of the reference's atomicAdd sprays
-// (el_stress.cu:113-124, el_velocity.cu:234-243), with the same target
+// (el_stress.cu:113-124, el_velocity.cu:104-112), with the same target
 // conditions (including the always-true `gidx+1<=gidx<=nx-1-nPml`).
-__global__ __launch_bounds__(1024) void k_finalize(Tiling tl, Geo g, Mdl m, const float* __restrict__ acc,
+__global__ __launch_bounds__(256) void k_finalize(Geo g, Mdl m, const float* __restrict__ acc,
                                                   float* __restrict__ grad) {
-  int tz, tx;
-  tile_of(tl, tz, tx);
-  const int z = tz * kBZ + threadIdx.x;
-  const int x = tx * (int)blockDim.y + threadIdx.y;
-  if (z >= g.nz || x >= g.nx) return;
+  const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int x = (int)blockIdx.y;
+  if (z >= g.nz) return;
   const int nz = g.nz;
   const long long N = g.N;
   const float* acc_l = acc;
@@ -1796,14 +1187,12 @@ __global__ __launch_bounds__(1024) void k_finalize(Tiling tl, Geo g, Mdl m, cons
 
 // aveMuInit / aveBycInit (utilities.cu:125-152) over [2, nz-3] x [2, nx-3];
 // defaults 0 and 1/1000 elsewhere (Model.cu:67-73).
-__global__ void k_model_init(Tiling tl, Geo g, const float* __restrict__ mu, const float* __restrict__ den,
+__global__ void k_model_init(Geo g, const float* __restrict__ mu, const float* __restrict__ den,
                              float* __restrict__ amu, float* __restrict__ ba,
                              float* __restrict__ bb) {
-  int tz, tx;
-  tile_of(tl, tz, tx);
-  const int z = tz * kBZ + threadIdx.x;
-  const int x = tx * (int)blockDim.y + threadIdx.y;
-  if (z >= g.nz || x >= g.nx) return;
+  const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int x = (int)blockIdx.y;
+  if (z >= g.nz) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   float am = 0.f, fa = 1.f / 1000.f, fb = 1.f / 1000.f;
@@ -1841,136 +1230,234 @@ __global__ void k_transpose(const float* __restrict__ in, float* __restrict__ ou
 
 }  // namespace
 
-// ===================================================================== plan
-static Tiling make_tiling(int nz_dom, int nx_dom, int bx, bool xcd) {
-  Tiling t;
-  t.gz = (nz_dom + kBZ - 1) / kBZ;
-  t.gx = (nx_dom + bx - 1) / bx;
-  t.n = t.gz * t.gx;
-  t.xcd = xcd ? 1 : 0;
-  return t;
-}
+// ===================================================================== host
+namespace {
 
-// Workgroup -> tile order for an XCD-banded launch whose first `heavy_left`
-// and last `heavy_right` tile columns are heavier (the x-direction C-PML):
-// workgroup b runs on XCD b % 8, and each XCD gets one contiguous run of the
-// heavy tiles (run first) followed by one contiguous run of the others, so
-// every XCD carries the same share of heavy work and neighbouring tiles stay
-// on one XCD's L2.  Tile id = column * gz + row (rows fastest).
-static std::vector<int> balanced_tile_order(int gz, int gx, int heavy_left, int heavy_right) {
-  const int n = gz * gx;
-  std::vector<int> heavy, light;
-  for (int bx = 0; bx < gx; bx++) {
-    const bool h = bx < heavy_left || bx >= gx - heavy_right;
-    for (int bz = 0; bz < gz; bz++) (h ? heavy : light).push_back(bx * gz + bz);
-  }
-  std::vector<int> perm(n);
-  size_t ho = 0, lo = 0;
-  const size_t H = heavy.size();
-  for (int k = 0; k < 8; k++) {
-    const int len = n > k ? (n - k + 7) / 8 : 0;  // workgroups b < n with b % 8 == k
-    const size_t hk = std::min((size_t)len, H * (k + 1) / 8 - H * k / 8);
-    int j = 0;
-    for (size_t q = 0; q < hk; q++) perm[(size_t)(j++) * 8 + k] = heavy[ho++];
-    while (j < len) {
-      if (lo < light.size())
-        perm[(size_t)(j++) * 8 + k] = light[lo++];
-      else
-        perm[(size_t)(j++) * 8 + k] = heavy[ho++];
-    }
-  }
-  return perm;
-}
-
-// threads of a fused workgroup: whole wave64s covering the active rows
-static int fused_threads(const Geo& g) { return ((g.nzA + 63) / 64) * 64; }
-
-static int env_int(const char* name, int dflt) {
+int env_int(const char* name, int dflt) {
   const char* v = std::getenv(name);
   return v && *v ? std::atoi(v) : dflt;
 }
 
+// div1(x, d, RN(1/d)) == x / d for every normal x: checked over every
+// mantissa of the binades [1, 2) and [2, 4) (the computation scales exactly
+// with the exponent of x).  Cached per spacing; about 30 ms per new value.
+bool spacing_ok(float d) {
+  static std::mutex mu;
+  static std::map<float, bool> seen;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = seen.find(d);
+    if (it != seen.end()) return it->second;
+  }
+  const float y = 1.0f / d;
+  bool ok = std::isnormal(d) && std::isnormal(y);
+  for (unsigned e = 127; e <= 128 && ok; ++e)
+    for (unsigned m = 0; m < (1u << 23); ++m) {
+      const unsigned u = e << 23 | m;
+      float x;
+      std::memcpy(&x, &u, 4);
+      const float q = x * y;
+      const float v = std::fma(std::fma(-d, q, x), y, q);
+      if (v != x / d) {
+        ok = false;
+        break;
+      }
+    }
+  std::lock_guard<std::mutex> lk(mu);
+  seen[d] = ok;
+  return ok;
+}
+
+// ---------------------------------------------------- workgroup maps
+// Each XCD's ordered list of work; the launch grid interleaves them so that
+// workgroup b = 8 j + k is entry j of XCD k's list.
+struct Lists {
+  std::vector<unsigned> l[8];
+  size_t longest() const {
+    size_t n = 0;
+    for (const auto& v : l) n = std::max(n, v.size());
+    return n;
+  }
+  bool empty() const { return longest() == 0; }
+};
+
+// tiles [0, n) in 8 contiguous bands (z fastest, then x): neighbouring tiles,
+// which share halo columns, share an XCD's L2
+void band(Lists& L, unsigned kind, int part, int n) {
+  for (int k = 0; k < 8; k++)
+    for (int t = (int)((long long)n * k / 8); t < (int)((long long)n * (k + 1) / 8); t++)
+      L.l[k].push_back(wg_entry(kind, (unsigned)part, (unsigned)t));
+}
+
+// The update-region tiling, whose first `hl` and last `hr` tile columns are
+// heavier (the x-direction C-PML): each XCD gets one contiguous run of the
+// heavy tiles, first, then one contiguous run of the others.
+void balanced(Lists& L, unsigned kind, int part, int gz, int gx, int hl, int hr) {
+  std::vector<int> heavy, light;
+  for (int bx = 0; bx < gx; bx++) {
+    const bool h = bx < hl || bx >= gx - hr;
+    for (int bz = 0; bz < gz; bz++) (h ? heavy : light).push_back(bx * gz + bz);
+  }
+  const size_t H = heavy.size(), Ln = light.size();
+  for (int k = 0; k < 8; k++) {
+    for (size_t q = H * k / 8; q < H * (k + 1) / 8; q++)
+      L.l[k].push_back(wg_entry(kind, (unsigned)part, (unsigned)heavy[q]));
+    for (size_t q = Ln * k / 8; q < Ln * (k + 1) / 8; q++)
+      L.l[k].push_back(wg_entry(kind, (unsigned)part, (unsigned)light[q]));
+  }
+}
+
+// items [0, n) dealt round robin
+void dealt(Lists& L, unsigned kind, int part, int n) {
+  for (int t = 0; t < n; t++) L.l[t % 8].push_back(wg_entry(kind, (unsigned)part, (unsigned)t));
+}
+
+void append(Lists& A, const Lists& B) {
+  for (int k = 0; k < 8; k++) A.l[k].insert(A.l[k].end(), B.l[k].begin(), B.l[k].end());
+}
+
+// per XCD, the two lists interleaved in proportion to their lengths, so both
+// halves of a paired launch sweep their tiles at the same pace
+Lists interleave(const Lists& A, const Lists& B) {
+  Lists M;
+  for (int k = 0; k < 8; k++) {
+    const auto &a = A.l[k], &b = B.l[k];
+    const size_t na = a.size(), n = na + b.size();
+    size_t ia = 0, ib = 0;
+    for (size_t j = 0; j < n; j++) {
+      const bool takeA = ia < na && (ib >= b.size() || (j + 1) * na / n > ia);
+      M.l[k].push_back(takeA ? a[ia++] : b[ib++]);
+    }
+  }
+  return M;
+}
+
+std::vector<unsigned> flatten(const Lists& L) {
+  const size_t n = L.longest();
+  std::vector<unsigned> m(8 * n, wg_entry(kNop, 0, 0));
+  for (int k = 0; k < 8; k++)
+    for (size_t j = 0; j < L.l[k].size(); j++) m[8 * j + k] = L.l[k][j];
+  return m;
+}
+
+}  // namespace
+
+// Device-resident workgroup maps and part tables of a job, kept across runs.
+struct DevMap {
+  unsigned* ptr = nullptr;
+  int n = 0;  // workgroups of the launch
+};
+
+struct JobCache {
+  int dev = 0;
+  std::map<std::string, DevMap> maps;
+  Part* table = nullptr;  // part tables of the multi-part launches of the last run
+  size_t table_cap = 0;
+  ~JobCache() {
+    (void)hipSetDevice(dev);
+    for (auto& kv : maps) (void)hipFree(kv.second.ptr);
+    if (table) (void)hipFree(table);
+  }
+  Status map(const std::string& key, const Lists& L, DevMap& out) {
+    auto it = maps.find(key);
+    if (it != maps.end()) {
+      out = it->second;
+      return Status::Ok();
+    }
+    const std::vector<unsigned> m = flatten(L);
+    DevMap dm;
+    dm.n = (int)m.size();
+    HIP_TRY(hipMalloc(&dm.ptr, std::max<size_t>(1, m.size()) * sizeof(unsigned)));
+    if (!m.empty())
+      HIP_TRY(hipMemcpy(dm.ptr, m.data(), m.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+    maps[key] = dm;
+    out = dm;
+    return Status::Ok();
+  }
+  Status upload_table(const std::vector<Part>& parts) {
+    if (parts.size() > table_cap) {
+      if (table) HIP_TRY(hipFree(table));
+      table = nullptr;
+      table_cap = 0;
+      HIP_TRY(hipMalloc(&table, parts.size() * sizeof(Part)));
+      table_cap = parts.size();
+    }
+    if (!parts.empty())
+      HIP_TRY(hipMemcpy(table, parts.data(), parts.size() * sizeof(Part), hipMemcpyHostToDevice));
+    return Status::Ok();
+  }
+};
+
+// ===================================================================== device state
 struct DeviceState {
   int dev = 0;
   hipStream_t stream = nullptr;
-  Geo g{};
+  Geo g{};  // whole plan: ns = plan shots, s0 = 0
   int R = 0;
-  int ckpt_every = 0;     // interior checkpoint interval of the reverse sweep (0: none)
-  float* ck = nullptr;    // interior checkpoints [shot][slot][5][interior]
-  size_t ck_floats = 0;
-  int tile_x = 4;    // x extent of a workgroup tile (blockDim.y)
-  bool xcd = true;   // XCD-banded workgroup order
-  bool fused = false; // column-marching fused kernels (FWIFLOW_FUSED=1)
-  bool tiled = false;  // tiled fused forward step (FWIFLOW_TILED)
-  int seg_x = 32;    // x extent of a fused segment
-  int rev_group = 8; // shots per workgroup of the fused reverse step
-  bool fused_rev = true;  // fused reverse step (needs its LDS to fit)
-  int unroll = 1;         // shots whose loads a split-kernel thread issues together
-  int unroll_fwd = 2;     // ... in the forward kernels
-  int shot_group = -1;    // shots per launch of the split/tiled kernels (0: all, -1: auto)
-  int shot_group_bwd = -1;  // the same for the backward sweep (0: all, -1: auto)
-  // Two streams (default priority both): shot groups alternate between them,
-  // so one stream's kernel fills the other's launch gap; the backward groups
-  // of the second stream accumulate into acc + 5N.
-  int streams = 1;
-  hipStream_t stream2 = nullptr;
-  hipEvent_t join[2] = {nullptr, nullptr};
-  int spt = 0;            // shots per thread of the forward/adjoint split kernels (0: all)
-  bool pairs = true;      // backward kernel pairs share a launch
-  bool mix = true;        // ... with their workgroups interleaved round by round (pair a,
-  bool mix_b = true;      // and pair b: slower with 4-shot groups, faster with 3)
-  bool balance = true;    // C-PML columns shared evenly by the XCDs (update-region launches)
-  int* perm_u = nullptr;  // workgroup -> tile order of the update-region tiling
-  bool illum_on = false;  // sum the forward illumination
-  bool illum_valid = false;
-  // residual conditioning, shot by shot: the optional stages, and the path
-  // that keeps the conditioned data for the scratch files
-  bool cond = false;
-  FftDesc fft{};
-  int* fperm = nullptr;                    // [2*nSteps] digit-reversal permutation
-  float2* ftw = nullptr;                   // [2*nSteps] twiddles
-  float2* bpH = nullptr;                   // [nSteps+1] band-pass amp^2
-  float* wpad = nullptr;                   // [2*nSteps] window ratio 0.01 over the padded length
-  float *wst = nullptr, *wen = nullptr, *wwt = nullptr;  // [R] per-trace windows and weights
-  float *cobs = nullptr, *csyn = nullptr;  // [nSteps][nrec of one shot] conditioned data
-  float2 *spo = nullptr, *spc = nullptr;   // [nrec][nSteps+1] spectra (source update)
-  float2* coef = nullptr;                  // [nSteps+1] source-update filter of one shot
-  float* srcamp = nullptr;                 // [ns][nSteps] source rows as injected (stf * window)
-  float* srcout = nullptr;                 // [ns][nSteps] updated source rows
-  float* inj_rev = nullptr;                // [ns][nSteps] increments of the reversal
-  double* cpart = nullptr;                 // [ns][kCondBlocks] misfit partial sums
-  int nrec_max = 0;
-  bool src_updated = false;                // the last run updated the source rows
-  bool fold_inject = true;                 // residual injection inside the adjoint stress update
-  int* rec_at = nullptr;                   // [ns][nx] receiver of each column (one-row surveys)
-  int rec_row = -1;                        // the row of those receivers
-  std::vector<float> keep_obs, keep_syn;   // conditioned data [R][nSteps] for the scratch files
-  float* illum = nullptr; // [NX][NZ]
   std::vector<int> roff;  // first receiver of each shot (+ total)
-  std::vector<void*> allocs;
-  long long bytes = 0;
+  // engine settings (environment variables read at plan creation)
+  int tile_x = 4;          // x extent of a workgroup tile (blockDim.y)
+  bool xcd = true;         // XCD-banded workgroup order
+  bool balance = true;     // C-PML columns shared evenly by the XCDs (update-region launches)
+  bool mix = true;         // backward pair halves interleaved (pair a)
+  bool mix_b = true;       // ... (pair b)
+  int unroll = 1;          // shots whose loads a backward thread issues together
+  int unroll_fwd = 2;      // ... a forward thread
+  int shot_group = -1;     // shots per forward launch (0: all, -1: automatic)
+  int shot_group_bwd = -1; // shots per backward launch (0: all, -1: automatic)
+  bool fold_inject = true; // residual injection inside the adjoint stress update
+  bool part_table = false; // single-part launches also read their part from device memory
+  int ckpt_every = 0;      // 0: boundary ring every step; k > 0: checkpoints every k steps
+  double ring_budget = 32e9;  // device bytes a gradient run may hold for ring + checkpoints
   // model + tables
-  float *lam = nullptr, *mu = nullptr, *amu = nullptr, *ba = nullptr, *bb = nullptr;
-  float* den = nullptr;
+  float *lam = nullptr, *mu = nullptr, *den = nullptr, *amu = nullptr, *ba = nullptr,
+        *bb = nullptr;
   float* pml = nullptr;   // 6*nzA + 6*nx
   float* win2 = nullptr;  // nSteps
   int* src = nullptr;     // [ns][2]
-  float* inj = nullptr;   // [ns][nSteps]
-  int* rec = nullptr;     // [R][3]
+  float* srcamp = nullptr;  // [ns][nSteps] windowed source rows
+  int* rec = nullptr;     // [R][3] (z, x, plan shot)
+  int* rec_at = nullptr;  // [ns][nx] receiver of each column (one-row surveys)
+  int rec_row = -1;
+  // residual injection cells (surveys the folded injection does not cover)
+  int *icell = nullptr, *iptr = nullptr, *irid = nullptr;
+  std::vector<int> coff;  // first injection cell of each shot (+ total)
   // wavefields
-  float *fwd = nullptr, *adj = nullptr, *mem = nullptr;
-  float *fwd2 = nullptr, *mem2 = nullptr, *adj2 = nullptr;  // ping-pong partners
-  int *rptr = nullptr, *rzr = nullptr, *rrg = nullptr;       // receivers by column
-  float* accp = nullptr;  // per-group partial gradient accumulators
-  int accp_groups = 0;
+  float *fwd = nullptr, *mem = nullptr, *adj = nullptr, *madj = nullptr;
   float* ring = nullptr;
+  size_t ring_floats = 0;
+  float* snap = nullptr;
+  size_t snap_floats = 0;
   float *syn = nullptr, *obs = nullptr, *res = nullptr, *stage = nullptr;
-  float *acc = nullptr, *grad = nullptr, *stf_grad = nullptr;
+  float *acc = nullptr, *grad = nullptr, *stf_grad = nullptr, *illum = nullptr;
   double* partial = nullptr;
+  int npartial = 0;
   double* misfit = nullptr;
-  int npart = 0;
   bool model_set = false, stf_set = false, obs_set = false;
+  bool illum_on = false, illum_valid = false;
   double misfit_host = 0.0;
+  // residual conditioning, shot by shot
+  bool cond = false;
+  FftDesc fft{};
+  int* fperm = nullptr;
+  float2* ftw = nullptr;
+  float2* bpH = nullptr;
+  float* wpad = nullptr;
+  float *wst = nullptr, *wen = nullptr, *wwt = nullptr;
+  float *cobs = nullptr, *csyn = nullptr;
+  float2 *spo = nullptr, *spc = nullptr;
+  float2* coef = nullptr;
+  float* srcout = nullptr;
+  double* cpart = nullptr;
+  int nrec_max = 0;
+  bool src_updated = false;
+  std::vector<float> keep_obs, keep_syn;
+  // last gradient run's memory plan and shot groups
+  int last_k = 0, last_rows_sg = 0, last_groups_f = 0, last_groups_b = 0;
+  long long last_ring_bytes = 0, last_snap_bytes = 0;
+  std::unique_ptr<JobCache> cache;  // maps of this plan's own runs
+  std::vector<void*> allocs;
+  long long bytes = 0;
   // timing
   bool timing = false;
   struct Rec {
@@ -1983,15 +1470,13 @@ struct DeviceState {
 
   ~DeviceState() {
     if (stream) (void)hipSetDevice(dev);
+    cache.reset();
     for (auto& r : pending) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
     }
     for (auto e : pool) (void)hipEventDestroy(e);
-    for (auto e : join)
-      if (e) (void)hipEventDestroy(e);  // a null handle would leave a sticky error
     for (void* p : allocs) (void)hipFree(p);
-    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -2008,6 +1493,15 @@ struct DeviceState {
     *p = (T*)q;
     return Status::Ok();
   }
+  template <typename T>
+  void release(T*& p, size_t count) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), (void*)p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+    bytes -= (long long)(count * sizeof(T));
+    p = nullptr;
+  }
 
   hipEvent_t take_event() {
     if (!pool.empty()) {
@@ -2019,8 +1513,6 @@ struct DeviceState {
     (void)hipEventCreate(&e);
     return e;
   }
-
-  // Launch helper: optionally brackets the launch with events.
   template <typename F>
   void launch(const char* name, bool sample, F&& f) {
     if (timing && sample) {
@@ -2033,7 +1525,6 @@ struct DeviceState {
       f();
     }
   }
-
   void collect_timing() {
     for (auto& r : pending) {
       float ms = 0.f;
@@ -2052,70 +1543,41 @@ struct DeviceState {
     const int nzA = g.nzA, nx = g.nx;
     const float* z = pml;
     const float* x = pml + 6 * nzA;
-    return Pml{z,     z + nzA,     z + 2 * nzA, z + 3 * nzA, z + 4 * nzA, z + 5 * nzA,
-               x,     x + nx,      x + 2 * nx,  x + 3 * nx,  x + 4 * nx,  x + 5 * nx};
+    return Pml{z, z + nzA, z + 2 * nzA, z + 3 * nzA, z + 4 * nzA, z + 5 * nzA,
+               x, x + nx,  x + 2 * nx,  x + 3 * nx,  x + 4 * nx,  x + 5 * nx};
   }
   Mdl mdl() const { return Mdl{lam, mu, amu, ba, bb}; }
-  Wave wave(float* base) const {
-    const size_t F = (size_t)g.ns * g.N;
-    return Wave{base, base + F, base + 2 * F, base + 3 * F, base + 4 * F};
-  }
-  Mem memv(float* b) const {
-    const size_t F = (size_t)g.ns * g.N;
-    return Mem{b, b + F, b + 2 * F, b + 3 * F, b + 4 * F, b + 5 * F, b + 6 * F, b + 7 * F};
-  }
-  // Views of shots [s0, s0 + n) of the launch: every per-shot array is
-  // offset to shot s0 and the geometry counts n shots.
-  Geo group_geo(int s0, int n) const {
-    Geo q = g;
-    q.s0 = s0;
-    q.ns = n;
-    return q;
-  }
-  Wave wave(float* base, int s0) const {
-    Wave w = wave(base);
+  size_t F() const { return (size_t)g.ns * g.N; }
+  Wave wave(float* base, int s0 = 0) const {
     const size_t o = (size_t)s0 * g.N;
-    return Wave{w.vz + o, w.vx + o, w.szz + o, w.sxx + o, w.sxz + o};
+    return Wave{base + o, base + F() + o, base + 2 * F() + o, base + 3 * F() + o,
+                base + 4 * F() + o};
   }
-  Mem memv(float* b, int s0) const {
-    Mem p = memv(b);
-    const size_t o = (size_t)s0 * g.N;
-    return Mem{p.vzz + o, p.vzx + o, p.vxz + o, p.vxx + o,
-               p.szz + o, p.sxx + o, p.sxzz + o, p.sxzx + o};
+  Mem memv(float* b, int s0 = 0) const {
+    const size_t o = (size_t)s0 * g.N, f = F();
+    return Mem{b + o,         b + f + o,     b + 2 * f + o, b + 3 * f + o,
+               b + 4 * f + o, b + 5 * f + o, b + 6 * f + o, b + 7 * f + o};
   }
-  // Shots per launch of the time-step kernels.  Automatic: as many shots as
-  // keep one group's backward working set (forward and adjoint state, C-PML
-  // memory variables, about 48 B per cell and shot) near 224 MB, i.e. inside
-  // the 256 MB Infinity Cache next to the model and accumulators.  Measured
-  // at 608x1780: 3, 4 and 5 shots give 55.2, 59.4 and 56.4 shot-gradients/s
-  // at nt 400; the rule picks 4.
-  int group_size() const {
-    if (fused || shot_group == 0) return g.ns;
-    if (shot_group > 0) return std::min(shot_group, g.ns);
-    // ~40 B per cell and shot stay cache-resident from step to step in the
-    // forward sweep (5 at 608x1780: 4.9 groups-of-5 measured 7.8 us per
-    // shot-step against 8.1 for 4 and 9.3 for 8)
-    const double per_shot = 40.0 * (double)g.N;
-    const int n = (int)std::lround(224e6 / per_shot);
-    return std::max(1, std::min(n, g.ns));
+  int gzU() const { return (g.nzA + kBZ - 1) / kBZ; }
+  int gxU() const { return (g.nx + tile_x - 1) / tile_x; }
+  int gzR() const { return (g.ringH + kBZ - 1) / kBZ; }
+  int gxR() const { return (g.ringW + tile_x - 1) / tile_x; }
+  // ~40 B per cell and shot of forward state stay cache-resident from step to
+  // step; the backward holds forward and adjoint state (~64 B).  224 MB of
+  // the 256 MB Infinity Cache for them (measured at 608x1780: 5 shots per
+  // forward launch, 3 per backward launch).
+  double fwd_bytes_per_shot() const { return 40.0 * (double)g.N; }
+  double bwd_bytes_per_shot() const { return 64.0 * (double)g.N; }
+  size_t ring_step_floats() const { return (size_t)5 * g.ringlen; }
+  size_t snap_floats_per_shot() const { return (size_t)13 * g.N; }
+  // zero [s0, s0+n) of a [component][shot][cell] array of `comps` components
+  // on stream st (the stream of the run, which may be another plan's)
+  Status zero_shots(float* base, int comps, int s0, int n, hipStream_t st) {
+    const size_t o = (size_t)s0 * g.N, len = (size_t)n * g.N;
+    for (int c = 0; c < comps; c++)
+      HIP_TRY(hipMemsetAsync(base + (size_t)c * F() + o, 0, len * sizeof(float), st));
+    return Status::Ok();
   }
-  // The backward sweep holds forward and adjoint state of its group: fewer
-  // shots fit the 256 MB Infinity Cache (3 at 608x1780, where 4 shots'
-  // 4 x 43 MB of state plus model and accumulators overflow it).
-  int group_size_bwd() const {
-    if (fused || (shot_group_bwd < 0 && shot_group == 0)) return g.ns;
-    if (shot_group_bwd == 0) return g.ns;
-    if (shot_group_bwd > 0) return std::min(shot_group_bwd, g.ns);
-    if (shot_group > 0) return std::min(shot_group, g.ns);
-    const double per_shot = 64.0 * (double)g.N;
-    const int n = (int)std::lround(224e6 / per_shot);
-    return std::max(1, std::min(n, g.ns));
-  }
-  Src src_view(int s0) const { return Src{src + 2 * s0, inj + (size_t)s0 * g.nSteps}; }
-  Src src_view(int s0, const float* rows) const {
-    return Src{src + 2 * s0, rows + (size_t)s0 * g.nSteps};
-  }
-  float* ring_view(int) const { return ring; }  // ring_ptr adds the group's first shot
 };
 
 Plan::Plan() = default;
@@ -2125,6 +1587,11 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   TRY(parse_para(para_fname, para));
   TRY(parse_survey(para, group_size, shot_ids, survey));
   gpu_id = gpu;
+  if (!spacing_ok(para.dz) || !spacing_ok(para.dx))
+    return Status::Err(FWIFLOW_ERR_UNSUPPORTED,
+                       "grid spacing: the reciprocal division of the stencils is not correctly "
+                       "rounded for dz = " + std::to_string(para.dz) + " or dx = " +
+                           std::to_string(para.dx));
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));
   if (gpu < 0 || gpu >= ndev)
@@ -2135,48 +1602,38 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.dev = gpu;
   d.tile_x = std::max(1, std::min(4, env_int("FWIFLOW_TILE_X", 4)));  // <= 256 threads
   d.xcd = env_int("FWIFLOW_XCD_REMAP", 1) != 0;
-  d.fused = env_int("FWIFLOW_FUSED", 0) != 0;
-  d.tiled = env_int("FWIFLOW_TILED", 0) != 0 && !d.fused;
-  d.seg_x = std::max(4, std::min(256, env_int("FWIFLOW_SEG_X", 32)));
-  d.rev_group = std::max(1, std::min(64, env_int("FWIFLOW_REV_GROUP", 8)));
-  {
-    const int u = env_int("FWIFLOW_SHOT_UNROLL", 1);
-    d.unroll = u >= 4 ? 4 : (u >= 2 ? 2 : 1);
-    // the forward kernels keep occupancy 4-5 with two shots' loads in flight
-    // and gained (6.07-6.09 vs 6.05-6.06 shot-gradients/s)
-    const int uf = env_int("FWIFLOW_SHOT_UNROLL_FWD", 2);
-    d.unroll_fwd = uf >= 4 ? 4 : (uf >= 2 ? 2 : 1);
-  }
-  d.shot_group = std::max(-1, env_int("FWIFLOW_SHOT_GROUP", -1));
-  d.shot_group_bwd = std::max(-1, env_int("FWIFLOW_SHOT_GROUP_BWD", -1));
-  d.streams = env_int("FWIFLOW_STREAMS", 1) == 2 ? 2 : 1;
-  d.spt = std::max(0, env_int("FWIFLOW_SHOTS_PER_THREAD", 0));
-  d.pairs = env_int("FWIFLOW_BWD_PAIRS", 1) != 0;
+  d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
   d.mix_b = env_int("FWIFLOW_PAIR_MIX_B", 1) != 0;
-  d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
+  d.unroll = env_int("FWIFLOW_SHOT_UNROLL", 1) >= 2 ? 2 : 1;
+  d.unroll_fwd = env_int("FWIFLOW_SHOT_UNROLL_FWD", 2) >= 2 ? 2 : 1;
+  d.shot_group = std::max(-1, env_int("FWIFLOW_SHOT_GROUP", -1));
+  d.shot_group_bwd = std::max(-1, env_int("FWIFLOW_SHOT_GROUP_BWD", -1));
+  d.fold_inject = env_int("FWIFLOW_FOLD_INJECT", 1) != 0;
+  d.part_table = env_int("FWIFLOW_PART_TABLE", 0) != 0;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
-  if (d.streams == 2) {
-    HIP_TRY(hipStreamCreate(&d.stream2));
-    for (auto& e : d.join) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  }
   Geo& g = d.g;
   g.nz = para.nz;
   g.nx = para.nx;
   g.nzA = para.nzA();
   g.nPml = para.nPml;
   g.ns = group_size;
+  g.s0 = 0;
   g.nSteps = para.nSteps;
   g.N = (long long)para.nz * para.nx;
   g.dt = para.dt;
+  g.dz = para.dz;
+  g.dx = para.dx;
   g.rdz = 1.f / para.dz;
   g.rdx = 1.f / para.dx;
   g.ringH = g.nzA - 2 * g.nPml + 4;
   g.ringW = g.nx - 2 * g.nPml + 4;
   g.ringlen = 10 * g.ringW + 10 * (g.ringH - 10);
-  g.s0 = 0;
   d.R = survey.total_rec();
+  g.R = d.R;
+  g.rs0 = 0;
+  g.rnt = std::max(1, g.nSteps - 1);
 
   const size_t N = (size_t)g.N;
   TRY(d.alloc(&d.lam, N));
@@ -2185,7 +1642,6 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   TRY(d.alloc(&d.amu, N));
   TRY(d.alloc(&d.ba, N));
   TRY(d.alloc(&d.bb, N));
-  // host tables
   std::vector<float> tab(6 * (size_t)g.nzA + 6 * (size_t)g.nx);
   cpml_profiles(g.nzA, g.nPml, para.dz, para.f0, para.dt, tab.data());
   cpml_profiles(g.nx, g.nPml, para.dx, para.f0, para.dt, tab.data() + 6 * g.nzA);
@@ -2210,7 +1666,6 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
     }
   }
   d.roff[g.ns] = (int)r;
-  d.fold_inject = env_int("FWIFLOW_FOLD_INJECT", 1) != 0;
   {
     // one-row surveys with at most one receiver per cell get the injection
     // inside the adjoint stress update
@@ -2222,78 +1677,46 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
       if (rec[3 * k] != row || slot >= 0) one_row = false;
       slot = k;
     }
-    if (one_row) {
+    if (one_row && d.fold_inject) {
       TRY(d.alloc(&d.rec_at, at.size()));
       HIP_TRY(hipMemcpy(d.rec_at, at.data(), at.size() * sizeof(int), hipMemcpyHostToDevice));
       d.rec_row = row;
     }
   }
+  {
+    // injection cells of each shot, receivers of a cell in receiver order
+    std::vector<int> cells, ptr{0}, rid;
+    d.coff.assign(g.ns + 1, 0);
+    for (int s = 0; s < g.ns; s++) {
+      d.coff[s] = (int)cells.size() / 3;
+      std::map<std::pair<int, int>, std::vector<int>> by;  // (x, z) -> receivers
+      for (int k = d.roff[s]; k < d.roff[s + 1]; k++)
+        by[{rec[3 * k + 1], rec[3 * k]}].push_back(k);
+      for (auto& kv : by) {
+        cells.insert(cells.end(), {s, kv.first.second, kv.first.first});
+        rid.insert(rid.end(), kv.second.begin(), kv.second.end());
+        ptr.push_back((int)rid.size());
+      }
+    }
+    d.coff[g.ns] = (int)cells.size() / 3;
+    TRY(d.alloc(&d.icell, std::max<size_t>(1, cells.size())));
+    TRY(d.alloc(&d.iptr, ptr.size()));
+    TRY(d.alloc(&d.irid, std::max<size_t>(1, rid.size())));
+    if (!cells.empty())
+      HIP_TRY(hipMemcpy(d.icell, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.iptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!rid.empty())
+      HIP_TRY(hipMemcpy(d.irid, rid.data(), rid.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   TRY(d.alloc(&d.src, src.size()));
   HIP_TRY(hipMemcpy(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
-  TRY(d.alloc(&d.rec, rec.size()));
-  HIP_TRY(hipMemcpy(d.rec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
-  {
-    // receivers of each shot grouped by column (CSR) for the fused adjoint
-    std::vector<int> ptr((size_t)g.ns * (g.nx + 1), 0), zr(d.R), rg(d.R);
-    size_t base = 0, rglob = 0;
-    for (int s = 0; s < g.ns; s++) {
-      const Shot& sh = survey.shots[s];
-      int* P = ptr.data() + (size_t)s * (g.nx + 1);
-      std::vector<int> cnt(g.nx + 1, 0);
-      for (size_t k = 0; k < sh.x_rec.size(); k++) cnt[sh.x_rec[k] + 1]++;
-      for (int x = 0; x < g.nx; x++) cnt[x + 1] += cnt[x];
-      for (int x = 0; x <= g.nx; x++) P[x] = (int)base + cnt[x];
-      std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-      for (size_t k = 0; k < sh.x_rec.size(); k++) {
-        const size_t pos = base + fill[sh.x_rec[k]]++;
-        zr[pos] = sh.z_rec[k];
-        rg[pos] = (int)(rglob + k);
-      }
-      base += sh.x_rec.size();
-      rglob += sh.x_rec.size();
-    }
-    TRY(d.alloc(&d.rptr, ptr.size()));
-    TRY(d.alloc(&d.rzr, zr.size()));
-    TRY(d.alloc(&d.rrg, rg.size()));
-    HIP_TRY(hipMemcpy(d.rptr, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.rzr, zr.data(), zr.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.rrg, rg.data(), rg.size() * sizeof(int), hipMemcpyHostToDevice));
-  }
-  if (d.balance) {
-    const Tiling t = make_tiling(g.nzA, g.nx, d.tile_x, true);
-    const int hl = (g.nPml + d.tile_x - 1) / d.tile_x;
-    const int hr = t.gx - (g.nx - g.nPml) / d.tile_x;
-    const std::vector<int> perm = balanced_tile_order(t.gz, t.gx, hl, hr);
-    std::vector<char> seen(perm.size(), 0);
-    for (int v : perm) {
-      if (v < 0 || v >= t.n || seen[v])
-        return Status::Err(FWIFLOW_ERR_HIP, "internal: workgroup order is not a permutation");
-      seen[v] = 1;
-    }
-    TRY(d.alloc(&d.perm_u, perm.size()));
-    HIP_TRY(hipMemcpy(d.perm_u, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
-  }
-  TRY(d.alloc(&d.inj, (size_t)g.ns * g.nSteps));
+  TRY(d.alloc(&d.rec, std::max<size_t>(3, rec.size())));
+  if (!rec.empty())
+    HIP_TRY(hipMemcpy(d.rec, rec.data(), rec.size() * sizeof(int), hipMemcpyHostToDevice));
+  TRY(d.alloc(&d.srcamp, (size_t)g.ns * g.nSteps));
   const size_t F = (size_t)g.ns * N;
   TRY(d.alloc(&d.fwd, 5 * F));
   TRY(d.alloc(&d.mem, 8 * F));
-  if (d.tiled) {
-    TRY(d.alloc(&d.fwd2, 5 * F));
-    TRY(d.alloc(&d.mem2, 8 * F));
-    d.fused_rev = false;
-  } else if (d.fused && fused_threads(g) <= 1024) {
-    TRY(d.alloc(&d.fwd2, 5 * F));
-    TRY(d.alloc(&d.mem2, 8 * F));
-    const size_t ldsr = (size_t)(10 + 5 * TXR) * fused_threads(g) * sizeof(float);
-    d.fused_rev = env_int("FWIFLOW_FUSED_REV", 1) != 0 && ldsr <= 160 * 1024;
-    if (d.fused_rev)
-      HIP_TRY(hipFuncSetAttribute((const void*)k_rev_step,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsr));
-  } else {
-    d.fused = false;
-    d.fused_rev = false;
-  }
-  TRY(d.alloc(&d.srcamp, (size_t)g.ns * g.nSteps));
   d.cond = para.conditioned() || para.save_scratch;
   if (d.cond) {
     const int nt = g.nSteps, L = 2 * nt;
@@ -2339,13 +1762,9 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
       TRY(d.alloc(&d.spc, (size_t)d.nrec_max * (nt + 1)));
       TRY(d.alloc(&d.coef, (size_t)nt + 1));
       TRY(d.alloc(&d.srcout, (size_t)g.ns * nt));
-      TRY(d.alloc(&d.inj_rev, (size_t)g.ns * nt));
     }
   }
-  const size_t T = (size_t)g.nSteps * d.R;
-  TRY(d.alloc(&d.syn, T));
-  d.npart = 1024;
-  TRY(d.alloc(&d.partial, (size_t)d.npart));
+  TRY(d.alloc(&d.syn, std::max<size_t>(1, (size_t)g.nSteps * d.R)));
   TRY(d.alloc(&d.misfit, 1));
   return Status::Ok();
 }
@@ -2362,9 +1781,8 @@ Status Plan::set_model(const double* Lambda, const double* Mu, const double* Den
   HIP_TRY(hipMemcpyAsync(d.lam, lam.data(), nb, hipMemcpyHostToDevice, d.stream));
   HIP_TRY(hipMemcpyAsync(d.mu, mu.data(), nb, hipMemcpyHostToDevice, d.stream));
   HIP_TRY(hipMemcpyAsync(d.den, den.data(), nb, hipMemcpyHostToDevice, d.stream));
-  const Tiling tA = make_tiling(d.g.nz, d.g.nx, d.tile_x, d.xcd);
-  hipLaunchKernelGGL(k_model_init, dim3(tA.n), dim3(kBZ, d.tile_x), 0, d.stream, tA, d.g, d.mu,
-                     d.den, d.amu, d.ba, d.bb);
+  hipLaunchKernelGGL(k_model_init, dim3((d.g.nz + 255) / 256, d.g.nx), dim3(256), 0, d.stream,
+                     d.g, d.mu, d.den, d.amu, d.ba, d.bb);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(d.stream));
   d.model_set = true;
@@ -2376,27 +1794,21 @@ Status Plan::set_stf(const double* stf) {
   if (!stf) return Status::Err(FWIFLOW_ERR_ARG, "null stf pointer");
   DeviceState& d = *d_;
   const int nSteps = d.g.nSteps;
-  // Src_Rec.cu:132-143: row shot_id -> float -> window ratio 0.001; the
-  // increment added to szz is 1500^2 * stf * dt (add_source, utilities.cu:528-538)
-  std::vector<float> w2(nSteps), inj((size_t)d.g.ns * nSteps), amps(inj.size());
+  // Src_Rec.cu:132-143: row shot_id -> float -> window ratio 0.001
+  std::vector<float> w2(nSteps), amps((size_t)d.g.ns * nSteps);
   window_squared(nSteps, para.dt, 0.001f, w2.data());
   for (int s = 0; s < d.g.ns; s++) {
     const double* row = stf + (size_t)survey.shots[s].id * nSteps;
-    for (int it = 0; it < nSteps; it++) {
-      const float amp = (float)row[it] * w2[it];
-      amps[(size_t)s * nSteps + it] = amp;
-      inj[(size_t)s * nSteps + it] = (float)(2250000.0 * (double)amp * (double)para.dt);
-    }
+    for (int it = 0; it < nSteps; it++) amps[(size_t)s * nSteps + it] = (float)row[it] * w2[it];
   }
   HIP_TRY(hipSetDevice(d.dev));
-  HIP_TRY(hipMemcpy(d.inj, inj.data(), inj.size() * sizeof(float), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(d.srcamp, amps.data(), amps.size() * sizeof(float), hipMemcpyHostToDevice));
   d.stf_set = true;
   return Status::Ok();
 }
 
 static Status ensure_obs_buffers(DeviceState& d) {
-  const size_t T = (size_t)d.g.nSteps * d.R;
+  const size_t T = std::max<size_t>(1, (size_t)d.g.nSteps * d.R);
   if (!d.obs) TRY(d.alloc(&d.obs, T));
   if (!d.res) TRY(d.alloc(&d.res, T));
   if (!d.stage) TRY(d.alloc(&d.stage, T));
@@ -2443,10 +1855,24 @@ Status Plan::load_obs() {
 Status Plan::set_checkpoint(int k) {
   if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
   if (k < 0) return Status::Err(FWIFLOW_ERR_ARG, "checkpoint interval must be >= 0");
-  if (k > 0 && d_->fused)
-    return Status::Err(FWIFLOW_ERR_UNSUPPORTED,
-                       "interior checkpoints need the split or tiled kernels (FWIFLOW_FUSED=0)");
   d_->ckpt_every = k;
+  return Status::Ok();
+}
+
+Status Plan::set_memory_budget(double bytes) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (!(bytes > 0)) return Status::Err(FWIFLOW_ERR_ARG, "memory budget must be positive");
+  d_->ring_budget = bytes;
+  return Status::Ok();
+}
+
+Status Plan::memory_plan(int* k, int* shots_per_sweep, long long* ring_bytes,
+                         long long* checkpoint_bytes) const {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (k) *k = d_->last_k;
+  if (shots_per_sweep) *shots_per_sweep = d_->last_rows_sg;
+  if (ring_bytes) *ring_bytes = d_->last_ring_bytes;
+  if (checkpoint_bytes) *checkpoint_bytes = d_->last_snap_bytes;
   return Status::Ok();
 }
 
@@ -2467,28 +1893,38 @@ Status Plan::timing(const std::string& name, double* ms, long long* count) {
 
 long long Plan::device_bytes() const { return d_ ? d_->bytes : 0; }
 
-int Plan::shots_per_launch() const { return d_ ? d_->group_size() : 0; }
-int Plan::shots_per_launch_bwd() const { return d_ ? d_->group_size_bwd() : 0; }
+int Plan::shots_per_launch() const {
+  if (!d_) return 0;
+  return d_->last_groups_f > 0 ? (d_->g.ns + d_->last_groups_f - 1) / d_->last_groups_f : 0;
+}
+int Plan::shots_per_launch_bwd() const {
+  if (!d_) return 0;
+  return d_->last_groups_b > 0 ? (d_->g.ns + d_->last_groups_b - 1) / d_->last_groups_b : 0;
+}
+int Plan::launch_groups(int backward) const {
+  if (!d_) return 0;
+  return backward ? d_->last_groups_b : d_->last_groups_f;
+}
 
-// Residual with the optional conditioning stages, one shot at a time
-// (libCUFD.cu:254-313): window obs and syn (per-trace windows and weights, or
-// the default ratio-0.005 window), band-pass both, source update (syn and the
-// source row replaced), res = obs - syn and its misfit, then the adjoint
-// stages on res: source-update filter, band-pass, window (ratio 0.1 with
-// per-trace windows).  `keep` copies the conditioned obs and syn to the host
-// for the scratch files.
-static Status condition_residual(DeviceState& d, const Para& para, bool keep) {
+// Residual with the optional conditioning stages for shots [sb, se), one shot
+// at a time (libCUFD.cu:254-313): window obs and syn (per-trace windows and
+// weights, or the default ratio-0.005 window), band-pass both, source update
+// (syn and the source row replaced), res = obs - syn and its misfit, then the
+// adjoint stages on res: source-update filter, band-pass, window (ratio 0.1
+// with per-trace windows).  `keep` copies the conditioned obs and syn to the
+// host for the scratch files.
+static Status condition_residual(DeviceState& d, const Para& para, bool keep, int sb, int se,
+                                 hipStream_t st) {
   const Geo& g = d.g;
   const int nt = g.nSteps, nf = nt + 1, kc = touched_bins(nt);
   const float scale = 1.0f / (float)(2 * nt);
   const float dt = para.dt;
-  hipStream_t st = d.stream;
-  if (keep) {
+  if (keep && d.keep_obs.size() != (size_t)d.R * nt) {
     d.keep_obs.assign((size_t)d.R * nt, 0.f);
     d.keep_syn.assign((size_t)d.R * nt, 0.f);
   }
   std::vector<float> h;
-  for (int s = 0; s < g.ns; s++) {
+  for (int s = sb; s < se; s++) {
     const int r0 = d.roff[s], n = d.roff[s + 1] - r0;
     const float* ws = para.if_win ? d.wst + r0 : nullptr;
     const float* we = para.if_win ? d.wen + r0 : nullptr;
@@ -2512,7 +1948,6 @@ static Status condition_residual(DeviceState& d, const Para& para, bool keep) {
       HIP_TRY(trace_filter(st, d.fft, d.fperm, d.ftw, nt,
                            TraceSet{d.srcamp + so, 1, d.srcout + so, 1, 1}, nullptr, d.coef, nf,
                            scale));
-      HIP_TRY(source_increment(st, nt, d.srcout + so, dt, d.inj_rev + so));
     }
     HIP_TRY(shot_residual(st, nt, n, d.cobs, d.csyn, n, d.res + r0, d.R,
                           d.cpart + (size_t)s * kCondBlocks, kCondBlocks));
@@ -2534,393 +1969,605 @@ static Status condition_residual(DeviceState& d, const Para& para, bool keep) {
       }
     }
   }
-  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d.cpart, g.ns * kCondBlocks,
-                     d.misfit);
-  HIP_TRY(hipGetLastError());
   return Status::Ok();
 }
+
+// ===================================================================== job runner
+// One run of calc_id over one or several plans on one device.  The plans'
+// shots are lined up in rows (row r = shot r of every plan that has one); a
+// launch of a time-step kernel covers a group of rows of every plan, each
+// plan one part of the launch's workgroup map.
+//
+// Gradient runs take the rows in sweeps (super-groups): a sweep runs the
+// forward of its rows, their residual and their backward before the next
+// sweep starts, so the boundary store holds one sweep's shots, not all.
+// The store is either the reference's ring of every step (k = 0) or
+// checkpoints every k steps plus the ring of one k-step segment, the segment
+// recomputed from its checkpoint in the backward sweep.
+namespace {
+
+struct Member {
+  Plan* plan;
+  DeviceState* d;
+};
+
+struct RowRange {
+  int r0, r1;
+};
+
+std::vector<RowRange> split_even(int a, int b, int per) {
+  std::vector<RowRange> v;
+  if (b <= a) return v;
+  const int n = (b - a + per - 1) / per;
+  for (int k = 0; k < n; k++)
+    v.push_back({a + (int)((long long)k * (b - a) / n), a + (int)((long long)(k + 1) * (b - a) / n)});
+  return v;
+}
+
+enum Phase { kPhFwd = 0, kPhRec = 1, kPhBwd = 2 };
+
+// the parts of one group of rows in one phase
+struct GroupParts {
+  std::vector<Part> parts;
+  std::vector<int> member;  // member of each part
+  size_t table_off = 0;     // offset of the parts in the job's device table
+  std::string key;          // row range, for the map cache
+};
+
+struct StepLaunch {
+  const Part* table = nullptr;  // device table (multi-part)
+  const Part* single = nullptr;
+  const unsigned* map = nullptr;
+  int nwg = 0;
+};
+
+class Runner {
+ public:
+  Runner(std::vector<Member> m, JobCache& c) : M(std::move(m)), cache(c) {}
+  Status run(int calc_id);
+
+ private:
+  std::vector<Member> M;
+  JobCache& cache;
+  DeviceState* d0 = nullptr;
+  hipStream_t st = nullptr;
+  dim3 blk;
+  int nSteps = 0, rows = 0, gf = 1, gb = 1, S = 1, K = 0, nsnap = 0;
+  bool if_res = false, withAdj = false, ckpt = false;
+  std::vector<Part> table;  // host copy of every multi-part group's parts
+
+  int shots_in(const Member& m, int r) const { return std::min(r, m.d->g.ns); }
+  GroupParts parts_of(RowRange rr, int sg0, Phase ph);
+  Status maps_of(const GroupParts& gp, const char* kind, DevMap& out);
+  StepLaunch step(const GroupParts& gp, const DevMap& m) const {
+    StepLaunch s;
+    s.map = m.ptr;
+    s.nwg = m.n;
+    if (gp.parts.size() == 1 && !d0->part_table)
+      s.single = &gp.parts[0];
+    else
+      s.table = cache.table + gp.table_off;
+    return s;
+  }
+  void fwd_stress(const StepLaunch& s, int it, int U);
+  void fwd_vel(const StepLaunch& s, int it, int U);
+  void bwd_a(const StepLaunch& s, int it, int U);
+  void bwd_b(const StepLaunch& s, int it, int U);
+  Status snapshot(const GroupParts& gp, int sg0, int j, bool save);
+};
+
+#define STEP_LAUNCH(KER)                                                                    \
+  do {                                                                                      \
+    if (!s.nwg) break;                                                                      \
+    const dim3 grid((unsigned)s.nwg);                                                       \
+    if (s.single) {                                                                         \
+      const Parts<1> a{*s.single};                                                          \
+      if (U == 2)                                                                           \
+        hipLaunchKernelGGL((KER<1, 2>), grid, blk, 0, st, a, s.map, it);                    \
+      else                                                                                  \
+        hipLaunchKernelGGL((KER<1, 1>), grid, blk, 0, st, a, s.map, it);                    \
+    } else {                                                                                \
+      const Parts<0> a{s.table};                                                            \
+      if (U == 2)                                                                           \
+        hipLaunchKernelGGL((KER<0, 2>), grid, blk, 0, st, a, s.map, it);                    \
+      else                                                                                  \
+        hipLaunchKernelGGL((KER<0, 1>), grid, blk, 0, st, a, s.map, it);                    \
+    }                                                                                       \
+  } while (0)
+
+void Runner::fwd_stress(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_fwd_stress); }
+void Runner::fwd_vel(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_fwd_vel); }
+void Runner::bwd_a(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_a); }
+void Runner::bwd_b(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_b); }
+#undef STEP_LAUNCH
+
+GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
+  GroupParts gp;
+  gp.key = std::to_string(rr.r0) + ":" + std::to_string(rr.r1);
+  for (size_t mi = 0; mi < M.size(); mi++) {
+    DeviceState& d = *M[mi].d;
+    const int s0 = shots_in(M[mi], rr.r0), s1 = shots_in(M[mi], rr.r1);
+    if (s1 <= s0) continue;
+    const int n = s1 - s0;
+    Part P{};
+    P.g = d.g;
+    P.g.ns = n;
+    P.g.s0 = s0;
+    P.g.rs0 = s0 - shots_in(M[mi], sg0);
+    P.g.rnt = ckpt ? K : std::max(1, nSteps - 1);
+    P.m = d.mdl();
+    P.c = d.pml_ptrs();
+    P.w = d.wave(d.fwd, s0);
+    P.a = d.adj ? d.wave(d.adj, s0) : Wave{};
+    P.p = ph == kPhBwd ? d.memv(ckpt ? d.madj : d.mem, s0) : d.memv(d.mem, s0);
+    const float* amp = (ph == kPhBwd && d.src_updated) ? d.srcout : d.srcamp;
+    P.src = Src{d.src + 2 * s0, amp + (size_t)s0 * nSteps};
+    P.inj = Inj{d.rec_at && ph == kPhBwd ? d.res : nullptr, d.rec_at, d.rec_row};
+    const bool ring = withAdj && ((ph == kPhFwd && !ckpt) || ph == kPhRec || ph == kPhBwd);
+    P.ring = ring ? d.ring : nullptr;
+    P.acc = d.acc;
+    P.stf_grad = d.stf_grad ? d.stf_grad + (size_t)s0 * nSteps : nullptr;
+    P.illum = ph == kPhFwd && d.illum_on ? d.illum : nullptr;
+    P.rec = d.rec + 3 * (size_t)d.roff[s0];
+    P.syn = d.syn + d.roff[s0];
+    P.nrec = d.roff[s1] - d.roff[s0];
+    P.gzU = d.gzU();
+    P.gzR = d.gzR();
+    gp.parts.push_back(P);
+    gp.member.push_back((int)mi);
+    gp.key += "|" + std::to_string(mi) + ":" + std::to_string(s0) + "-" + std::to_string(s1);
+  }
+  if (gp.parts.size() > 1 || d0->part_table) {
+    gp.table_off = table.size();
+    table.insert(table.end(), gp.parts.begin(), gp.parts.end());
+  }
+  return gp;
+}
+
+// Workgroup map of one launch kind over the parts of a group:
+//   "S" forward stress, "V" forward velocity + recording, "W" forward velocity,
+//   "A" reverse stress + adjoint velocity, "B" adjoint stress + reverse velocity,
+//   "AV" adjoint velocity, "AS" adjoint stress, "RV" reverse velocity.
+Status Runner::maps_of(const GroupParts& gp, const char* kind, DevMap& out) {
+  const std::string key = std::string(kind) + "/" + gp.key + "/" + std::to_string(d0->tile_x);
+  const std::string k = kind;
+  Lists all;
+  for (size_t q = 0; q < gp.parts.size(); q++) {
+    const DeviceState& d = *M[gp.member[q]].d;
+    const int part = (int)q;
+    auto upd = [&](unsigned kd) {
+      Lists L;
+      const int gz = d.gzU(), gx = d.gxU();
+      if (!d.xcd)
+        dealt(L, kd, part, gz * gx);
+      else if (d.balance)
+        balanced(L, kd, part, gz, gx, (d.g.nPml + d.tile_x - 1) / d.tile_x,
+                 gx - (d.g.nx - d.g.nPml) / d.tile_x);
+      else
+        band(L, kd, part, gz * gx);
+      return L;
+    };
+    auto rng = [&](unsigned kd) {
+      Lists L;
+      if (d.xcd)
+        band(L, kd, part, d.gzR() * d.gxR());
+      else
+        dealt(L, kd, part, d.gzR() * d.gxR());
+      return L;
+    };
+    Lists L;
+    if (k == "S") {
+      L = upd(kStressFwd);
+    } else if (k == "V" || k == "W") {
+      L = upd(kVelFwd);
+      if (k == "V") {
+        Lists r;
+        const int per = kBZ * d.tile_x;  // receivers per recording workgroup
+        dealt(r, kRecord, part, (gp.parts[q].nrec + per - 1) / per);
+        append(L, r);
+      }
+    } else if (k == "A") {
+      const Lists a = rng(kRevStress), b = upd(kAdjVel);
+      if (d.mix)
+        L = interleave(a, b);
+      else
+        L = a, append(L, b);
+    } else if (k == "B") {
+      const Lists a = upd(kAdjStress), b = rng(kRevVel);
+      if (d.mix_b)
+        L = interleave(a, b);
+      else
+        L = a, append(L, b);
+    } else if (k == "AV") {
+      L = upd(kAdjVel);
+    } else if (k == "AS") {
+      L = upd(kAdjStress);
+    } else if (k == "RV") {
+      L = rng(kRevVel);
+    }
+    append(all, L);
+  }
+  return cache.map(key, all, out);
+}
+
+// Checkpoint j >= 1 of the shots of a group: the 5 wavefield components and
+// the 8 forward C-PML memory variables before forward step j * K.
+// Layout [j-1][array][shot of the sweep][cell].
+Status Runner::snapshot(const GroupParts& gp, int sg0, int j, bool save) {
+  for (size_t q = 0; q < gp.parts.size(); q++) {
+    DeviceState& d = *M[gp.member[q]].d;
+    const int s0 = gp.parts[q].g.s0, n = gp.parts[q].g.ns;
+    const int rows_m = std::min(S, d.g.ns), rs = s0 - shots_in(M[gp.member[q]], sg0);
+    const size_t N = (size_t)d.g.N, F = d.F();
+    for (int a = 0; a < 13; a++) {
+      float* field = a < 5 ? d.fwd + (size_t)a * F : d.mem + (size_t)(a - 5) * F;
+      field += (size_t)s0 * N;
+      float* cp = d.snap + (((size_t)(j - 1) * 13 + a) * rows_m + rs) * N;
+      const size_t nb = (size_t)n * N * sizeof(float);
+      HIP_TRY(hipMemcpyAsync(save ? cp : field, save ? field : cp, nb, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  return Status::Ok();
+}
+
+Status Runner::run(int calc_id) {
+  TRY(check_calc_id(calc_id));
+  if (M.empty()) return Status::Err(FWIFLOW_ERR_ARG, "no plans to run");
+  d0 = M[0].d;
+  for (const Member& m : M) {
+    if (!m.d) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+    if (!m.d->model_set) return Status::Err(FWIFLOW_ERR_ARG, "model not set");
+    if (!m.d->stf_set) return Status::Err(FWIFLOW_ERR_ARG, "source time functions not set");
+    if (calc_id != 2 && !m.d->obs_set) return Status::Err(FWIFLOW_ERR_ARG, "observed data not set");
+    if (m.d->dev != d0->dev) return Status::Err(FWIFLOW_ERR_ARG, "plans of a job must share the GPU");
+    if (m.d->g.nSteps != d0->g.nSteps)
+      return Status::Err(FWIFLOW_ERR_ARG, "plans of a job must share nSteps");
+    if (m.d->tile_x != d0->tile_x)
+      return Status::Err(FWIFLOW_ERR_ARG, "plans of a job must share the tile width");
+    m.d->illum_valid = false;
+    m.plan->last_calc = -1;
+  }
+  if (M.size() > 1 && M.size() > (size_t)kMaxPartsTable)
+    return Status::Err(FWIFLOW_ERR_ARG, "at most " + std::to_string(kMaxPartsTable) + " plans per job");
+  HIP_TRY(hipSetDevice(d0->dev));
+  st = d0->stream;
+  blk = dim3(kBZ, d0->tile_x);
+  nSteps = d0->g.nSteps;
+  if_res = calc_id != 2;
+  withAdj = calc_id == 1;
+  rows = 0;
+  double rowf = 0, rowb = 0, ring_row = 0, snap_row = 0;
+  for (const Member& m : M) {
+    rows = std::max(rows, m.d->g.ns);
+    rowf += m.d->fwd_bytes_per_shot();
+    rowb += m.d->bwd_bytes_per_shot();
+    ring_row += 4.0 * m.d->ring_step_floats();
+    snap_row += 4.0 * m.d->snap_floats_per_shot();
+  }
+  // rows per launch: forward and backward state of a group kept in the
+  // Infinity Cache from step to step
+  const DeviceState& c0 = *d0;
+  auto autosize = [&](double per_row) {
+    return std::max(1, std::min(rows, (int)std::lround(224e6 / per_row)));
+  };
+  gf = c0.shot_group == 0 ? rows : c0.shot_group > 0 ? std::min(c0.shot_group, rows) : autosize(rowf);
+  if (c0.shot_group_bwd == 0)
+    gb = rows;
+  else if (c0.shot_group_bwd > 0)
+    gb = std::min(c0.shot_group_bwd, rows);
+  else if (c0.shot_group >= 0)
+    gb = gf;
+  else
+    gb = autosize(rowb);
+  // memory plan of a gradient run: rows per sweep S and checkpoint interval K
+  const int steps = std::max(1, nSteps - 1);  // forward steps 0 .. nSteps-2
+  const double budget = c0.ring_budget;
+  K = withAdj ? c0.ckpt_every : 0;
+  if (K >= steps) K = 0;
+  S = rows;
+  if (withAdj) {
+    const int l = gf / std::__gcd(gf, gb) * gb;
+    auto fit = [&](double per_row) {
+      // the most rows within the budget, a multiple of both group sizes if
+      // possible (no partial groups inside a sweep), else of the backward's
+      int n = (int)std::floor(budget / per_row);
+      if (n >= l) n = n / l * l;
+      else if (n >= gb) n = n / gb * gb;
+      return std::max(1, std::min(rows, n));
+    };
+    const double full = ring_row * steps;
+    if (K == 0 && full * std::min(rows, gb) > budget && steps > 1) {
+      // the full ring of one backward group does not fit: checkpoints, at the
+      // interval that minimises ring + checkpoint memory
+      K = (int)std::lround(std::sqrt((double)steps * snap_row / ring_row));
+      K = std::max(1, std::min(K, steps - 1));
+    }
+    ckpt = K > 0;
+    if (ckpt) {
+      nsnap = (steps + K - 1) / K - 1;
+      S = fit(K * ring_row + nsnap * snap_row);
+    } else {
+      nsnap = 0;
+      S = fit(full);
+    }
+  }
+  // sweeps (super-groups) and their launch groups, as row ranges
+  std::vector<RowRange> sweeps;
+  for (int a = 0; a < rows; a += S) sweeps.push_back({a, std::min(rows, a + S)});
+  std::vector<std::vector<RowRange>> FR(sweeps.size()), BR(sweeps.size());
+  for (size_t j = 0; j < sweeps.size(); j++) {
+    FR[j] = split_even(sweeps[j].r0, sweeps[j].r1, gf);
+    if (withAdj) BR[j] = split_even(sweeps[j].r0, sweeps[j].r1, gb);
+  }
+  // buffers
+  for (const Member& m : M) {
+    DeviceState& d = *m.d;
+    const size_t N = (size_t)d.g.N, F = d.F();
+    if (withAdj) {
+      if (!d.adj) TRY(d.alloc(&d.adj, 5 * F));
+      if (ckpt && !d.madj) TRY(d.alloc(&d.madj, 8 * F));
+      if (!d.acc) TRY(d.alloc(&d.acc, 5 * N));
+      if (!d.grad) TRY(d.alloc(&d.grad, 3 * N));
+      if (!d.stf_grad) TRY(d.alloc(&d.stf_grad, (size_t)d.g.ns * nSteps));
+      const int rows_m = std::min(S, d.g.ns);
+      const size_t rf = (size_t)rows_m * (ckpt ? K : steps) * d.ring_step_floats();
+      if (d.ring_floats < rf || d.ring_floats > 2 * rf + (1u << 20)) {
+        d.release(d.ring, d.ring_floats);
+        d.ring_floats = 0;
+        TRY(d.alloc(&d.ring, rf));
+        d.ring_floats = rf;
+      }
+      const size_t sf = ckpt ? (size_t)nsnap * 13 * rows_m * N : 0;
+      if (d.snap_floats < sf || (sf == 0 && d.snap)) {
+        d.release(d.snap, d.snap_floats);
+        d.snap_floats = 0;
+        if (sf) {
+          TRY(d.alloc(&d.snap, sf));
+          d.snap_floats = sf;
+        }
+      }
+      d.last_k = K;
+      d.last_rows_sg = rows_m;
+      d.last_ring_bytes = (long long)(rf * sizeof(float));
+      d.last_snap_bytes = (long long)(sf * sizeof(float));
+    }
+    const int npart = (int)sweeps.size() * kPartials;
+    if (d.npartial < npart) {
+      d.release(d.partial, (size_t)d.npartial);
+      d.npartial = 0;
+      TRY(d.alloc(&d.partial, (size_t)npart));
+      d.npartial = npart;
+    }
+    if (d.illum_on && !d.illum) TRY(d.alloc(&d.illum, N));
+    d.src_updated = if_res && d.cond && m.plan->para.if_src_update;
+    d.last_groups_f = 0;
+    d.last_groups_b = 0;
+    for (size_t j = 0; j < sweeps.size(); j++) {
+      for (RowRange rr : FR[j]) d.last_groups_f += shots_in(m, rr.r1) > shots_in(m, rr.r0);
+      for (RowRange rr : BR[j]) d.last_groups_b += shots_in(m, rr.r1) > shots_in(m, rr.r0);
+    }
+  }
+  // parts of every group (pointing at the buffers above)
+  table.clear();
+  std::vector<std::vector<GroupParts>> FG(sweeps.size()), RG(sweeps.size()), BG(sweeps.size());
+  for (size_t j = 0; j < sweeps.size(); j++) {
+    for (RowRange rr : FR[j]) FG[j].push_back(parts_of(rr, sweeps[j].r0, kPhFwd));
+    for (RowRange rr : BR[j]) {
+      BG[j].push_back(parts_of(rr, sweeps[j].r0, kPhBwd));
+      if (ckpt) RG[j].push_back(parts_of(rr, sweeps[j].r0, kPhRec));
+    }
+  }
+  if (!table.empty()) TRY(cache.upload_table(table));
+  // clear the run's state
+  for (const Member& m : M) {
+    DeviceState& d = *m.d;
+    const size_t N = (size_t)d.g.N, F = d.F();
+    HIP_TRY(hipMemsetAsync(d.fwd, 0, 5 * F * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(d.syn, 0, (size_t)nSteps * d.R * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(d.partial, 0, (size_t)d.npartial * sizeof(double), st));
+    if (d.illum_on) HIP_TRY(hipMemsetAsync(d.illum, 0, N * sizeof(float), st));
+    if (withAdj) {
+      HIP_TRY(hipMemsetAsync(d.acc, 0, 5 * N * sizeof(float), st));
+      HIP_TRY(hipMemsetAsync(d.stf_grad, 0, (size_t)d.g.ns * nSteps * sizeof(float), st));
+    }
+  }
+  const int Uf = d0->unroll_fwd, Ub = d0->unroll;
+  const int sample_every = 8;
+  for (size_t j = 0; j < sweeps.size(); j++) {
+    const RowRange sw = sweeps[j];
+    // ------------------------------------------------------ forward
+    for (const GroupParts& gp : FG[j]) {
+      DevMap mS, mV;
+      TRY(maps_of(gp, "S", mS));
+      TRY(maps_of(gp, "V", mV));
+      const StepLaunch sS = step(gp, mS), sV = step(gp, mV);
+      for (int it = 0; it <= nSteps - 2; it++) {
+        const bool smp = it % sample_every == 0;
+        if (ckpt && it > 0 && it % K == 0) TRY(snapshot(gp, sw.r0, it / K, true));
+        d0->launch("stress_fwd", smp, [&] { fwd_stress(sS, it, Uf); });
+        d0->launch("vel_fwd", smp, [&] { fwd_vel(sV, it, Uf); });
+      }
+    }
+    HIP_TRY(hipGetLastError());
+    // ------------------------------------------------------ residual
+    if (if_res) {
+      for (const Member& m : M) {
+        DeviceState& d = *m.d;
+        const int sa = shots_in(m, sw.r0), sb = shots_in(m, sw.r1);
+        if (sb <= sa) continue;
+        if (d.cond) {
+          TRY(condition_residual(d, m.plan->para, m.plan->para.save_scratch && withAdj, sa, sb,
+                                 st));
+        } else {
+          const int r0 = d.roff[sa], nr = d.roff[sb] - r0;
+          hipLaunchKernelGGL(k_residual, dim3(kPartials), dim3(256), 0, st, nSteps, d.R, r0, nr,
+                             d.win2, d.obs, d.syn, d.res, d.partial + j * kPartials);
+        }
+      }
+      HIP_TRY(hipGetLastError());
+    }
+    if (!withAdj) continue;
+    // ------------------------------------------------------ backward
+    for (size_t q = 0; q < BG[j].size(); q++) {
+      const GroupParts& gp = BG[j][q];
+      for (size_t p = 0; p < gp.parts.size(); p++) {
+        DeviceState& d = *M[gp.member[p]].d;
+        const int s0 = gp.parts[p].g.s0, n = gp.parts[p].g.ns;
+        TRY(d.zero_shots(d.adj, 5, s0, n, st));
+        TRY(d.zero_shots(ckpt ? d.madj : d.mem, 8, s0, n, st));
+      }
+      DevMap mA, mB, mAV, mAS, mRV, mRS, mRW;
+      TRY(maps_of(gp, "A", mA));
+      TRY(maps_of(gp, "B", mB));
+      TRY(maps_of(gp, "AV", mAV));
+      TRY(maps_of(gp, "AS", mAS));
+      TRY(maps_of(gp, "RV", mRV));
+      const StepLaunch sA = step(gp, mA), sB = step(gp, mB), sAV = step(gp, mAV),
+                       sAS = step(gp, mAS), sRV = step(gp, mRV);
+      StepLaunch sRS, sRW;
+      if (ckpt) {
+        TRY(maps_of(RG[j][q], "S", mRS));
+        TRY(maps_of(RG[j][q], "W", mRW));
+        sRS = step(RG[j][q], mRS);
+        sRW = step(RG[j][q], mRW);
+      }
+      // residual injection of step t for plans without the folded injection
+      auto inject = [&](int t) {
+        for (size_t p = 0; p < gp.parts.size(); p++) {
+          DeviceState& d = *M[gp.member[p]].d;
+          if (d.rec_at) continue;
+          const int s0 = gp.parts[p].g.s0, s1 = s0 + gp.parts[p].g.ns;
+          const int c0 = d.coff[s0], nc = d.coff[s1] - c0;
+          if (nc > 0)
+            hipLaunchKernelGGL(k_res_inject, dim3((nc + 255) / 256), dim3(256), 0, st, d.g,
+                               d.wave(d.adj), d.icell, d.iptr, d.irid, c0, nc,
+                               d.res + (size_t)t * d.R);
+        }
+      };
+      // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
+      bwd_a(sAV, nSteps - 1, Ub);
+      inject(nSteps - 1);
+      if (nSteps < 2) {
+        bwd_b(sAS, nSteps - 1, Ub);
+        continue;
+      }
+      const int seg = ckpt ? K : steps, nseg = (steps + seg - 1) / seg;
+      for (int jj = nseg - 1; jj >= 0; jj--) {
+        const int t0 = jj * seg, t1 = std::min(t0 + seg, steps);
+        if (ckpt) {
+          // the segment's forward state from its checkpoint, then its ring
+          if (jj == 0) {
+            for (size_t p = 0; p < gp.parts.size(); p++) {
+              DeviceState& d = *M[gp.member[p]].d;
+              TRY(d.zero_shots(d.fwd, 5, gp.parts[p].g.s0, gp.parts[p].g.ns, st));
+              TRY(d.zero_shots(d.mem, 8, gp.parts[p].g.s0, gp.parts[p].g.ns, st));
+            }
+          } else {
+            TRY(snapshot(RG[j][q], sw.r0, jj, false));
+          }
+          for (int it = t0; it < t1; it++) {
+            fwd_stress(sRS, it, Uf);
+            fwd_vel(sRW, it, Uf);
+          }
+        }
+        if (jj == nseg - 1)
+          bwd_b(sB, nSteps - 1, Ub);  // adj_stress(nSteps-1) + rev_vel(nSteps-2)
+        else
+          bwd_b(sRV, t1, Ub);         // rev_vel(t1-1)
+        for (int it = t1 - 1; it >= t0; it--) {
+          const bool smp = it % sample_every == 0;
+          d0->launch("bwd_pair_a", smp, [&] { bwd_a(sA, it, Ub); });
+          inject(it);
+          if (it > t0)
+            d0->launch("bwd_pair_b", smp, [&] { bwd_b(sB, it, Ub); });
+          else
+            bwd_b(sAS, it, Ub);
+        }
+      }
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  // misfit and gradients
+  for (const Member& m : M) {
+    DeviceState& d = *m.d;
+    if (if_res) {
+      if (d.cond)
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d.cpart,
+                           d.g.ns * kCondBlocks, d.misfit);
+      else
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d.partial,
+                           (int)sweeps.size() * kPartials, d.misfit);
+    }
+    if (withAdj)
+      hipLaunchKernelGGL(k_finalize, dim3((d.g.nz + 255) / 256, d.g.nx), dim3(256), 0, st, d.g,
+                         d.mdl(), d.acc, d.grad);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  if (d0->timing) d0->collect_timing();
+  for (const Member& m : M) {
+    DeviceState& d = *m.d;
+    if (if_res) HIP_TRY(hipMemcpy(&d.misfit_host, d.misfit, sizeof(double), hipMemcpyDeviceToHost));
+    m.plan->last_calc = calc_id;
+    d.illum_valid = d.illum_on;
+  }
+  return Status::Ok();
+}
+
+}  // namespace
 
 Status Plan::run(int calc_id) {
   TRY(check_calc_id(calc_id));
   if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
-  DeviceState& d = *d_;
-  if (!d.model_set) return Status::Err(FWIFLOW_ERR_ARG, "model not set");
-  if (!d.stf_set) return Status::Err(FWIFLOW_ERR_ARG, "source time functions not set");
-  const bool if_res = calc_id != 2, withAdj = calc_id == 1;
-  if (if_res && !d.obs_set) return Status::Err(FWIFLOW_ERR_ARG, "observed data not set");
-  d.illum_valid = false;
-  last_calc = -1;
-  HIP_TRY(hipSetDevice(d.dev));
-  const Geo& g = d.g;
-  const size_t N = (size_t)g.N, F = (size_t)g.ns * N;
-  const size_t T = (size_t)g.nSteps * d.R;
-  if (withAdj) {
-    if (!d.adj) TRY(d.alloc(&d.adj, 5 * F));
-    if (d.fused && !d.adj2) TRY(d.alloc(&d.adj2, 5 * F));
-    if (!d.ring) TRY(d.alloc(&d.ring, (size_t)g.nSteps * 5 * g.ns * (size_t)g.ringlen));
-    if (!d.acc) TRY(d.alloc(&d.acc, (size_t)(d.streams == 2 ? 10 : 5) * N));
-    if (!d.grad) TRY(d.alloc(&d.grad, 3 * N));
-    if (!d.stf_grad) TRY(d.alloc(&d.stf_grad, (size_t)g.ns * g.nSteps));
+  HIP_TRY(hipSetDevice(d_->dev));
+  if (!d_->cache) {
+    d_->cache.reset(new JobCache());
+    d_->cache->dev = d_->dev;
   }
-  // interior checkpoints: slot it/k for it = k, 2k, .. <= nSteps-2 (slot 0 unused)
-  const int ck_k = withAdj ? d.ckpt_every : 0;
-  const int ck_slots = ck_k > 0 ? (g.nSteps - 2) / ck_k + 1 : 0;
-  const size_t ck_cells = (size_t)(g.nzA - 2 * g.nPml) * (g.nx - 2 * g.nPml);
-  if (ck_k > 0) {
-    const size_t need = (size_t)g.ns * ck_slots * 5 * ck_cells;
-    if (d.ck_floats < need) {
-      TRY(d.alloc(&d.ck, need));  // a smaller earlier buffer stays in the plan's pool
-      d.ck_floats = need;
-    }
-  }
-  const Tiling tI = make_tiling(g.nzA - 2 * g.nPml, g.nx - 2 * g.nPml, d.tile_x, d.xcd);
-  hipStream_t st = d.stream;
-  HIP_TRY(hipMemsetAsync(d.fwd, 0, 5 * F * sizeof(float), st));
-  HIP_TRY(hipMemsetAsync(d.mem, 0, 8 * F * sizeof(float), st));
-  if (d.fused || d.tiled) {
-    HIP_TRY(hipMemsetAsync(d.fwd2, 0, 5 * F * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(d.mem2, 0, 8 * F * sizeof(float), st));
-  }
-  Tiling tT;
-  tT.gz = (g.nzA + TT - 1) / TT;
-  tT.gx = (g.nx + TT - 1) / TT;
-  tT.n = tT.gz * tT.gx;
-  tT.xcd = d.xcd ? 1 : 0;
-  HIP_TRY(hipMemsetAsync(d.syn, 0, T * sizeof(float), st));
-  float* il_ptr = nullptr;
-  if (d.illum_on) {
-    if (d.fused || d.tiled)
-      return Status::Err(FWIFLOW_ERR_UNSUPPORTED,
-                         "illumination is summed by the split forward kernels only");
-    if (!d.illum) TRY(d.alloc(&d.illum, N));
-    HIP_TRY(hipMemsetAsync(d.illum, 0, N * sizeof(float), st));
-    il_ptr = d.illum;
-  }
+  Runner r({Member{this, d_.get()}}, *d_->cache);
+  return r.run(calc_id);
+}
 
-  const Mdl m = d.mdl();
-  const Pml c = d.pml_ptrs();
-  float* fa = d.fwd;   // state n
-  float* fb = d.fwd2;  // state n+1 (fused path)
-  float* ma = d.mem;
-  float* mb = d.mem2;
-  const Src src{d.src, d.inj};
-  const dim3 blk(kBZ, d.tile_x);
-  Tiling tU = make_tiling(g.nzA, g.nx, d.tile_x, d.xcd);
-  tU.perm = d.perm_u;
-  const int recBlocks = (d.R + 255) / 256;
-  const int sample_every = 8;
-  Seg q;
-  q.tx = d.seg_x;
-  q.nseg = (g.nx + q.tx - 1) / q.tx;
-  q.n = q.nseg * g.ns;
-  q.xcd = d.xcd ? 1 : 0;
-  const int NTf = fused_threads(g);
-  const size_t ldsf = (size_t)10 * NTf * sizeof(float);
-
-  // ---------------------------------------------------------- forward
-  // The split and tiled paths take the shots in groups of `shot_group`: a
-  // group runs every time step before the next group starts, so the state of
-  // one group stays resident in the last-level cache from step to step.
-  const int G = d.group_size();
-  const int U = d.unroll;
-  // two streams for the split kernels (the illumination has one writer per cell)
-  const bool two = d.streams == 2 && !d.fused && !d.tiled && !d.illum_on;
-#define LAUNCH_U(K, grid, block, ...)                                        \
-  do {                                                                       \
-    if (U == 4)                                                              \
-      hipLaunchKernelGGL(K<4>, grid, block, 0, st, __VA_ARGS__);             \
-    else if (U == 2)                                                         \
-      hipLaunchKernelGGL(K<2>, grid, block, 0, st, __VA_ARGS__);             \
-    else                                                                     \
-      hipLaunchKernelGGL(K<1>, grid, block, 0, st, __VA_ARGS__);             \
-  } while (0)
-  if (d.fused) {
-    for (int it = 0; it <= g.nSteps - 2; it++) {
-      const bool smp = (it % sample_every) == 0;
-      float* ring = withAdj ? d.ring : nullptr;
-      const Wave wa = d.wave(fa), wb = d.wave(fb);
-      const Mem pa = d.memv(ma), pb = d.memv(mb);
-      d.launch("fwd_step", smp, [&] {
-        hipLaunchKernelGGL(k_fwd_step, dim3(q.n), dim3(NTf), ldsf, st, q, g, m, c, wa, wb, pa, pb,
-                           src, it, ring);
-      });
-      hipLaunchKernelGGL(k_record, dim3(recBlocks), dim3(256), 0, st, g, wb.szz, wb.sxx, d.rec,
-                         d.R, d.syn + (size_t)(it + 1) * d.R);
-      std::swap(fa, fb);
-      std::swap(ma, mb);
-    }
-  } else {
-    float* fa0 = fa;
-    float* ma0 = ma;
-    // ceil(ns / G) groups of balanced sizes (64 shots by 3: 20 x 3 + 2 x 2)
-    const int NGf = (g.ns + G - 1) / G;
-    const int U = d.unroll_fwd;  // LAUNCH_U below
-    if (two) {
-      HIP_TRY(hipEventRecord(d.join[0], st));
-      HIP_TRY(hipStreamWaitEvent(d.stream2, d.join[0], 0));
-    }
-    for (int k = 0; k < NGf; k++) {
-      const bool second = two && (k & 1);
-      hipStream_t st = second ? d.stream2 : d.stream;  // the launches below use it
-      const int s0 = (int)((long long)k * g.ns / NGf);
-      const int n = (int)((long long)(k + 1) * g.ns / NGf) - s0;
-      const Geo gg = d.group_geo(s0, n);
-      Tiling tUg = tU;  // forward/adjoint split kernels: shots spread over chunks
-      // (one chunk when the illumination is summed: one writer per cell)
-      tUg.spt = d.spt > 0 && !d.illum_on ? std::min(d.spt, n) : n;
-      tUg.chunks = (n + tUg.spt - 1) / tUg.spt;
-      const dim3 grdUg(tUg.n * tUg.chunks);
-      const Src sg = d.src_view(s0);
-      float* ring = withAdj ? d.ring_view(s0) : nullptr;
-      const int r0 = d.roff[s0], Rg = d.roff[s0 + n] - r0;
-      const dim3 rgrid((Rg + 255) / 256);
-      fa = fa0, fb = d.fwd2, ma = ma0, mb = d.mem2;
-      float* ckg = ck_k > 0 ? d.ck + (size_t)s0 * ck_slots * 5 * ck_cells : nullptr;
-      for (int it = 0; it <= g.nSteps - 2; it++) {
-        const bool smp = !second && (it % sample_every) == 0;  // events on the first stream
-        float* out = fa;  // buffer that holds step it+1 after this step
-        if (ckg && it > 0 && it % ck_k == 0)
-          hipLaunchKernelGGL(k_ckpt, dim3(tI.n), blk, 0, st, tI, gg, d.wave(fa, s0), ckg,
-                             it / ck_k, ck_slots, 1);
-        if (d.tiled) {
-          const Wave wa = d.wave(fa, s0), wb = d.wave(fb, s0);
-          const Mem pa = d.memv(ma, s0), pb = d.memv(mb, s0);
-          d.launch("fwd_tile", smp, [&] {
-            hipLaunchKernelGGL(k_fwd_tile, dim3(tT.n), dim3(1024), 0, st, tT, gg, m, c, wa, wb,
-                               pa, pb, sg, it, ring);
-          });
-          out = fb;
-          std::swap(fa, fb);
-          std::swap(ma, mb);
-        } else {
-          const Wave w = d.wave(fa, s0);
-          const Mem p = d.memv(ma, s0);
-          d.launch("stress_fwd", smp, [&] {
-            LAUNCH_U(k_stress_fwd, grdUg, blk, tUg, gg, m, c, w, p, sg, it, ring);
-          });
-          // velocity update with this step's recording appended to its grid
-          const int nV = (tUg.n * tUg.chunks + 7) / 8 * 8;
-          const Wave wo = d.wave(fa);
-          float* row = d.syn + (size_t)(it + 1) * d.R + r0;
-          d.launch("vel_fwd", smp, [&] {
-            const int per = (int)(blk.x * blk.y);
-            LAUNCH_U(k_vel_fwd_rec, dim3(nV + (Rg + per - 1) / per), blk, nV, tUg, gg, m, c, w, p,
-                     il_ptr, g, wo.szz, wo.sxx, d.rec + 3 * r0, Rg, row);
-          });
-        }
-        if (Rg > 0 && d.tiled) {
-          const Wave wo = d.wave(out);
-          hipLaunchKernelGGL(k_record, rgrid, dim3(256), 0, st, g, wo.szz, wo.sxx, d.rec + 3 * r0,
-                             Rg, d.syn + (size_t)(it + 1) * d.R + r0);
-        }
-      }
-    }
-    if (two) {
-      HIP_TRY(hipEventRecord(d.join[1], d.stream2));
-      HIP_TRY(hipStreamWaitEvent(st, d.join[1], 0));
-    }
+Job::Job() = default;
+Job::~Job() {
+  if (cache_) {
+    (void)hipSetDevice(cache_->dev);
+    cache_.reset();
   }
-  HIP_TRY(hipGetLastError());
-  const Wave w = d.wave(fa);  // final forward state
+}
 
-  d.src_updated = false;
-  if (if_res && d.cond) {
-    TRY(condition_residual(d, para, para.save_scratch && withAdj));
-    d.src_updated = para.if_src_update;
-  } else if (if_res) {
-    hipLaunchKernelGGL(k_residual, dim3(d.npart), dim3(256), 0, st, g.nSteps, d.R, d.win2, d.obs,
-                       d.syn, d.res, d.partial);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, d.partial, d.npart, d.misfit);
-    HIP_TRY(hipGetLastError());
+Status Job::add(Plan* p) {
+  if (!p || !p->d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if ((int)plans_.size() >= kMaxPartsTable)
+    return Status::Err(FWIFLOW_ERR_ARG, "at most " + std::to_string(kMaxPartsTable) + " plans per job");
+  for (Plan* q : plans_)
+    if (q == p) return Status::Err(FWIFLOW_ERR_ARG, "plan already in the job");
+  if (!plans_.empty()) {
+    const DeviceState& a = *plans_[0]->d_;
+    if (p->d_->dev != a.dev) return Status::Err(FWIFLOW_ERR_ARG, "plans of a job must share the GPU");
+    if (p->d_->g.nSteps != a.g.nSteps)
+      return Status::Err(FWIFLOW_ERR_ARG, "plans of a job must share nSteps");
   }
-  // the reversal subtracts the source rows the residual stage left: updated
-  // ones after a source update (libCUFD.cu:326-329, 390-394)
-  const float* inj_b = d.src_updated ? d.inj_rev : d.inj;
-  const Src srcb{d.src, inj_b};
-
-  if (withAdj) {
-    // ------------------------------------------------------- backward
-    HIP_TRY(hipMemsetAsync(d.adj, 0, 5 * F * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(ma, 0, 8 * F * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(d.acc, 0, (two ? 10 : 5) * N * sizeof(float), st));
-    HIP_TRY(hipMemsetAsync(d.stf_grad, 0, (size_t)g.ns * g.nSteps * sizeof(float), st));
-    const Tiling tR = make_tiling(g.ringH, g.ringW, d.tile_x, d.xcd);
-    const dim3 grdR(tR.n);
-    float* acc = d.acc;
-    if (d.fused) {
-      HIP_TRY(hipMemsetAsync(d.adj2, 0, 5 * F * sizeof(float), st));
-      HIP_TRY(hipMemsetAsync(mb, 0, 8 * F * sizeof(float), st));
-      float* aa = d.adj;  // adjoint state before the step
-      float* ab = d.adj2;
-      float* qa = ma;     // adjoint memory variables
-      float* qb = mb;
-      const RecIdx rx{d.rptr, d.rzr, d.rrg};
-      const size_t ldsa = (size_t)18 * NTf * sizeof(float);
-      auto adj_step = [&](int res_t, int stf_it) {
-        hipLaunchKernelGGL(k_adj_step, dim3(q.n), dim3(NTf), ldsa, st, q, g, m, c, d.wave(aa),
-                           d.wave(ab), d.memv(qa), d.memv(qb), srcb, rx,
-                           d.res + (size_t)res_t * d.R, d.R, d.stf_grad, stf_it);
-        std::swap(aa, ab);
-        std::swap(qa, qb);
-      };
-      SegG qr;
-      qr.nseg = (g.nx + TXR - 1) / TXR;
-      qr.gsz = d.rev_group;
-      qr.ngrp = (g.ns + qr.gsz - 1) / qr.gsz;
-      qr.n = qr.nseg * qr.ngrp;
-      qr.xcd = d.xcd ? 1 : 0;
-      if (d.accp_groups < qr.ngrp) {
-        TRY(d.alloc(&d.accp, (size_t)qr.ngrp * 5 * N));
-        d.accp_groups = qr.ngrp;
-      }
-      HIP_TRY(hipMemsetAsync(d.accp, 0, (size_t)qr.ngrp * 5 * N * sizeof(float), st));
-      const size_t ldsr = (size_t)(10 + 5 * TXR) * NTf * sizeof(float);
-      float* ra = fa;  // forward state at step it+1
-      float* rb = fb;  // forward state at step it
-      // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
-      adj_step(g.nSteps - 1, -1);
-      for (int it = g.nSteps - 2; it >= 0; it--) {
-        const bool smp = (it % sample_every) == 0;
-        const Wave a = d.wave(aa);
-        if (d.fused_rev) {
-          const Wave wa = d.wave(ra), wb = d.wave(rb);
-          d.launch("rev_step", smp, [&] {
-            hipLaunchKernelGGL(k_rev_step, dim3(qr.n), dim3(NTf), ldsr, st, qr, g, m, wa, wb, a,
-                               srcb, d.ring, it, d.accp);
-          });
-          std::swap(ra, rb);
-        } else {
-          d.launch("rev_vel", smp, [&] {
-            hipLaunchKernelGGL(k_rev_vel<1>, grdR, blk, 0, st, tR, g, m, w, a, d.accp + 3 * N,
-                               d.accp + 4 * N, d.ring, it);
-          });
-          d.launch("rev_stress", smp, [&] {
-            hipLaunchKernelGGL(k_rev_stress<1>, grdR, blk, 0, st, tR, g, m, w, a, src, d.accp,
-                               d.accp + N, d.accp + 2 * N, d.ring, it);
-          });
-        }
-        d.launch("adj_step", smp, [&] { adj_step(it, it); });
-      }
-      const long long n5 = 5 * (long long)N;
-      hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)((n5 + 255) / 256)), dim3(256), 0, st,
-                         d.accp, n5, qr.ngrp);
-      acc = d.accp;
-    } else {
-      const Wave a_all = d.wave(d.adj);
-      const int GB = d.group_size_bwd();
-      const int NGb = (g.ns + GB - 1) / GB;
-      if (two) {
-        HIP_TRY(hipEventRecord(d.join[0], st));
-        HIP_TRY(hipStreamWaitEvent(d.stream2, d.join[0], 0));
-      }
-      float* const acc0 = acc;
-      for (int k = 0; k < NGb; k++) {
-        const bool second = two && (k & 1);
-        hipStream_t st = second ? d.stream2 : d.stream;  // the launches below use it
-        float* acc = second ? acc0 + 5 * N : acc0;        // and these accumulators
-        const int s0 = (int)((long long)k * g.ns / NGb);
-        const int n = (int)((long long)(k + 1) * g.ns / NGb) - s0;
-        const Geo gg = d.group_geo(s0, n);
-        Tiling tUg = tU;  // forward/adjoint split kernels: shots spread over chunks
-        tUg.spt = d.spt > 0 ? std::min(d.spt, n) : n;
-        tUg.chunks = (n + tUg.spt - 1) / tUg.spt;
-        const dim3 grdUg(tUg.n * tUg.chunks);
-        const Src sg = d.src_view(s0, inj_b);
-        const float* ring = d.ring_view(s0);
-        const int r0 = d.roff[s0], Rg = d.roff[s0 + n] - r0;
-        const dim3 rgrid((Rg + 255) / 256);
-        const Wave wg = d.wave(fa, s0);  // final forward state of the group
-        const Wave a = d.wave(d.adj, s0);
-        const Mem pg = d.memv(ma, s0);   // adjoint memory variables
-        float* sgrad = d.stf_grad + (size_t)s0 * g.nSteps;
-        float* ckg = ck_k > 0 ? d.ck + (size_t)s0 * ck_slots * 5 * ck_cells : nullptr;
-        // after reverse step it the group's state is the one before forward step it
-        auto reset = [&](int it) {
-          if (ckg && it > 0 && it % ck_k == 0)
-            hipLaunchKernelGGL(k_ckpt, dim3(tI.n), blk, 0, st, tI, gg, wg, ckg, it / ck_k,
-                               ck_slots, 0);
-        };
-        auto inject = [&](int t) {
-          if (Rg > 0)
-            hipLaunchKernelGGL(k_res_inject, rgrid, dim3(256), 0, st, g, a_all.szz, a_all.sxx,
-                               d.rec + 3 * r0, Rg, d.res + (size_t)t * d.R + r0);
-        };
-        // with backward pairs the residual of step t is injected by the
-        // adjoint stress update of step t itself (k_bwd_pair_b, k_adj_stress)
-        const bool fold = d.fold_inject && d.rec_at && d.pairs && g.nSteps >= 2;
-        auto inj_at = [&](int t) {
-          return Inj{fold ? d.res + (size_t)t * d.R : nullptr, d.rec_at, d.rec_row};
-        };
-        const Inj none{nullptr, d.rec_at, -1};
-        // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
-        LAUNCH_U(k_adj_vel, grdUg, blk, tUg, gg, m, c, a, pg, sg, sgrad, -1);
-        if (!fold) inject(g.nSteps - 1);
-        if (d.pairs && g.nSteps >= 2) {
-          const int nU8 = (tUg.n * tUg.chunks + 7) / 8 * 8, nR8 = (tR.n + 7) / 8 * 8;
-          const dim3 grdP(nR8 + nU8);
-          const int mix = d.mix ? 1 : 0, mix_b = d.mix_b ? 1 : 0;
-          // adj_stress(nSteps-1) with rev_vel(nSteps-2)
-          LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix_b, tUg, tR, gg, m, c, wg, a, pg, acc, ring,
-                   g.nSteps - 2, inj_at(g.nSteps - 1));
-          for (int it = g.nSteps - 2; it >= 0; it--) {
-            const bool smp = !second && (it % sample_every) == 0;
-            d.launch("bwd_pair_a", smp, [&] {
-              LAUNCH_U(k_bwd_pair_a, grdP, blk, nR8, nU8, mix, tR, tUg, gg, m, c, wg, a, pg, sg, acc,
-                       ring, sgrad, it);
-            });
-            if (!fold) inject(it);
-            reset(it);
-            if (it >= 1) {
-              d.launch("bwd_pair_b", smp, [&] {
-                LAUNCH_U(k_bwd_pair_b, grdP, blk, nU8, nR8, mix_b, tUg, tR, gg, m, c, wg, a, pg, acc,
-                         ring, it - 1, inj_at(it));
-              });
-            } else {
-              LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg, inj_at(0));
-            }
-          }
-          continue;
-        }
-        LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg, none);
-        for (int it = g.nSteps - 2; it >= 0; it--) {
-          const bool smp = !second && (it % sample_every) == 0;
-          d.launch("rev_vel", smp, [&] {
-            LAUNCH_U(k_rev_vel, grdR, blk, tR, gg, m, wg, a, acc + 3 * N, acc + 4 * N, ring, it);
-          });
-          d.launch("rev_stress", smp, [&] {
-            LAUNCH_U(k_rev_stress, grdR, blk, tR, gg, m, wg, a, sg, acc, acc + N, acc + 2 * N,
-                     ring, it);
-          });
-          reset(it);
-          d.launch("adj_vel", smp, [&] {
-            LAUNCH_U(k_adj_vel, grdUg, blk, tUg, gg, m, c, a, pg, sg, sgrad, it);
-          });
-          inject(it);
-          d.launch("adj_stress", smp, [&] {
-            LAUNCH_U(k_adj_stress, grdUg, blk, tUg, gg, m, c, a, pg, none);
-          });
-        }
-      }
-      if (two) {
-        HIP_TRY(hipEventRecord(d.join[1], d.stream2));
-        HIP_TRY(hipStreamWaitEvent(st, d.join[1], 0));
-        const long long n5 = 5 * (long long)N;
-        hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)((n5 + 255) / 256)), dim3(256), 0, st,
-                           acc0, n5, 2);
-      }
-    }
-#undef LAUNCH_U
-    const Tiling tA = make_tiling(g.nz, g.nx, d.tile_x, d.xcd);
-    hipLaunchKernelGGL(k_finalize, dim3(tA.n), blk, 0, st, tA, g, m, acc, d.grad);
-    HIP_TRY(hipGetLastError());
-  }
-  HIP_TRY(hipStreamSynchronize(st));
-  if (d.timing) d.collect_timing();
-  if (if_res) HIP_TRY(hipMemcpy(&d.misfit_host, d.misfit, sizeof(double), hipMemcpyDeviceToHost));
-  last_calc = calc_id;
-  d.illum_valid = d.illum_on;
+  plans_.push_back(p);
+  cache_.reset();  // maps depend on the members
   return Status::Ok();
 }
+
+Status Job::run(int calc_id) {
+  if (plans_.empty()) return Status::Err(FWIFLOW_ERR_ARG, "empty job");
+  HIP_TRY(hipSetDevice(plans_[0]->d_->dev));
+  if (!cache_) {
+    cache_.reset(new JobCache());
+    cache_->dev = plans_[0]->d_->dev;
+  }
+  std::vector<Member> m;
+  for (Plan* p : plans_) m.push_back(Member{p, p->d_.get()});
+  Runner r(std::move(m), *cache_);
+  return r.run(calc_id);
+}
+
+int Job::size() const { return (int)plans_.size(); }
 
 Status Plan::misfit(double* out) {
   if (!d_ || (last_calc != 0 && last_calc != 1))

@@ -11,6 +11,8 @@
 namespace fwiflow {
 
 struct DeviceState;  // defined in engine.hip
+struct JobCache;     // defined in engine.hip
+class Job;
 
 class Plan {
  public:
@@ -29,12 +31,21 @@ class Plan {
   Status source(float* rows);   // source rows [group][nSteps] after the last run
   Status write_data();
   Status write_scratch();
+  // backward boundary store: 0 automatic (the reference's ring of every step
+  // when the sweep's ring fits the memory budget, else checkpoints at the
+  // interval that minimises memory); k > 0 checkpoints every k steps
   Status set_checkpoint(int k);
+  Status set_memory_budget(double bytes);  // ring + checkpoints of a gradient run
+  // the last gradient run's checkpoint interval (0: ring of every step), shots
+  // per sweep, and device bytes of the ring and of the checkpoints
+  Status memory_plan(int* k, int* shots_per_sweep, long long* ring_bytes,
+                     long long* checkpoint_bytes) const;
   Status set_timing(bool on);
   Status timing(const std::string& name, double* ms, long long* count);
   long long device_bytes() const;
-  int shots_per_launch() const;      // forward sweep
+  int shots_per_launch() const;      // forward sweep (largest group of the last run)
   int shots_per_launch_bwd() const;  // backward sweep
+  int launch_groups(int backward) const;  // shot groups of the last run
   Status set_illumination(bool on);
   Status illumination(double* out);
 
@@ -45,7 +56,23 @@ class Plan {
   float cp_max = 0.f;
 
  private:
+  friend class Job;
   std::unique_ptr<DeviceState> d_;
+};
+
+// Several plans (different grids or surveys on one GPU, same nSteps) run
+// together: every time-step launch covers a group of shots of every plan.
+class Job {
+ public:
+  Job();
+  ~Job();
+  Status add(Plan* p);
+  Status run(int calc_id);
+  int size() const;
+
+ private:
+  std::vector<Plan*> plans_;
+  std::unique_ptr<JobCache> cache_;
 };
 
 }  // namespace fwiflow

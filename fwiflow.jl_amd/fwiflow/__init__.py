@@ -5,7 +5,7 @@ hand-written HIP kernels behind the C ABI of ``include/fwiflow.h``
 (``lib/libfwiflow_hip.so``); this package is the host-side mirror of the
 reference's Julia API (src/Core.jl, src/FWI.jl, src/Utils.jl).
 """
-from .ops import Plan, check_call, cufd, fwi_obs_op, fwi_op, fwi_op_grad, host_tables
+from .ops import Job, Plan, check_call, cufd, fwi_obs_op, fwi_op, fwi_op_grad, host_tables
 from .utils import (moduli_to_velocity_grad, npad, padding, padding_adjoint, paraGen,
                     sourceGene, surveyGen, velocity_to_moduli)
 from .fwi import (FWI, FWIExample, compute_misfit, compute_misfit_and_gradient,
@@ -15,7 +15,7 @@ from .precond import CNN, learned_descent, learned_lbfgs
 from ._lib import FwiflowError, version
 
 __all__ = [
-    "Plan", "check_call", "cufd", "fwi_obs_op", "fwi_op", "fwi_op_grad", "host_tables",
+    "Job", "Plan", "check_call", "cufd", "fwi_obs_op", "fwi_op", "fwi_op_grad", "host_tables",
     "moduli_to_velocity_grad", "npad", "padding", "padding_adjoint", "paraGen", "sourceGene",
     "surveyGen", "velocity_to_moduli", "FWI", "FWIExample", "compute_misfit",
     "compute_misfit_and_gradient", "compute_observation", "make_fwi", "FwiflowError",

@@ -32,6 +32,8 @@ EXPORTS = [
     "fwiflow_plan_dims", "fwiflow_plan_set_checkpoint", "fwiflow_plan_set_timing",
     "fwiflow_plan_timing", "fwiflow_plan_device_bytes", "fwiflow_plan_shots_per_launch",
     "fwiflow_plan_set_illumination", "fwiflow_plan_illumination",
+    "fwiflow_plan_launch_groups", "fwiflow_plan_set_memory_budget", "fwiflow_plan_memory_plan",
+    "fwiflow_job_create", "fwiflow_job_destroy", "fwiflow_job_add", "fwiflow_job_run",
     "fwiflow_cnn_create", "fwiflow_cnn_destroy", "fwiflow_cnn_num_params",
     "fwiflow_cnn_get_params", "fwiflow_cnn_set_params", "fwiflow_cnn_forward",
     "fwiflow_cnn_gradients", "fwiflow_cnn_train_step",
@@ -88,6 +90,14 @@ def load():
     L.fwiflow_plan_shots_per_launch.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fwiflow_plan_set_illumination.argtypes = [vp, C.c_int]
     L.fwiflow_plan_illumination.argtypes = [vp, dp]
+    L.fwiflow_plan_launch_groups.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fwiflow_plan_set_memory_budget.argtypes = [vp, C.c_double]
+    L.fwiflow_plan_memory_plan.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.fwiflow_job_create.argtypes = [C.POINTER(vp)]
+    L.fwiflow_job_destroy.argtypes = [vp]
+    L.fwiflow_job_add.argtypes = [vp, vp]
+    L.fwiflow_job_run.argtypes = [vp, C.c_int]
     L.fwiflow_cnn_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint]
     L.fwiflow_cnn_destroy.argtypes = [vp]
     L.fwiflow_cnn_num_params.argtypes = [vp, ip]

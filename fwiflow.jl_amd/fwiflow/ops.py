@@ -208,10 +208,69 @@ class Plan:
         return out
 
     def shots_per_launch(self):
-        """(forward, backward) shots per launch of the time-step kernels."""
+        """(forward, backward) shots of the largest launch group of the last run."""
         f, b = _lib.C.c_int(0), _lib.C.c_int(0)
         check(self._L.fwiflow_plan_shots_per_launch(self._h, _lib.C.byref(f), _lib.C.byref(b)))
         return int(f.value), int(b.value)
+
+    def launch_groups(self):
+        """(forward, backward) shot groups, i.e. launches per time step, of the last run."""
+        f, b = _lib.C.c_int(0), _lib.C.c_int(0)
+        check(self._L.fwiflow_plan_launch_groups(self._h, _lib.C.byref(f), _lib.C.byref(b)))
+        return int(f.value), int(b.value)
+
+    def set_memory_budget(self, nbytes):
+        """Device bytes a gradient run may hold for the boundary ring and checkpoints."""
+        check(self._L.fwiflow_plan_set_memory_budget(self._h, float(nbytes)))
+
+    def memory_plan(self):
+        """(checkpoint interval, shots per sweep, ring bytes, checkpoint bytes) of
+        the last gradient run; interval 0 is the ring of every step."""
+        k, s = _lib.C.c_int(0), _lib.C.c_int(0)
+        rb, cb = _lib.C.c_longlong(0), _lib.C.c_longlong(0)
+        check(self._L.fwiflow_plan_memory_plan(self._h, _lib.C.byref(k), _lib.C.byref(s),
+                                               _lib.C.byref(rb), _lib.C.byref(cb)))
+        return int(k.value), int(s.value), int(rb.value), int(cb.value)
+
+
+class Job:
+    """Several plans run together (fwiflow_job_* in include/fwiflow.h): every
+    time-step launch covers a group of shots of every plan, e.g. baseline and
+    monitor surveys on grids of different sizes.  The plans must share the GPU
+    and nSteps and must outlive the job; results are read from each plan."""
+
+    def __init__(self, plans=()):
+        L = _lib.load()
+        self._L = L
+        self._h = _lib.C.c_void_p()
+        self.plans = []
+        check(L.fwiflow_job_create(_lib.C.byref(self._h)))
+        for p in plans:
+            self.add(p)
+
+    def add(self, plan):
+        check(self._L.fwiflow_job_add(self._h, plan._h))
+        self.plans.append(plan)
+
+    def run(self, calc_id):
+        check(self._L.fwiflow_job_run(self._h, int(calc_id)))
+
+    def close(self):
+        if self._h:
+            self._L.fwiflow_job_destroy(self._h)
+            self._h = _lib.C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 try:  # torch is plumbing for autodiff callers; the op itself does not need it

@@ -35,7 +35,9 @@ enum fwiflow_status {
   FWIFLOW_ERR_JSON = 3,        /* malformed JSON or missing/ill-typed key (Parameter.cpp:41-113) */
   FWIFLOW_ERR_COURANT = 4,     /* Courant number > 1 (utilities.cu:225-240) */
   FWIFLOW_ERR_SHAPE = 5,       /* shot id / geometry / source or receiver outside the grid */
-  FWIFLOW_ERR_UNSUPPORTED = 6, /* "filter", "if_src_update" or "if_win" requested */
+  FWIFLOW_ERR_UNSUPPORTED = 6, /* band-pass or source-update transform longer than 20480
+                                  points (2*nSteps) or with a prime factor above 61; a grid
+                                  spacing whose reciprocal division is not exact */
   FWIFLOW_ERR_HIP = 7,         /* HIP runtime error, including "no GPU" */
   FWIFLOW_ERR_ARG = 8,         /* null pointer or invalid argument */
   FWIFLOW_ERR_MEMORY = 9       /* device or pinned-host allocation failed */
@@ -82,10 +84,11 @@ const char* fwiflow_version(void);
 
 /* ------------------------------------------------------------------------
  * Persistent plan: parse the JSON files once, keep the model, observed data,
- * wavefields and boundary store resident in HBM, run all shots of the group
- * batched (one launch per kernel per time step covers every shot).  No
- * counterpart in the reference, which rebuilds everything on each call
- * (libCUFD.cu:51-145); fwiflow_cufd is implemented on top of it.
+ * wavefields and boundary store resident in HBM, run the shots of the group
+ * batched (one launch per kernel per time step covers a group of shots sized
+ * for the last-level cache).  No counterpart in the reference, which
+ * rebuilds everything on each call (libCUFD.cu:51-145) and runs one shot at
+ * a time; fwiflow_cufd is implemented on top of it.
  * ---------------------------------------------------------------------- */
 typedef struct fwiflow_plan fwiflow_plan;
 
@@ -131,13 +134,29 @@ int fwiflow_plan_source(fwiflow_plan* plan, float* rows);
 /* Geometry of the plan: dims[0..5] = NZ, NX, nSteps, group_size, total receivers,
  * nPml. */
 int fwiflow_plan_dims(fwiflow_plan* plan, int* dims);
-/* Backward scheme: ckpt_every = 0 reproduces the reference (boundary ring
- * every step + time-reversed re-materialisation); k > 0 additionally stores
- * the interior wavefield every k steps and resets the reverse sweep to it
- * there, which bounds the FP32 drift of the reversal.  Extra device memory:
- * shots x ((nSteps-2)/k + 1) x 5 x interior cells x 4 B.  No reference
- * counterpart (the reference's scheme is k = 0). */
+/* Boundary store of the backward sweep.  A gradient run takes the shots in
+ * sweeps: each sweep runs the forward of its shots, their residual and their
+ * backward before the next starts, so the store holds one sweep's shots.
+ * ckpt_every = 0 (default): automatic.  The reference's store (a 5-layer
+ * boundary ring of every step, Boundary.cu:17-41, and time-reversed
+ * re-materialisation of the forward wavefield) when one backward shot group's
+ * ring fits the memory budget; otherwise checkpoints at the interval k that
+ * minimises ring + checkpoint memory.  k > 0: checkpoints every k steps: the
+ * full forward state (5 wavefield components and 8 C-PML memory variables)
+ * before steps k, 2k, ..., plus the ring of one k-step segment, which the
+ * backward sweep recomputes from its checkpoint (one more forward sweep).
+ * The reverse sweep then starts every segment from the recomputed state
+ * instead of the reversed one.  No reference counterpart (the reference's
+ * scheme is the ring of every step for one shot at a time). */
 int fwiflow_plan_set_checkpoint(fwiflow_plan* plan, int ckpt_every);
+/* Device bytes a gradient run may hold for the boundary ring and the
+ * checkpoints (default 32e9).  It sets the shots per sweep. */
+int fwiflow_plan_set_memory_budget(fwiflow_plan* plan, double bytes);
+/* The memory plan of the last gradient run: checkpoint interval (0: ring of
+ * every step), shots per sweep, device bytes of the ring and of the
+ * checkpoints.  Any pointer may be NULL. */
+int fwiflow_plan_memory_plan(fwiflow_plan* plan, int* ckpt_every, int* shots_per_sweep,
+                             long long* ring_bytes, long long* checkpoint_bytes);
 /* Kernel timing with HIP events on the plan's stream (off by default).
  * fwiflow_plan_timing(name) returns the summed milliseconds and launch count
  * of kernel `name` since timing was enabled. */
@@ -151,6 +170,9 @@ int fwiflow_plan_device_bytes(fwiflow_plan* plan, long long* bytes);
  * may be NULL.  No reference counterpart: it sizes per-launch byte counts for
  * kernel timing. */
 int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* forward, int* backward);
+/* Number of shot groups (launches per time step) of the plan's last run in
+ * the forward and the backward sweep.  Either pointer may be NULL. */
+int fwiflow_plan_launch_groups(fwiflow_plan* plan, int* forward, int* backward);
 /* Forward illumination: with enable != 0, later runs also sum, over the
  * plan's shots and the steps it = 0..nSteps-2, vz^2 + vx^2 after each
  * velocity update on the update region [2, NZ-nPad-3] x [2, NX-3] (zero
@@ -159,6 +181,24 @@ int fwiflow_plan_shots_per_launch(fwiflow_plan* plan, int* forward, int* backwar
  * input of the inversion driver (gradient / illumination). */
 int fwiflow_plan_set_illumination(fwiflow_plan* plan, int enable);
 int fwiflow_plan_illumination(fwiflow_plan* plan, double* illum);
+
+/* ------------------------------------------------------------------------
+ * Job: several plans run together on one GPU, e.g. the baseline and
+ * monitor surveys of a time-lapse study on grids of different sizes.  Every
+ * time-step launch covers a group of shots of every plan (a workgroup map
+ * assigns each workgroup a plan and a tile), so the job takes as many
+ * launches per step as one plan.  The plans must share gpu_id and nSteps;
+ * each keeps its own model, data, misfit and gradients, read with the plan
+ * calls above after fwiflow_job_run.  No reference counterpart: its callers
+ * run one fwi_op per survey (docs/codes/src_fwi_coupled/
+ * main_two_phase_flow_inversion.jl:82-92).  At most 31 plans per job; a job
+ * does not own its plans, which must outlive it.
+ * ---------------------------------------------------------------------- */
+typedef struct fwiflow_job fwiflow_job;
+int fwiflow_job_create(fwiflow_job** job);
+int fwiflow_job_destroy(fwiflow_job* job);
+int fwiflow_job_add(fwiflow_job* job, fwiflow_plan* plan);
+int fwiflow_job_run(fwiflow_job* job, int calc_id);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner network.  No reference counterpart: FwiFlow's plain

@@ -815,10 +815,12 @@ struct Runner {
 
   // add_source (utilities.cu:521-555): the 9x9 Gaussian exp(-1000 r^2) is 1 at
   // the centre and underflows to 0 elsewhere, so only the centre cell changes.
+  // `float scale = pow(1500.0, 2)` (utilities.cu:528): the szz increment is a
+  // float product, the sxx one (RSXXZZ * scale * ...) a double product.
   void add_source(State<T>& s, T amp, int zs, int xs, bool isFor) {
     const T dt = p.dt;
     const size_t i = (size_t)xs * c.g.nz + zs;
-    const double scale = std::pow(1500.0, 2);
+    const T scale = (T)std::pow(1500.0, 2);
     const T gauss = std::exp((T)(-1000.0 * 0));
     if (isFor) {
       s.szz[i] += scale * amp * dt * gauss;

@@ -202,6 +202,10 @@ def main():
     P.set_obs(P.data())
     lam0, mu0, den0 = models.moduli(c, "start")
     P.set_model(lam0, mu0, den0)
+    # timing is on from the warmup so that the time loops the engine captures
+    # as graphs there (with their event records) are the ones the timed steps
+    # replay; set_timing(True) again clears the warmup's totals
+    P.set_timing(True)
     for i in range(a.warmup):
         t0 = time.perf_counter()
         P.run(1)

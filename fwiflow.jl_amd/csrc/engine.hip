@@ -40,6 +40,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -146,20 +147,20 @@ struct Pml {
   const float *Kx, *ax, *bx, *Kxh, *axh, *bxh;  // length nx
 };
 
-struct Wave {
-  float *vz, *vx, *szz, *sxx, *sxz;
-};
-
-// C-PML memory variables, named after the reference's d_mem_* arrays
-struct Mem {
-  float* vzz;   // mem_dvz_dz
-  float* vzx;   // mem_dvz_dx
-  float* vxz;   // mem_dvx_dz
-  float* vxx;   // mem_dvx_dx
-  float* szz;   // mem_dszz_dz
-  float* sxx;   // mem_dsxx_dx
-  float* sxzz;  // mem_dsxz_dz
-  float* sxzx;  // mem_dsxz_dx
+// Wavefield arrays are [shot][component][cell]: the 5 components of a state
+// (vz, vx, szz, sxx, sxz) and the 8 C-PML memory variables, named after the
+// reference's d_mem_* arrays.
+enum WaveComp { kVz = 0, kVx = 1, kSzz = 2, kSxx = 3, kSxz = 4, kWaveComps = 5 };
+enum MemComp {
+  kMvzz = 0,   // mem_dvz_dz
+  kMvzx = 1,   // mem_dvz_dx
+  kMvxz = 2,   // mem_dvx_dz
+  kMvxx = 3,   // mem_dvx_dx
+  kMszz = 4,   // mem_dszz_dz
+  kMsxx = 5,   // mem_dsxx_dx
+  kMsxzz = 6,  // mem_dsxz_dz
+  kMsxzx = 7,  // mem_dsxz_dx
+  kMemComps = 8
 };
 
 struct Src {
@@ -182,10 +183,10 @@ struct Part {
   Geo g;
   Mdl m;
   Pml c;
-  Wave w;           // forward state
-  Wave a;           // adjoint state
-  Mem p;            // C-PML memory variables (forward ones in forward launches,
-                    // adjoint ones in backward launches)
+  float* w;         // forward state [shot][5][cell] from the part's first shot
+  float* a;         // adjoint state, same layout
+  float* p;         // C-PML memory variables [shot][8][cell] (forward ones in
+                    // forward launches, adjoint ones in backward launches)
   Src src;          // source rows: injected (forward) or removed (reversal)
   Inj inj;
   float* ring;      // boundary ring buffer (null: not saved)
@@ -278,19 +279,36 @@ __device__ __forceinline__ Cell cell_of(unsigned bo, int nz) {
 __device__ __forceinline__ float ldb(Rsrc r, int so, int vo) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
 }
-// z neighbour k (-2..2) and x neighbour k (-2, -1, 1, 2) of the cell, shot offset so
-__device__ __forceinline__ float lz(Rsrc r, int so, const Cell& c, int k) {
-  return ldb(r, so, c.zm2 + 4 * (k + 2));
+// A [shot][component][cell] array of one part: its buffer resource and the
+// bytes of one component.  Accessors take the shot's byte offset `so` and a
+// component.
+struct Arr {
+  Rsrc r;
+  int nb;
+};
+__device__ __forceinline__ Arr arr(const float* p, int nb) { return Arr{rsrc(p), nb}; }
+// z neighbour k (-2..2) and x neighbour k (-2, -1, 1, 2) of the cell
+__device__ __forceinline__ float lz(const Arr& A, int so, int comp, const Cell& c, int k) {
+  return ldb(A.r, so + comp * A.nb, c.zm2 + 4 * (k + 2));
 }
-__device__ __forceinline__ float lx(Rsrc r, int so, const Cell& c, int k) {
-  return ldb(r, so, k == -2 ? c.xm2 : k == -1 ? c.xm1 : k == 1 ? c.xp1 : c.xp2);
+__device__ __forceinline__ float lx(const Arr& A, int so, int comp, const Cell& c, int k) {
+  return ldb(A.r, so + comp * A.nb, k == -2 ? c.xm2 : k == -1 ? c.xm1 : k == 1 ? c.xp1 : c.xp2);
 }
 // z neighbours k0 .. k0+3 of the cell in one 16-byte load
-__device__ __forceinline__ f4 lz4(Rsrc r, int so, const Cell& c, int k0) {
-  return raw_load4(r, c.zm2 + 4 * (k0 + 2), so, 0);
+__device__ __forceinline__ f4 lz4(const Arr& A, int so, int comp, const Cell& c, int k0) {
+  return raw_load4(A.r, c.zm2 + 4 * (k0 + 2), so + comp * A.nb, 0);
 }
-__device__ __forceinline__ void st(Rsrc r, int so, const Cell& c, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, c.zm2 + 8, so, 0);
+__device__ __forceinline__ void st(const Arr& A, int so, int comp, const Cell& c, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), A.r, c.zm2 + 8,
+                                        so + comp * A.nb, 0);
+}
+
+// Column of the calling thread in a tile whose first column is x0.  A
+// workgroup is 64 x tile_x threads, so each wave64 is one column: the column
+// is wave-uniform, and the x-direction C-PML coefficients and x branches it
+// selects are scalar.
+__device__ __forceinline__ int wave_col(int x0) {
+  return __builtin_amdgcn_readfirstlane(x0 + (int)threadIdx.y);
 }
 
 // Staggered 4th-order differences as the reference writes them:
@@ -325,22 +343,15 @@ template <int U>
 __device__ __forceinline__ void stress_fwd(const Part& P, int tile, int it) {
   const Geo& g = P.g;
   const int z = (tile % P.gzU) * kBZ + (int)threadIdx.x;
-  const int x = (tile / P.gzU) * (int)blockDim.y + (int)threadIdx.y;
+  const int x = wave_col((tile / P.gzU) * (int)blockDim.y);
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const Cell C = cell_of(bo, nz);
   const int Nb = (int)(g.N * 4);  // bytes per shot
-  const Rsrc Rp_vxx = rsrc(P.p.vxx);
-  const Rsrc Rp_vxz = rsrc(P.p.vxz);
-  const Rsrc Rp_vzx = rsrc(P.p.vzx);
-  const Rsrc Rp_vzz = rsrc(P.p.vzz);
-  const Rsrc Rw_sxx = rsrc(P.w.sxx);
-  const Rsrc Rw_sxz = rsrc(P.w.sxz);
-  const Rsrc Rw_szz = rsrc(P.w.szz);
-  const Rsrc Rw_vx = rsrc(P.w.vx);
-  const Rsrc Rw_vz = rsrc(P.w.vz);
+  const Arr W = arr(P.w, Nb);
+  const Arr M = arr(P.p, Nb);
   const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i];
   const double l2m = (double)lam + 2.0 * (double)mu;  // (d_Lambda + 2.0*d_Mu)
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
@@ -365,34 +376,34 @@ __device__ __forceinline__ void stress_fwd(const Part& P, int tile, int it) {
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb;
       StressIn& q = r[u];
-      const f4 vzc = lz4(Rw_vz, so, C, -2), vxc = lz4(Rw_vx, so, C, -1);
+      const f4 vzc = lz4(W, sw, kVz, C, -2), vxc = lz4(W, sw, kVx, C, -1);
       q.vzm2 = vzc[0], q.vzm1 = vzc[1], q.vz0 = vzc[2], q.vzp1 = vzc[3];
-      q.vzpx = lx(Rw_vz, so, C, 1), q.vzp2x = lx(Rw_vz, so, C, 2), q.vzmx = lx(Rw_vz, so, C, -1);
+      q.vzpx = lx(W, sw, kVz, C, 1), q.vzp2x = lx(W, sw, kVz, C, 2), q.vzmx = lx(W, sw, kVz, C, -1);
       q.vxm1 = vxc[0], q.vx0 = vxc[1], q.vxp1 = vxc[2], q.vxp2 = vxc[3];
-      q.vxmx = lx(Rw_vx, so, C, -1), q.vxpx = lx(Rw_vx, so, C, 1), q.vxm2x = lx(Rw_vx, so, C, -2);
-      q.szz = lz(Rw_szz, so, C, 0), q.sxx = lz(Rw_sxx, so, C, 0), q.sxz = lz(Rw_sxz, so, C, 0);
-      if (pz) q.mvzz = lz(Rp_vzz, so, C, 0), q.mvxz = lz(Rp_vxz, so, C, 0);
-      if (px) q.mvxx = lz(Rp_vxx, so, C, 0), q.mvzx = lz(Rp_vzx, so, C, 0);
+      q.vxmx = lx(W, sw, kVx, C, -1), q.vxpx = lx(W, sw, kVx, C, 1), q.vxm2x = lx(W, sw, kVx, C, -2);
+      q.szz = lz(W, sw, kSzz, C, 0), q.sxx = lz(W, sw, kSxx, C, 0), q.sxz = lz(W, sw, kSxz, C, 0);
+      if (pz) q.mvzz = lz(M, sm, kMvzz, C, 0), q.mvxz = lz(M, sm, kMvxz, C, 0);
+      if (px) q.mvxx = lz(M, sm, kMvxx, C, 0), q.mvzx = lz(M, sm, kMvzx, C, 0);
       q.sz = P.src.zx[2 * s], q.sx = P.src.zx[2 * s + 1];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb;
       const StressIn& q = r[u];
       float dvz_dz = dfw(q.vz0, q.vzm1, q.vzp1, q.vzm2, dz, rdz);
       float dvx_dx = dfw(q.vx0, q.vxmx, q.vxpx, q.vxm2x, dx, rdx);
       if (pz) {
         const float mm = bz * q.mvzz + az * dvz_dz;
-        st(Rp_vzz, so, C, mm);
+        st(M, sm, kMvzz, C, mm);
         dvz_dz = kdiv(dvz_dz, kz) + mm;
       }
       if (px) {
         const float mm = bx * q.mvxx + ax * dvx_dx;
-        st(Rp_vxx, so, C, mm);
+        st(M, sm, kMvxx, C, mm);
         dvx_dx = kdiv(dvx_dx, kx) + mm;
       }
       float szz = q.szz, sxx = q.sxx, sxz = q.sxz;
@@ -409,12 +420,12 @@ __device__ __forceinline__ void stress_fwd(const Part& P, int tile, int it) {
       float dvz_dx = dfw(q.vzpx, q.vz0, q.vzp2x, q.vzmx, dx, rdx);
       if (pz) {
         const float mm = bzh * q.mvxz + azh * dvx_dz;
-        st(Rp_vxz, so, C, mm);
+        st(M, sm, kMvxz, C, mm);
         dvx_dz = kdiv(dvx_dz, kzh) + mm;
       }
       if (px) {
         const float mm = bxh * q.mvzx + axh * dvz_dx;
-        st(Rp_vzx, so, C, mm);
+        st(M, sm, kMvzx, C, mm);
         dvz_dx = kdiv(dvz_dx, kxh) + mm;
       }
       sxz = sxz + amu * (dvx_dz + dvz_dx) * dt;
@@ -425,9 +436,9 @@ __device__ __forceinline__ void stress_fwd(const Part& P, int tile, int it) {
         szz = szz + kSrcScale * amp * dt;
         sxx = (float)((double)sxx + kRS * (double)kSrcScale * (double)amp * dt64);
       }
-      st(Rw_szz, so, C, szz);
-      st(Rw_sxx, so, C, sxx);
-      st(Rw_sxz, so, C, sxz);
+      st(W, sw, kSzz, C, szz);
+      st(W, sw, kSxx, C, sxx);
+      st(W, sw, kSxz, C, sxz);
     }
   }
 }
@@ -447,22 +458,15 @@ template <int U>
 __device__ __forceinline__ void vel_fwd(const Part& P, int tile) {
   const Geo& g = P.g;
   const int z = (tile % P.gzU) * kBZ + (int)threadIdx.x;
-  const int x = (tile / P.gzU) * (int)blockDim.y + (int)threadIdx.y;
+  const int x = wave_col((tile / P.gzU) * (int)blockDim.y);
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const Cell C = cell_of(bo, nz);
   const int Nb = (int)(g.N * 4);  // bytes per shot
-  const Rsrc Rp_sxx = rsrc(P.p.sxx);
-  const Rsrc Rp_sxzx = rsrc(P.p.sxzx);
-  const Rsrc Rp_sxzz = rsrc(P.p.sxzz);
-  const Rsrc Rp_szz = rsrc(P.p.szz);
-  const Rsrc Rw_sxx = rsrc(P.w.sxx);
-  const Rsrc Rw_sxz = rsrc(P.w.sxz);
-  const Rsrc Rw_szz = rsrc(P.w.szz);
-  const Rsrc Rw_vx = rsrc(P.w.vx);
-  const Rsrc Rw_vz = rsrc(P.w.vz);
+  const Arr W = arr(P.w, Nb);
+  const Arr M = arr(P.p, Nb);
   const float ba = P.m.ba[i], bb = P.m.bb[i];
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
   const bool px = x < g.nPml || x > g.nx - g.nPml;
@@ -485,33 +489,33 @@ __device__ __forceinline__ void vel_fwd(const Part& P, int tile) {
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb;
       VelIn& q = r[u];
-      const f4 szc = lz4(Rw_szz, so, C, -1), sxc = lz4(Rw_sxz, so, C, -2);
+      const f4 szc = lz4(W, sw, kSzz, C, -1), sxc = lz4(W, sw, kSxz, C, -2);
       q.szzm1 = szc[0], q.szz0 = szc[1], q.szzp1 = szc[2], q.szzp2 = szc[3];
       q.sxzm2 = sxc[0], q.sxzm1 = sxc[1], q.sxz0 = sxc[2], q.sxzp1 = sxc[3];
-      q.sxzmx = lx(Rw_sxz, so, C, -1), q.sxzpx = lx(Rw_sxz, so, C, 1), q.sxzm2x = lx(Rw_sxz, so, C, -2);
-      q.sxxpx = lx(Rw_sxx, so, C, 1), q.sxx0 = lz(Rw_sxx, so, C, 0), q.sxxp2x = lx(Rw_sxx, so, C, 2), q.sxxmx = lx(Rw_sxx, so, C, -1);
-      if (pz) q.mszz = lz(Rp_szz, so, C, 0), q.msxzz = lz(Rp_sxzz, so, C, 0);
-      if (px) q.msxzx = lz(Rp_sxzx, so, C, 0), q.msxx = lz(Rp_sxx, so, C, 0);
-      q.vz = lz(Rw_vz, so, C, 0), q.vx = lz(Rw_vx, so, C, 0);
+      q.sxzmx = lx(W, sw, kSxz, C, -1), q.sxzpx = lx(W, sw, kSxz, C, 1), q.sxzm2x = lx(W, sw, kSxz, C, -2);
+      q.sxxpx = lx(W, sw, kSxx, C, 1), q.sxx0 = lz(W, sw, kSxx, C, 0), q.sxxp2x = lx(W, sw, kSxx, C, 2), q.sxxmx = lx(W, sw, kSxx, C, -1);
+      if (pz) q.mszz = lz(M, sm, kMszz, C, 0), q.msxzz = lz(M, sm, kMsxzz, C, 0);
+      if (px) q.msxzx = lz(M, sm, kMsxzx, C, 0), q.msxx = lz(M, sm, kMsxx, C, 0);
+      q.vz = lz(W, sw, kVz, C, 0), q.vx = lz(W, sw, kVx, C, 0);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb;
       const VelIn& q = r[u];
       float dszz_dz = dfw(q.szzp1, q.szz0, q.szzp2, q.szzm1, dz, rdz);
       float dsxz_dx = dfw(q.sxz0, q.sxzmx, q.sxzpx, q.sxzm2x, dx, rdx);
       if (pz) {
         const float mm = bzh * q.mszz + azh * dszz_dz;
-        st(Rp_szz, so, C, mm);
+        st(M, sm, kMszz, C, mm);
         dszz_dz = kdiv(dszz_dz, kzh) + mm;
       }
       if (px) {
         const float mm = bx * q.msxzx + ax * dsxz_dx;
-        st(Rp_sxzx, so, C, mm);
+        st(M, sm, kMsxzx, C, mm);
         dsxz_dx = kdiv(dsxz_dx, kx) + mm;
       }
       const float vz = q.vz + (dszz_dz + dsxz_dx) * ba * dt;
@@ -519,17 +523,17 @@ __device__ __forceinline__ void vel_fwd(const Part& P, int tile) {
       float dsxx_dx = dfw(q.sxxpx, q.sxx0, q.sxxp2x, q.sxxmx, dx, rdx);
       if (pz) {
         const float mm = bz * q.msxzz + az * dsxz_dz;
-        st(Rp_sxzz, so, C, mm);
+        st(M, sm, kMsxzz, C, mm);
         dsxz_dz = kdiv(dsxz_dz, kz) + mm;
       }
       if (px) {
         const float mm = bxh * q.msxx + axh * dsxx_dx;
-        st(Rp_sxx, so, C, mm);
+        st(M, sm, kMsxx, C, mm);
         dsxx_dx = kdiv(dsxx_dx, kxh) + mm;
       }
       const float vx = q.vx + (dsxz_dz + dsxx_dx) * bb * dt;
-      st(Rw_vz, so, C, vz);
-      st(Rw_vx, so, C, vx);
+      st(W, sw, kVz, C, vz);
+      st(W, sw, kVx, C, vx);
       il += vz * vz + vx * vx;
     }
   }
@@ -545,8 +549,8 @@ __device__ __forceinline__ void record(const Part& P, int blk, int it) {
   if (r >= P.nrec) return;
   const Geo& g = P.g;
   const int z = P.rec[3 * r], x = P.rec[3 * r + 1], s = P.rec[3 * r + 2] - g.s0;
-  const long long o = s * g.N + (long long)x * g.nz + z;
-  P.syn[(size_t)(it + 1) * g.R + r] = (float)((double)P.w.szz[o] + kRS * (double)P.w.sxx[o]);
+  const float* w = P.w + (size_t)s * kWaveComps * g.N + (long long)x * g.nz + z;
+  P.syn[(size_t)(it + 1) * g.R + r] = (float)((double)w[kSzz * g.N] + kRS * (double)w[kSxx * g.N]);
 }
 
 // ------------------------------------------------------------------ backward
@@ -566,7 +570,7 @@ template <int U>
 __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
   const Geo& g = P.g;
   const int z = (tile % P.gzR) * kBZ + (int)threadIdx.x + (g.nPml - 2);
-  const int x = (tile / P.gzR) * (int)blockDim.y + (int)threadIdx.y + (g.nPml - 2);
+  const int x = wave_col((tile / P.gzR) * (int)blockDim.y) + (g.nPml - 2);
   if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
   const int slot = ring_slot(z, x, g);
@@ -576,13 +580,8 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
   const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const Cell C = cell_of(bo, nz);
   const int Nb = (int)(g.N * 4);  // bytes per shot
-  const Rsrc Ra_vx = rsrc(P.a.vx);
-  const Rsrc Ra_vz = rsrc(P.a.vz);
-  const Rsrc Rw_sxx = rsrc(P.w.sxx);
-  const Rsrc Rw_sxz = rsrc(P.w.sxz);
-  const Rsrc Rw_szz = rsrc(P.w.szz);
-  const Rsrc Rw_vx = rsrc(P.w.vx);
-  const Rsrc Rw_vz = rsrc(P.w.vz);
+  const Arr W = arr(P.w, Nb);
+  const Arr A = arr(P.a, Nb);
   const float ba = P.m.ba[i], bb = P.m.bb[i];
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
   const int ns = g.ns;
@@ -593,18 +592,18 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb;
       RevVelIn& q = r[u];
       if (in) {
-        const f4 szc = lz4(Rw_szz, so, C, -1), sxc = lz4(Rw_sxz, so, C, -2);
+        const f4 szc = lz4(W, sw, kSzz, C, -1), sxc = lz4(W, sw, kSxz, C, -2);
         q.szzm1 = szc[0], q.szz0 = szc[1], q.szzp1 = szc[2], q.szzp2 = szc[3];
         q.sxzm2 = sxc[0], q.sxzm1 = sxc[1], q.sxz0 = sxc[2], q.sxzp1 = sxc[3];
-        q.sxzmx = lx(Rw_sxz, so, C, -1), q.sxzpx = lx(Rw_sxz, so, C, 1), q.sxzm2x = lx(Rw_sxz, so, C, -2);
-        q.sxxpx = lx(Rw_sxx, so, C, 1), q.sxx0 = lz(Rw_sxx, so, C, 0), q.sxxp2x = lx(Rw_sxx, so, C, 2), q.sxxmx = lx(Rw_sxx, so, C, -1);
-        q.avz = lz(Ra_vz, so, C, 0), q.avx = lz(Ra_vx, so, C, 0);
+        q.sxzmx = lx(W, sw, kSxz, C, -1), q.sxzpx = lx(W, sw, kSxz, C, 1), q.sxzm2x = lx(W, sw, kSxz, C, -2);
+        q.sxxpx = lx(W, sw, kSxx, C, 1), q.sxx0 = lz(W, sw, kSxx, C, 0), q.sxxp2x = lx(W, sw, kSxx, C, 2), q.sxxmx = lx(W, sw, kSxx, C, -1);
+        q.avz = lz(A, sw, kVz, C, 0), q.avx = lz(A, sw, kVx, C, 0);
       }
       if (slot < 0) {
-        q.vz = lz(Rw_vz, so, C, 0), q.vx = lz(Rw_vx, so, C, 0);
+        q.vz = lz(W, sw, kVz, C, 0), q.vx = lz(W, sw, kVx, C, 0);
       } else {
         q.vz = *ring_ptr(P.ring, g, it, 3, s, slot);
         q.vx = *ring_ptr(P.ring, g, it, 4, s, slot);
@@ -614,7 +613,7 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb;
       const RevVelIn& q = r[u];
       float vz = q.vz, vx = q.vx;
       if (in) {
@@ -630,8 +629,8 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
           vx = vx - fx * bb * dt;
         }
       }
-      st(Rw_vz, so, C, vz);
-      st(Rw_vx, so, C, vx);
+      st(W, sw, kVz, C, vz);
+      st(W, sw, kVx, C, vx);
     }
   }
   if (in) {
@@ -655,7 +654,7 @@ template <int U>
 __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
   const Geo& g = P.g;
   const int z = (tile % P.gzR) * kBZ + (int)threadIdx.x + (g.nPml - 2);
-  const int x = (tile / P.gzR) * (int)blockDim.y + (int)threadIdx.y + (g.nPml - 2);
+  const int x = wave_col((tile / P.gzR) * (int)blockDim.y) + (g.nPml - 2);
   if (z > g.nzA - g.nPml + 1 || x > g.nx - g.nPml + 1) return;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
   const int slot = ring_slot(z, x, g);
@@ -665,14 +664,8 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
   const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const Cell C = cell_of(bo, nz);
   const int Nb = (int)(g.N * 4);  // bytes per shot
-  const Rsrc Ra_sxx = rsrc(P.a.sxx);
-  const Rsrc Ra_sxz = rsrc(P.a.sxz);
-  const Rsrc Ra_szz = rsrc(P.a.szz);
-  const Rsrc Rw_sxx = rsrc(P.w.sxx);
-  const Rsrc Rw_sxz = rsrc(P.w.sxz);
-  const Rsrc Rw_szz = rsrc(P.w.szz);
-  const Rsrc Rw_vx = rsrc(P.w.vx);
-  const Rsrc Rw_vz = rsrc(P.w.vz);
+  const Arr W = arr(P.w, Nb);
+  const Arr A = arr(P.a, Nb);
   const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i];
   const double l2m = (double)lam + 2.0 * (double)mu;
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
@@ -685,18 +678,18 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb;
       RevStressIn& q = r[u];
       if (in) {
-        const f4 vzc = lz4(Rw_vz, so, C, -2), vxc = lz4(Rw_vx, so, C, -1);
+        const f4 vzc = lz4(W, sw, kVz, C, -2), vxc = lz4(W, sw, kVx, C, -1);
         q.vzm2 = vzc[0], q.vzm1 = vzc[1], q.vz0 = vzc[2], q.vzp1 = vzc[3];
-        q.vzpx = lx(Rw_vz, so, C, 1), q.vzp2x = lx(Rw_vz, so, C, 2), q.vzmx = lx(Rw_vz, so, C, -1);
+        q.vzpx = lx(W, sw, kVz, C, 1), q.vzp2x = lx(W, sw, kVz, C, 2), q.vzmx = lx(W, sw, kVz, C, -1);
         q.vxm1 = vxc[0], q.vx0 = vxc[1], q.vxp1 = vxc[2], q.vxp2 = vxc[3];
-        q.vxmx = lx(Rw_vx, so, C, -1), q.vxpx = lx(Rw_vx, so, C, 1), q.vxm2x = lx(Rw_vx, so, C, -2);
-        q.aszz = lz(Ra_szz, so, C, 0), q.asxx = lz(Ra_sxx, so, C, 0), q.asxz = lz(Ra_sxz, so, C, 0);
+        q.vxmx = lx(W, sw, kVx, C, -1), q.vxpx = lx(W, sw, kVx, C, 1), q.vxm2x = lx(W, sw, kVx, C, -2);
+        q.aszz = lz(A, sw, kSzz, C, 0), q.asxx = lz(A, sw, kSxx, C, 0), q.asxz = lz(A, sw, kSxz, C, 0);
       }
       if (slot < 0) {
-        q.szz = lz(Rw_szz, so, C, 0), q.sxx = lz(Rw_sxx, so, C, 0), q.sxz = lz(Rw_sxz, so, C, 0);
+        q.szz = lz(W, sw, kSzz, C, 0), q.sxx = lz(W, sw, kSxx, C, 0), q.sxz = lz(W, sw, kSxz, C, 0);
         q.sz = P.src.zx[2 * s], q.sx = P.src.zx[2 * s + 1];
       } else {
         q.szz = *ring_ptr(P.ring, g, it, 0, s, slot);
@@ -708,7 +701,7 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb;
       const RevStressIn& q = r[u];
       float szz = q.szz, sxx = q.sxx, sxz = q.sxz;
       if (in) {
@@ -730,9 +723,9 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
           sxz = sxz - amu * (dvx_dz + dvz_dx) * dt;
         }
       }
-      st(Rw_szz, so, C, szz);
-      st(Rw_sxx, so, C, sxx);
-      st(Rw_sxz, so, C, sxz);
+      st(W, sw, kSzz, C, szz);
+      st(W, sw, kSxx, C, sxx);
+      st(W, sw, kSxz, C, sxz);
     }
   }
   if (in) {
@@ -761,26 +754,15 @@ template <int U>
 __device__ __forceinline__ void adj_vel(const Part& P, int tile, int it) {
   const Geo& g = P.g;
   const int z = (tile % P.gzU) * kBZ + (int)threadIdx.x;
-  const int x = (tile / P.gzU) * (int)blockDim.y + (int)threadIdx.y;
+  const int x = wave_col((tile / P.gzU) * (int)blockDim.y);
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const Cell C = cell_of(bo, nz);
   const int Nb = (int)(g.N * 4);  // bytes per shot
-  const Rsrc Ra_sxx = rsrc(P.a.sxx);
-  const Rsrc Ra_sxz = rsrc(P.a.sxz);
-  const Rsrc Ra_szz = rsrc(P.a.szz);
-  const Rsrc Ra_vx = rsrc(P.a.vx);
-  const Rsrc Ra_vz = rsrc(P.a.vz);
-  const Rsrc Rp_sxx = rsrc(P.p.sxx);
-  const Rsrc Rp_sxzx = rsrc(P.p.sxzx);
-  const Rsrc Rp_sxzz = rsrc(P.p.sxzz);
-  const Rsrc Rp_szz = rsrc(P.p.szz);
-  const Rsrc Rp_vxx = rsrc(P.p.vxx);
-  const Rsrc Rp_vxz = rsrc(P.p.vxz);
-  const Rsrc Rp_vzz = rsrc(P.p.vzz);
-  const Rsrc Rp_vzx = rsrc(P.p.vzx);
+  const Arr A = arr(P.a, Nb);
+  const Arr M = arr(P.p, Nb);
   const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i], ba = P.m.ba[i], bb = P.m.bb[i];
   const double l2m = (double)lam + 2.0 * (double)mu;
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
@@ -800,39 +782,39 @@ __device__ __forceinline__ void adj_vel(const Part& P, int tile, int it) {
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb;
       AdjVelIn& q = r[u];
-      const f4 szc = lz4(Ra_szz, so, C, -1), sxc = lz4(Ra_sxx, so, C, -1), sxzc = lz4(Ra_sxz, so, C, -2);
+      const f4 szc = lz4(A, sw, kSzz, C, -1), sxc = lz4(A, sw, kSxx, C, -1), sxzc = lz4(A, sw, kSxz, C, -2);
       q.szzm1 = szc[0], q.szz0 = szc[1], q.szzp1 = szc[2], q.szzp2 = szc[3];
-      q.szzpx = lx(Ra_szz, so, C, 1), q.szzp2x = lx(Ra_szz, so, C, 2), q.szzmx = lx(Ra_szz, so, C, -1);
+      q.szzpx = lx(A, sw, kSzz, C, 1), q.szzp2x = lx(A, sw, kSzz, C, 2), q.szzmx = lx(A, sw, kSzz, C, -1);
       q.sxxm1 = sxc[0], q.sxx0 = sxc[1], q.sxxp1 = sxc[2], q.sxxp2 = sxc[3];
-      q.sxxpx = lx(Ra_sxx, so, C, 1), q.sxxp2x = lx(Ra_sxx, so, C, 2), q.sxxmx = lx(Ra_sxx, so, C, -1);
+      q.sxxpx = lx(A, sw, kSxx, C, 1), q.sxxp2x = lx(A, sw, kSxx, C, 2), q.sxxmx = lx(A, sw, kSxx, C, -1);
       q.sxzm2 = sxzc[0], q.sxzm1 = sxzc[1], q.sxz0 = sxzc[2], q.sxzp1 = sxzc[3];
-      q.sxzmx = lx(Ra_sxz, so, C, -1), q.sxzpx = lx(Ra_sxz, so, C, 1), q.sxzm2x = lx(Ra_sxz, so, C, -2);
+      q.sxzmx = lx(A, sw, kSxz, C, -1), q.sxzpx = lx(A, sw, kSxz, C, 1), q.sxzm2x = lx(A, sw, kSxz, C, -2);
       if (ax != 0.f) {  // psi_vxx at x+1, x, x+2, x-1
-        q.q1[0] = lx(Rp_vxx, so, C, 1), q.q1[1] = lz(Rp_vxx, so, C, 0), q.q1[2] = lx(Rp_vxx, so, C, 2), q.q1[3] = lx(Rp_vxx, so, C, -1);
+        q.q1[0] = lx(M, sm, kMvxx, C, 1), q.q1[1] = lz(M, sm, kMvxx, C, 0), q.q1[2] = lx(M, sm, kMvxx, C, 2), q.q1[3] = lx(M, sm, kMvxx, C, -1);
       }
       if (azh != 0.f) {  // psi_vxz at z, z-1, z+1, z-2
-        const f4 t = lz4(Rp_vxz, so, C, -2);
+        const f4 t = lz4(M, sm, kMvxz, C, -2);
         q.q2[0] = t[2], q.q2[1] = t[1], q.q2[2] = t[3], q.q2[3] = t[0];
       }
       if (az != 0.f) {  // psi_vzz at z+1, z, z+2, z-1
-        const f4 t = lz4(Rp_vzz, so, C, -1);
+        const f4 t = lz4(M, sm, kMvzz, C, -1);
         q.q3[0] = t[2], q.q3[1] = t[1], q.q3[2] = t[3], q.q3[3] = t[0];
       }
       if (axh != 0.f) {  // psi_vzx at x, x-1, x+1, x-2
-        q.q4[0] = lz(Rp_vzx, so, C, 0), q.q4[1] = lx(Rp_vzx, so, C, -1), q.q4[2] = lx(Rp_vzx, so, C, 1), q.q4[3] = lx(Rp_vzx, so, C, -2);
+        q.q4[0] = lz(M, sm, kMvzx, C, 0), q.q4[1] = lx(M, sm, kMvzx, C, -1), q.q4[2] = lx(M, sm, kMvzx, C, 1), q.q4[3] = lx(M, sm, kMvzx, C, -2);
       }
-      if (px) q.m_sxx = lz(Rp_sxx, so, C, 0), q.m_sxzx = lz(Rp_sxzx, so, C, 0);
-      if (pz) q.m_sxzz = lz(Rp_sxzz, so, C, 0), q.m_szz = lz(Rp_szz, so, C, 0);
-      q.vx = lz(Ra_vx, so, C, 0), q.vz = lz(Ra_vz, so, C, 0);
+      if (px) q.m_sxx = lz(M, sm, kMsxx, C, 0), q.m_sxzx = lz(M, sm, kMsxzx, C, 0);
+      if (pz) q.m_sxzz = lz(M, sm, kMsxzz, C, 0), q.m_szz = lz(M, sm, kMszz, C, 0);
+      q.vx = lz(A, sw, kVx, C, 0), q.vz = lz(A, sw, kVz, C, 0);
       q.sz = P.src.zx[2 * s], q.sx = P.src.zx[2 * s + 1];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb;
       const AdjVelIn& q = r[u];
       if (stf && z == q.sz && x == q.sx)  // -(szz_adj + RSXXZZ sxx_adj) dt, double, stored as float
         P.stf_grad[(size_t)s * g.nSteps + it] = (float)(-((double)q.szz0 + kRS * (double)q.sxx0) * dt64);
@@ -847,9 +829,9 @@ __device__ __forceinline__ void adj_vel(const Part& P, int tile, int it) {
       if (azh != 0.f) S = S + (double)(azh * dad(q.q2[0], q.q2[1], q.q2[2], q.q2[3], dz, rdz));
       S = S + (double)(amu_kzh * dsxz_dz * dt);
       const float vx = (float)((double)q.vx + S);
-      st(Ra_vx, so, C, vx);
-      if (px) st(Rp_sxx, so, C, bxh * q.m_sxx + bb * vx * dt);
-      if (pz) st(Rp_sxzz, so, C, bz * q.m_sxzz + bb * vx * dt);
+      st(A, sw, kVx, C, vx);
+      if (px) st(M, sm, kMsxx, C, bxh * q.m_sxx + bb * vx * dt);
+      if (pz) st(M, sm, kMsxzz, C, bz * q.m_sxzz + bb * vx * dt);
       // update vz: a_z dpsi_zz/dz + (lambda + 2.0 mu) dszz/dz / K_z dt + lambda dsxx/dz / K_z dt
       //            + a_x_half dpsi_zx/dx + ave_mu / K_x_half dsxz/dx dt
       const float dszz_dz = dad(q.szzp1, q.szz0, q.szzp2, q.szzm1, dz, rdz);
@@ -861,9 +843,9 @@ __device__ __forceinline__ void adj_vel(const Part& P, int tile, int it) {
       if (axh != 0.f) S = S + (double)(axh * dad(q.q4[0], q.q4[1], q.q4[2], q.q4[3], dx, rdx));
       S = S + (double)(amu_kxh * dsxz_dx * dt);
       const float vz = (float)((double)q.vz + S);
-      st(Ra_vz, so, C, vz);
-      if (px) st(Rp_sxzx, so, C, bx * q.m_sxzx + ba * vz * dt);
-      if (pz) st(Rp_szz, so, C, bzh * q.m_szz + ba * vz * dt);
+      st(A, sw, kVz, C, vz);
+      if (px) st(M, sm, kMsxzx, C, bx * q.m_sxzx + ba * vz * dt);
+      if (pz) st(M, sm, kMszz, C, bzh * q.m_szz + ba * vz * dt);
     }
   }
 }
@@ -888,26 +870,15 @@ template <int U>
 __device__ __forceinline__ void adj_stress(const Part& P, int tile, int it) {
   const Geo& g = P.g;
   const int z = (tile % P.gzU) * kBZ + (int)threadIdx.x;
-  const int x = (tile / P.gzU) * (int)blockDim.y + (int)threadIdx.y;
+  const int x = wave_col((tile / P.gzU) * (int)blockDim.y);
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const unsigned bo = (unsigned)i * 4u;  // byte offset of the cell in a per-shot array
   const Cell C = cell_of(bo, nz);
   const int Nb = (int)(g.N * 4);  // bytes per shot
-  const Rsrc Ra_sxx = rsrc(P.a.sxx);
-  const Rsrc Ra_sxz = rsrc(P.a.sxz);
-  const Rsrc Ra_szz = rsrc(P.a.szz);
-  const Rsrc Ra_vx = rsrc(P.a.vx);
-  const Rsrc Ra_vz = rsrc(P.a.vz);
-  const Rsrc Rp_sxx = rsrc(P.p.sxx);
-  const Rsrc Rp_sxzx = rsrc(P.p.sxzx);
-  const Rsrc Rp_sxzz = rsrc(P.p.sxzz);
-  const Rsrc Rp_szz = rsrc(P.p.szz);
-  const Rsrc Rp_vxx = rsrc(P.p.vxx);
-  const Rsrc Rp_vxz = rsrc(P.p.vxz);
-  const Rsrc Rp_vzx = rsrc(P.p.vzx);
-  const Rsrc Rp_vzz = rsrc(P.p.vzz);
+  const Arr A = arr(P.a, Nb);
+  const Arr M = arr(P.p, Nb);
   const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i], ba = P.m.ba[i], bb = P.m.bb[i];
   const double l2m = (double)lam + 2.0 * (double)mu;
   const bool zlive = z <= g.nPml + 1 || z >= g.nzA - g.nPml - 2;
@@ -938,36 +909,36 @@ __device__ __forceinline__ void adj_stress(const Part& P, int tile, int it) {
           has[u] = true;
         }
       }
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb;
       AdjStressIn& q = r[u];
-      const f4 vzc = lz4(Ra_vz, so, C, -2), vxc = lz4(Ra_vx, so, C, -1);
+      const f4 vzc = lz4(A, sw, kVz, C, -2), vxc = lz4(A, sw, kVx, C, -1);
       q.vzm2 = vzc[0], q.vzm1 = vzc[1], q.vz0 = vzc[2], q.vzp1 = vzc[3];
-      q.vzpx = lx(Ra_vz, so, C, 1), q.vzp2x = lx(Ra_vz, so, C, 2), q.vzmx = lx(Ra_vz, so, C, -1);
+      q.vzpx = lx(A, sw, kVz, C, 1), q.vzp2x = lx(A, sw, kVz, C, 2), q.vzmx = lx(A, sw, kVz, C, -1);
       q.vxm1 = vxc[0], q.vx0 = vxc[1], q.vxp1 = vxc[2], q.vxp2 = vxc[3];
-      q.vxmx = lx(Ra_vx, so, C, -1), q.vxpx = lx(Ra_vx, so, C, 1), q.vxm2x = lx(Ra_vx, so, C, -2);
+      q.vxmx = lx(A, sw, kVx, C, -1), q.vxpx = lx(A, sw, kVx, C, 1), q.vxm2x = lx(A, sw, kVx, C, -2);
       if (ax != 0.f) {  // psi_sxzx at x+1, x, x+2, x-1
-        q.q1[0] = lx(Rp_sxzx, so, C, 1), q.q1[1] = lz(Rp_sxzx, so, C, 0), q.q1[2] = lx(Rp_sxzx, so, C, 2), q.q1[3] = lx(Rp_sxzx, so, C, -1);
+        q.q1[0] = lx(M, sm, kMsxzx, C, 1), q.q1[1] = lz(M, sm, kMsxzx, C, 0), q.q1[2] = lx(M, sm, kMsxzx, C, 2), q.q1[3] = lx(M, sm, kMsxzx, C, -1);
       }
       if (az != 0.f) {  // psi_sxzz at z+1, z, z+2, z-1
-        const f4 t = lz4(Rp_sxzz, so, C, -1);
+        const f4 t = lz4(M, sm, kMsxzz, C, -1);
         q.q2[0] = t[2], q.q2[1] = t[1], q.q2[2] = t[3], q.q2[3] = t[0];
       }
       if (axh != 0.f) {  // psi_sxx at x, x-1, x+1, x-2
-        q.q3[0] = lz(Rp_sxx, so, C, 0), q.q3[1] = lx(Rp_sxx, so, C, -1), q.q3[2] = lx(Rp_sxx, so, C, 1), q.q3[3] = lx(Rp_sxx, so, C, -2);
+        q.q3[0] = lz(M, sm, kMsxx, C, 0), q.q3[1] = lx(M, sm, kMsxx, C, -1), q.q3[2] = lx(M, sm, kMsxx, C, 1), q.q3[3] = lx(M, sm, kMsxx, C, -2);
       }
       if (azh != 0.f) {  // psi_szz at z, z-1, z+1, z-2
-        const f4 t = lz4(Rp_szz, so, C, -2);
+        const f4 t = lz4(M, sm, kMszz, C, -2);
         q.q4[0] = t[2], q.q4[1] = t[1], q.q4[2] = t[3], q.q4[3] = t[0];
       }
-      if (xlive) q.m_vzx = lz(Rp_vzx, so, C, 0), q.m_vxx = lz(Rp_vxx, so, C, 0);
-      if (zlive) q.m_vxz = lz(Rp_vxz, so, C, 0), q.m_vzz = lz(Rp_vzz, so, C, 0);
-      q.sxz = lz(Ra_sxz, so, C, 0), q.sxx = lz(Ra_sxx, so, C, 0), q.szz = lz(Ra_szz, so, C, 0);
+      if (xlive) q.m_vzx = lz(M, sm, kMvzx, C, 0), q.m_vxx = lz(M, sm, kMvxx, C, 0);
+      if (zlive) q.m_vxz = lz(M, sm, kMvxz, C, 0), q.m_vzz = lz(M, sm, kMvzz, C, 0);
+      q.sxz = lz(A, sw, kSxz, C, 0), q.sxx = lz(A, sw, kSxx, C, 0), q.szz = lz(A, sw, kSzz, C, 0);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int s = s0 + u;
       if (s >= ns) break;
-      const int so = s * Nb;
+      const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb;
       const AdjStressIn& q = r[u];
       // sxz += a_x dphi_xz_x/dx + dvz/dx / K_x ba dt + a_z dphi_xz_z/dz + dvx/dz / K_z bb dt
       const float dvz_dx = dad(q.vzpx, q.vz0, q.vzp2x, q.vzmx, dx, rdx);
@@ -977,9 +948,9 @@ __device__ __forceinline__ void adj_stress(const Part& P, int tile, int it) {
       if (az != 0.f) e = e + az * dad(q.q2[0], q.q2[1], q.q2[2], q.q2[3], dz, rdz);
       e = e + kdiv(dvx_dz, kz) * bb * dt;
       const float sxz = q.sxz + e;
-      st(Ra_sxz, so, C, sxz);
-      if (xlive) st(Rp_vzx, so, C, bxh * q.m_vzx + sxz * amu * dt);
-      if (zlive) st(Rp_vxz, so, C, bzh * q.m_vxz + sxz * amu * dt);
+      st(A, sw, kSxz, C, sxz);
+      if (xlive) st(M, sm, kMvzx, C, bxh * q.m_vzx + sxz * amu * dt);
+      if (zlive) st(M, sm, kMvxz, C, bzh * q.m_vxz + sxz * amu * dt);
       const float dvx_dx = dad(q.vx0, q.vxmx, q.vxpx, q.vxm2x, dx, rdx);
       const float dvz_dz = dad(q.vz0, q.vzm1, q.vzp1, q.vzm2, dz, rdz);
       float sxx0 = q.sxx, szz0 = q.szz;
@@ -994,14 +965,14 @@ __device__ __forceinline__ void adj_stress(const Part& P, int tile, int it) {
       t = kdiv(ba * dvz_dz, kzh) * dt;
       if (azh != 0.f) t = azh * dad(q.q4[0], q.q4[1], q.q4[2], q.q4[3], dz, rdz) + t;
       const float szz = szz0 + t;
-      st(Ra_sxx, so, C, sxx);
-      st(Ra_szz, so, C, szz);
+      st(A, sw, kSxx, C, sxx);
+      st(A, sw, kSzz, C, szz);
       // psi_xx = b_x psi + lambda szz dt + (lambda + 2.0 mu) sxx dt;
       // psi_zz = b_z psi + (lambda + 2.0 mu) szz dt + lambda sxx dt
       if (xlive)
-        st(Rp_vxx, so, C, (float)((double)(bx * q.m_vxx + lam * szz * dt) + l2m * (double)sxx * dt64));
+        st(M, sm, kMvxx, C, (float)((double)(bx * q.m_vxx + lam * szz * dt) + l2m * (double)sxx * dt64));
       if (zlive)
-        st(Rp_vzz, so, C, (float)((double)(bz * q.m_vzz) + l2m * (double)szz * dt64 + (double)(lam * sxx * dt)));
+        st(M, sm, kMvzz, C, (float)((double)(bz * q.m_vzz) + l2m * (double)szz * dt64 + (double)(lam * sxx * dt)));
     }
   }
 }
@@ -1068,20 +1039,21 @@ __global__ __launch_bounds__(256) void k_bwd_b(Parts<NP> ps, const unsigned* __r
 // order in which the reference's per-receiver += lands on a shared cell when
 // its threads run in index order.  cells[c] = (plan shot, z, x); receivers
 // of cell c are rid[ptr[c] .. ptr[c+1]).
-__global__ void k_res_inject(Geo g, Wave a_all, const int* __restrict__ cells,
+__global__ void k_res_inject(Geo g, float* __restrict__ adj, const int* __restrict__ cells,
                              const int* __restrict__ ptr, const int* __restrict__ rid, int c0,
                              int ncells, const float* __restrict__ row) {
   const int c = c0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (c >= c0 + ncells) return;
-  const long long o = cells[3 * c] * g.N + (long long)cells[3 * c + 2] * g.nz + cells[3 * c + 1];
-  float szz = a_all.szz[o], sxx = a_all.sxx[o];
+  float* a = adj + (size_t)cells[3 * c] * kWaveComps * g.N + (long long)cells[3 * c + 2] * g.nz +
+             cells[3 * c + 1];
+  float szz = a[kSzz * g.N], sxx = a[kSxx * g.N];
   for (int j = ptr[c]; j < ptr[c + 1]; ++j) {
     const float v = row[rid[j]];
     szz = szz + v;
     sxx = (float)((double)sxx + kRS * (double)v);
   }
-  a_all.szz[o] = szz;
-  a_all.sxx[o] = sxx;
+  a[kSzz * g.N] = szz;
+  a[kSxx * g.N] = sxx;
 }
 
 // Residual (libCUFD.cu:258-313 without the optional filters) over receivers
@@ -1342,7 +1314,21 @@ std::vector<unsigned> flatten(const Lists& L) {
 
 }  // namespace
 
-// Device-resident workgroup maps and part tables of a job, kept across runs.
+// A launch bracketed by timing events.
+struct TimedLaunch {
+  std::string name;
+  hipEvent_t a, b;
+  bool pooled = true;  // events go back to the pool (not those of a captured graph)
+};
+
+// A shot group's time loop captured as a graph, and its timed launches.
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  std::vector<TimedLaunch> recs;
+};
+
+// Device-resident workgroup maps, part tables and captured time loops of a
+// job, kept across runs.
 struct DevMap {
   unsigned* ptr = nullptr;
   int n = 0;  // workgroups of the launch
@@ -1353,10 +1339,22 @@ struct JobCache {
   std::map<std::string, DevMap> maps;
   Part* table = nullptr;  // part tables of the multi-part launches of the last run
   size_t table_cap = 0;
+  std::map<std::string, GraphEntry> graphs;
   ~JobCache() {
     (void)hipSetDevice(dev);
+    clear_graphs();
     for (auto& kv : maps) (void)hipFree(kv.second.ptr);
     if (table) (void)hipFree(table);
+  }
+  void clear_graphs() {
+    for (auto& kv : graphs) {
+      (void)hipGraphExecDestroy(kv.second.exec);
+      for (auto& r : kv.second.recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+      }
+    }
+    graphs.clear();
   }
   Status map(const std::string& key, const Lists& L, DevMap& out) {
     auto it = maps.find(key);
@@ -1407,6 +1405,8 @@ struct DeviceState {
   int shot_group_bwd = -1; // shots per backward launch (0: all, -1: automatic)
   bool fold_inject = true; // residual injection inside the adjoint stress update
   bool part_table = false; // single-part launches also read their part from device memory
+  bool graphs = false;     // shot groups' time loops captured and replayed as graphs
+                           // (measured no faster at 608x1780; timed runs stay eager)
   int ckpt_every = 0;      // 0: boundary ring every step; k > 0: checkpoints every k steps
   double ring_budget = 32e9;  // device bytes a gradient run may hold for ring + checkpoints
   // model + tables
@@ -1460,10 +1460,8 @@ struct DeviceState {
   long long bytes = 0;
   // timing
   bool timing = false;
-  struct Rec {
-    std::string name;
-    hipEvent_t a, b;
-  };
+  using Rec = TimedLaunch;
+  std::vector<Rec>* sink = nullptr;  // while a graph is captured: its sampled launches
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> totals;
@@ -1472,6 +1470,7 @@ struct DeviceState {
     if (stream) (void)hipSetDevice(dev);
     cache.reset();
     for (auto& r : pending) {
+      if (!r.pooled) continue;  // owned by a captured graph
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
     }
@@ -1516,11 +1515,11 @@ struct DeviceState {
   template <typename F>
   void launch(const char* name, bool sample, F&& f) {
     if (timing && sample) {
-      Rec r{name, take_event(), take_event()};
+      Rec r{name, take_event(), take_event(), sink == nullptr};
       (void)hipEventRecord(r.a, stream);
       f();
       (void)hipEventRecord(r.b, stream);
-      pending.push_back(r);
+      (sink ? *sink : pending).push_back(r);
     } else {
       f();
     }
@@ -1533,8 +1532,10 @@ struct DeviceState {
       auto& t = totals[r.name];
       t.first += ms;
       t.second += 1;
-      pool.push_back(r.a);
-      pool.push_back(r.b);
+      if (r.pooled) {
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+      }
     }
     pending.clear();
   }
@@ -1548,16 +1549,8 @@ struct DeviceState {
   }
   Mdl mdl() const { return Mdl{lam, mu, amu, ba, bb}; }
   size_t F() const { return (size_t)g.ns * g.N; }
-  Wave wave(float* base, int s0 = 0) const {
-    const size_t o = (size_t)s0 * g.N;
-    return Wave{base + o, base + F() + o, base + 2 * F() + o, base + 3 * F() + o,
-                base + 4 * F() + o};
-  }
-  Mem memv(float* b, int s0 = 0) const {
-    const size_t o = (size_t)s0 * g.N, f = F();
-    return Mem{b + o,         b + f + o,     b + 2 * f + o, b + 3 * f + o,
-               b + 4 * f + o, b + 5 * f + o, b + 6 * f + o, b + 7 * f + o};
-  }
+  // first cell of shot s0 in a [shot][comps][cell] array
+  float* shot(float* base, int comps, int s0) const { return base + (size_t)s0 * comps * g.N; }
   int gzU() const { return (g.nzA + kBZ - 1) / kBZ; }
   int gxU() const { return (g.nx + tile_x - 1) / tile_x; }
   int gzR() const { return (g.ringH + kBZ - 1) / kBZ; }
@@ -1570,12 +1563,10 @@ struct DeviceState {
   double bwd_bytes_per_shot() const { return 64.0 * (double)g.N; }
   size_t ring_step_floats() const { return (size_t)5 * g.ringlen; }
   size_t snap_floats_per_shot() const { return (size_t)13 * g.N; }
-  // zero [s0, s0+n) of a [component][shot][cell] array of `comps` components
+  // zero shots [s0, s0+n) of a [shot][comps][cell] array
   // on stream st (the stream of the run, which may be another plan's)
   Status zero_shots(float* base, int comps, int s0, int n, hipStream_t st) {
-    const size_t o = (size_t)s0 * g.N, len = (size_t)n * g.N;
-    for (int c = 0; c < comps; c++)
-      HIP_TRY(hipMemsetAsync(base + (size_t)c * F() + o, 0, len * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(shot(base, comps, s0), 0, (size_t)n * comps * g.N * sizeof(float), st));
     return Status::Ok();
   }
 };
@@ -1611,6 +1602,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.shot_group_bwd = std::max(-1, env_int("FWIFLOW_SHOT_GROUP_BWD", -1));
   d.fold_inject = env_int("FWIFLOW_FOLD_INJECT", 1) != 0;
   d.part_table = env_int("FWIFLOW_PART_TABLE", 0) != 0;
+  d.graphs = env_int("FWIFLOW_GRAPHS", 0) != 0;
   HIP_TRY(hipSetDevice(gpu));
   HIP_TRY(hipStreamCreate(&d.stream));
   Geo& g = d.g;
@@ -2054,7 +2046,65 @@ class Runner {
   void bwd_a(const StepLaunch& s, int it, int U);
   void bwd_b(const StepLaunch& s, int it, int U);
   Status snapshot(const GroupParts& gp, int sg0, int j, bool save);
+  // run `body` (launches only, no memsets) as a graph captured on first use
+  Status replay(const std::string& key, const std::function<void()>& body);
+  std::string graph_key(const char* phase, const GroupParts& gp,
+                        std::initializer_list<const DevMap*> maps) const;
 };
+
+std::string Runner::graph_key(const char* phase, const GroupParts& gp,
+                              std::initializer_list<const DevMap*> maps) const {
+  // FNV-1a over everything the captured launches bake in: parts, maps,
+  // unroll factors and whether launches are timed
+  unsigned long long h = 1469598103934665603ULL;
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ULL;
+  };
+  for (const Part& P : gp.parts) mix(&P, sizeof(Part));
+  for (const DevMap* m : maps) mix(m, sizeof(DevMap));
+  const int flags[4] = {d0->unroll, d0->unroll_fwd, d0->timing ? 1 : 0, nSteps};
+  mix(flags, sizeof(flags));
+  if (gp.parts.size() > 1 || d0->part_table) mix(&cache.table, sizeof(cache.table));
+  return std::string(phase) + "/" + gp.key + "/" + std::to_string(h);
+}
+
+Status Runner::replay(const std::string& key, const std::function<void()>& body) {
+  // events recorded inside a captured graph cannot be timed: timed runs are eager
+  if (!d0->graphs || d0->timing) {
+    body();
+    return Status::Ok();
+  }
+  auto it = cache.graphs.find(key);
+  if (it == cache.graphs.end()) {
+    if (cache.graphs.size() >= 512) cache.clear_graphs();
+    GraphEntry e;
+    d0->sink = &e.recs;
+    hipError_t err = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (err == hipSuccess) {
+      body();
+      hipGraph_t g = nullptr;
+      err = hipStreamEndCapture(st, &g);
+      if (err == hipSuccess) {
+        err = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+      }
+    }
+    d0->sink = nullptr;
+    if (err != hipSuccess) {
+      for (auto& r : e.recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+      }
+      return Status::Err(FWIFLOW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(err));
+    }
+    it = cache.graphs.emplace(key, std::move(e)).first;
+  }
+  HIP_TRY(hipGraphLaunch(it->second.exec, st));
+  if (d0->timing)
+    for (const TimedLaunch& r : it->second.recs) d0->pending.push_back(r);
+  return Status::Ok();
+}
 
 #define STEP_LAUNCH(KER)                                                                    \
   do {                                                                                      \
@@ -2097,9 +2147,9 @@ GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
     P.g.rnt = ckpt ? K : std::max(1, nSteps - 1);
     P.m = d.mdl();
     P.c = d.pml_ptrs();
-    P.w = d.wave(d.fwd, s0);
-    P.a = d.adj ? d.wave(d.adj, s0) : Wave{};
-    P.p = ph == kPhBwd ? d.memv(ckpt ? d.madj : d.mem, s0) : d.memv(d.mem, s0);
+    P.w = d.shot(d.fwd, kWaveComps, s0);
+    P.a = d.adj ? d.shot(d.adj, kWaveComps, s0) : nullptr;
+    P.p = d.shot(ph == kPhBwd && ckpt ? d.madj : d.mem, kMemComps, s0);
     const float* amp = (ph == kPhBwd && d.src_updated) ? d.srcout : d.srcamp;
     P.src = Src{d.src + 2 * s0, amp + (size_t)s0 * nSteps};
     P.inj = Inj{d.rec_at && ph == kPhBwd ? d.res : nullptr, d.rec_at, d.rec_row};
@@ -2191,21 +2241,25 @@ Status Runner::maps_of(const GroupParts& gp, const char* kind, DevMap& out) {
 }
 
 // Checkpoint j >= 1 of the shots of a group: the 5 wavefield components and
-// the 8 forward C-PML memory variables before forward step j * K.
-// Layout [j-1][array][shot of the sweep][cell].
+// the 8 forward C-PML memory variables before forward step j * K.  Layout
+// [j-1] x {wavefields [shot of the sweep][5][cell], memory variables
+// [shot of the sweep][8][cell]}: a group's shots are one contiguous block of
+// each.
 Status Runner::snapshot(const GroupParts& gp, int sg0, int j, bool save) {
   for (size_t q = 0; q < gp.parts.size(); q++) {
     DeviceState& d = *M[gp.member[q]].d;
     const int s0 = gp.parts[q].g.s0, n = gp.parts[q].g.ns;
     const int rows_m = std::min(S, d.g.ns), rs = s0 - shots_in(M[gp.member[q]], sg0);
-    const size_t N = (size_t)d.g.N, F = d.F();
-    for (int a = 0; a < 13; a++) {
-      float* field = a < 5 ? d.fwd + (size_t)a * F : d.mem + (size_t)(a - 5) * F;
-      field += (size_t)s0 * N;
-      float* cp = d.snap + (((size_t)(j - 1) * 13 + a) * rows_m + rs) * N;
-      const size_t nb = (size_t)n * N * sizeof(float);
-      HIP_TRY(hipMemcpyAsync(save ? cp : field, save ? field : cp, nb, hipMemcpyDeviceToDevice, st));
-    }
+    const size_t N = (size_t)d.g.N;
+    float* cp = d.snap + (size_t)(j - 1) * (kWaveComps + kMemComps) * rows_m * N;
+    float* cw = cp + (size_t)rs * kWaveComps * N;
+    float* cm = cp + (size_t)rows_m * kWaveComps * N + (size_t)rs * kMemComps * N;
+    float* fw = d.shot(d.fwd, kWaveComps, s0);
+    float* fm = d.shot(d.mem, kMemComps, s0);
+    const size_t bw = (size_t)n * kWaveComps * N * sizeof(float);
+    const size_t bm = (size_t)n * kMemComps * N * sizeof(float);
+    HIP_TRY(hipMemcpyAsync(save ? cw : fw, save ? fw : cw, bw, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(save ? cm : fm, save ? fm : cm, bm, hipMemcpyDeviceToDevice, st));
   }
   return Status::Ok();
 }
@@ -2259,6 +2313,14 @@ Status Runner::run(int calc_id) {
     gb = gf;
   else
     gb = autosize(rowb);
+  // the kernels address a part's arrays with 32-bit byte offsets: one part's
+  // C-PML memory variables (8 arrays per shot) must stay below 2^31 bytes
+  int cap = rows;
+  for (const Member& m : M)
+    cap = std::min(cap, (int)(2147483647LL / ((long long)kMemComps * 4 * m.d->g.N)));
+  if (cap < 1) return Status::Err(FWIFLOW_ERR_SHAPE, "grid too large for one shot per launch");
+  gf = std::min(gf, cap);
+  gb = std::min(gb, cap);
   // memory plan of a gradient run: rows per sweep S and checkpoint interval K
   const int steps = std::max(1, nSteps - 1);  // forward steps 0 .. nSteps-2
   const double budget = c0.ring_budget;
@@ -2373,7 +2435,10 @@ Status Runner::run(int calc_id) {
     }
   }
   const int Uf = d0->unroll_fwd, Ub = d0->unroll;
-  const int sample_every = 8;
+  // every 64th step of each launch kind is bracketed by timing events (when
+  // timing is on): events between launches cost the stream about 2% at
+  // every 8th step
+  const int sample_every = 64;
   for (size_t j = 0; j < sweeps.size(); j++) {
     const RowRange sw = sweeps[j];
     // ------------------------------------------------------ forward
@@ -2382,11 +2447,21 @@ Status Runner::run(int calc_id) {
       TRY(maps_of(gp, "S", mS));
       TRY(maps_of(gp, "V", mV));
       const StepLaunch sS = step(gp, mS), sV = step(gp, mV);
-      for (int it = 0; it <= nSteps - 2; it++) {
-        const bool smp = it % sample_every == 0;
-        if (ckpt && it > 0 && it % K == 0) TRY(snapshot(gp, sw.r0, it / K, true));
-        d0->launch("stress_fwd", smp, [&] { fwd_stress(sS, it, Uf); });
-        d0->launch("vel_fwd", smp, [&] { fwd_vel(sV, it, Uf); });
+      if (ckpt) {  // checkpoint copies between the steps: no graph
+        for (int it = 0; it <= nSteps - 2; it++) {
+          const bool smp = it % sample_every == 0;
+          if (it > 0 && it % K == 0) TRY(snapshot(gp, sw.r0, it / K, true));
+          d0->launch("stress_fwd", smp, [&] { fwd_stress(sS, it, Uf); });
+          d0->launch("vel_fwd", smp, [&] { fwd_vel(sV, it, Uf); });
+        }
+      } else {
+        TRY(replay(graph_key("F", gp, {&mS, &mV}), [&] {
+          for (int it = 0; it <= nSteps - 2; it++) {
+            const bool smp = it % sample_every == 0;
+            d0->launch("stress_fwd", smp, [&] { fwd_stress(sS, it, Uf); });
+            d0->launch("vel_fwd", smp, [&] { fwd_vel(sV, it, Uf); });
+          }
+        }));
       }
     }
     HIP_TRY(hipGetLastError());
@@ -2414,8 +2489,8 @@ Status Runner::run(int calc_id) {
       for (size_t p = 0; p < gp.parts.size(); p++) {
         DeviceState& d = *M[gp.member[p]].d;
         const int s0 = gp.parts[p].g.s0, n = gp.parts[p].g.ns;
-        TRY(d.zero_shots(d.adj, 5, s0, n, st));
-        TRY(d.zero_shots(ckpt ? d.madj : d.mem, 8, s0, n, st));
+        TRY(d.zero_shots(d.adj, kWaveComps, s0, n, st));
+        TRY(d.zero_shots(ckpt ? d.madj : d.mem, kMemComps, s0, n, st));
       }
       DevMap mA, mB, mAV, mAS, mRV, mRS, mRW;
       TRY(maps_of(gp, "A", mA));
@@ -2441,37 +2516,14 @@ Status Runner::run(int calc_id) {
           const int c0 = d.coff[s0], nc = d.coff[s1] - c0;
           if (nc > 0)
             hipLaunchKernelGGL(k_res_inject, dim3((nc + 255) / 256), dim3(256), 0, st, d.g,
-                               d.wave(d.adj), d.icell, d.iptr, d.irid, c0, nc,
+                               d.adj, d.icell, d.iptr, d.irid, c0, nc,
                                d.res + (size_t)t * d.R);
         }
       };
-      // initial adjoint step at t = nSteps-1 (libCUFD.cu:353-373)
-      bwd_a(sAV, nSteps - 1, Ub);
-      inject(nSteps - 1);
-      if (nSteps < 2) {
-        bwd_b(sAS, nSteps - 1, Ub);
-        continue;
-      }
-      const int seg = ckpt ? K : steps, nseg = (steps + seg - 1) / seg;
-      for (int jj = nseg - 1; jj >= 0; jj--) {
-        const int t0 = jj * seg, t1 = std::min(t0 + seg, steps);
-        if (ckpt) {
-          // the segment's forward state from its checkpoint, then its ring
-          if (jj == 0) {
-            for (size_t p = 0; p < gp.parts.size(); p++) {
-              DeviceState& d = *M[gp.member[p]].d;
-              TRY(d.zero_shots(d.fwd, 5, gp.parts[p].g.s0, gp.parts[p].g.ns, st));
-              TRY(d.zero_shots(d.mem, 8, gp.parts[p].g.s0, gp.parts[p].g.ns, st));
-            }
-          } else {
-            TRY(snapshot(RG[j][q], sw.r0, jj, false));
-          }
-          for (int it = t0; it < t1; it++) {
-            fwd_stress(sRS, it, Uf);
-            fwd_vel(sRW, it, Uf);
-          }
-        }
-        if (jj == nseg - 1)
+      // reverse steps t1-1 .. t0 of one segment; `last` is the segment that
+      // ends at nSteps-1
+      auto segment = [&](int t0, int t1, bool last) {
+        if (last)
           bwd_b(sB, nSteps - 1, Ub);  // adj_stress(nSteps-1) + rev_vel(nSteps-2)
         else
           bwd_b(sRV, t1, Ub);         // rev_vel(t1-1)
@@ -2484,6 +2536,46 @@ Status Runner::run(int calc_id) {
           else
             bwd_b(sAS, it, Ub);
         }
+      };
+      if (!ckpt) {
+        // the whole backward sweep of the group, from the initial adjoint
+        // step at t = nSteps-1 (libCUFD.cu:353-373)
+        TRY(replay(graph_key("B", gp, {&mA, &mB, &mAV, &mAS, &mRV}), [&] {
+          bwd_a(sAV, nSteps - 1, Ub);
+          inject(nSteps - 1);
+          if (nSteps < 2)
+            bwd_b(sAS, nSteps - 1, Ub);
+          else
+            segment(0, steps, true);
+        }));
+        HIP_TRY(hipGetLastError());
+        continue;
+      }
+      // checkpoints: each segment is recomputed from its checkpoint, then
+      // reversed with its ring
+      bwd_a(sAV, nSteps - 1, Ub);
+      inject(nSteps - 1);
+      if (nSteps < 2) {
+        bwd_b(sAS, nSteps - 1, Ub);
+        continue;
+      }
+      const int nseg = (steps + K - 1) / K;
+      for (int jj = nseg - 1; jj >= 0; jj--) {
+        const int t0 = jj * K, t1 = std::min(t0 + K, steps);
+        if (jj == 0) {
+          for (size_t p = 0; p < gp.parts.size(); p++) {
+            DeviceState& d = *M[gp.member[p]].d;
+            TRY(d.zero_shots(d.fwd, kWaveComps, gp.parts[p].g.s0, gp.parts[p].g.ns, st));
+            TRY(d.zero_shots(d.mem, kMemComps, gp.parts[p].g.s0, gp.parts[p].g.ns, st));
+          }
+        } else {
+          TRY(snapshot(RG[j][q], sw.r0, jj, false));
+        }
+        for (int it = t0; it < t1; it++) {
+          fwd_stress(sRS, it, Uf);
+          fwd_vel(sRW, it, Uf);
+        }
+        segment(t0, t1, jj == nseg - 1);
       }
       HIP_TRY(hipGetLastError());
     }

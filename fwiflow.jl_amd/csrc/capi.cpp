@@ -76,14 +76,14 @@ Status host_checks(int calc_id, int group_size, const int* shot_ids, const char*
 Status cufd_impl(double* misfit, double* gL, double* gM, double* gD, double* gS,
                  const double* Lambda, const double* Mu, const double* Den, const double* stf,
                  int calc_id, int gpu_id, int group_size, const int* shot_ids,
-                 const char* para_fname) {
+                 const char* para_fname, bool in_scratch) {
   Para para;
   Survey sv;
   TRY(host_checks(calc_id, group_size, shot_ids, para_fname, Lambda, Mu, Den, para, sv));
   if (!stf) return Status::Err(FWIFLOW_ERR_ARG, "null stf pointer");
   if (calc_id == 0 && !misfit) return Status::Err(FWIFLOW_ERR_ARG, "null misfit pointer");
   Plan plan;
-  TRY(plan.create(para_fname, gpu_id, group_size, shot_ids));
+  TRY(plan.create(para_fname, gpu_id, group_size, shot_ids, in_scratch));
   TRY(plan.set_model(Lambda, Mu, Den, false));
   TRY(plan.set_stf(stf));
   if (calc_id != 2) TRY(plan.load_obs());
@@ -104,7 +104,16 @@ int fwiflow_cufd(double* misfit, double* grad_Lambda, double* grad_Mu, double* g
                  const double* stf, int calc_id, int gpu_id, int group_size,
                  const int* shot_ids, const char* para_fname) {
   return report(cufd_impl(misfit, grad_Lambda, grad_Mu, grad_Den, grad_stf, Lambda, Mu, Den,
-                          stf, calc_id, gpu_id, group_size, shot_ids, para_fname));
+                          stf, calc_id, gpu_id, group_size, shot_ids, para_fname, false));
+}
+
+int fwiflow_cufd_noalloc(double* misfit, double* grad_Lambda, double* grad_Mu,
+                         double* grad_Den, double* grad_stf, const double* Lambda,
+                         const double* Mu, const double* Den, const double* stf, int calc_id,
+                         int gpu_id, int group_size, const int* shot_ids,
+                         const char* para_fname) {
+  return report(cufd_impl(misfit, grad_Lambda, grad_Mu, grad_Den, grad_stf, Lambda, Mu, Den,
+                          stf, calc_id, gpu_id, group_size, shot_ids, para_fname, true));
 }
 
 int fwiflow_check(int calc_id, int group_size, const int* shot_ids, const char* para_fname,

@@ -52,6 +52,14 @@
 
 #pragma clang fp contract(off)
 
+// Device memory of the allocation-free entry point (fwiflow_cufd_noalloc):
+// every buffer of such a call is carved out of this array, which each GPU
+// the library runs on holds from module load to process end.  Host code
+// never passes the array's name to a kernel or a copy (on the host it is a
+// stand-in, not a device address): it uses the address that
+// hipGetSymbolAddress returned for the current device (scratch_device).
+static __device__ __align__(256) unsigned char scratch[FWIFLOW_SCRATCH_BYTES];
+
 namespace fwiflow {
 
 #define HIP_TRY(expr)                                                                    \
@@ -1334,8 +1342,33 @@ struct DevMap {
   int n = 0;  // workgroups of the launch
 };
 
+// Bump allocator over a fixed device region (the scratch array of the
+// allocation-free entry point): 256-byte aligned pieces, never returned
+// before the region is dropped as a whole.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  Status take(void** p, size_t nb) {
+    const size_t at = (used + 255) & ~(size_t)255;
+    const size_t n = nb ? nb : 4;
+    if (at > cap || n > cap - at)
+      return Status::Err(FWIFLOW_ERR_MEMORY,
+                         "device scratch exhausted: " + std::to_string(n) + " more bytes needed, " +
+                             std::to_string(at) + " of the " + std::to_string(cap) +
+                             " scratch bytes (FWIFLOW_SCRATCH_BYTES) in use");
+    *p = base + at;
+    used = at + n;
+    return Status::Ok();
+  }
+  size_t free_bytes() const {
+    const size_t at = (used + 255) & ~(size_t)255;
+    return at < cap ? cap - at : 0;
+  }
+};
+
 struct JobCache {
   int dev = 0;
+  Arena* arena = nullptr;  // device memory comes from here instead of hipMalloc when set
   std::map<std::string, DevMap> maps;
   Part* table = nullptr;  // part tables of the multi-part launches of the last run
   size_t table_cap = 0;
@@ -1343,8 +1376,14 @@ struct JobCache {
   ~JobCache() {
     (void)hipSetDevice(dev);
     clear_graphs();
+    if (arena) return;
     for (auto& kv : maps) (void)hipFree(kv.second.ptr);
     if (table) (void)hipFree(table);
+  }
+  Status device_alloc(void** p, size_t nb) {
+    if (arena) return arena->take(p, nb);
+    HIP_TRY(hipMalloc(p, nb));
+    return Status::Ok();
   }
   void clear_graphs() {
     for (auto& kv : graphs) {
@@ -1365,7 +1404,7 @@ struct JobCache {
     const std::vector<unsigned> m = flatten(L);
     DevMap dm;
     dm.n = (int)m.size();
-    HIP_TRY(hipMalloc(&dm.ptr, std::max<size_t>(1, m.size()) * sizeof(unsigned)));
+    TRY(device_alloc((void**)&dm.ptr, std::max<size_t>(1, m.size()) * sizeof(unsigned)));
     if (!m.empty())
       HIP_TRY(hipMemcpy(dm.ptr, m.data(), m.size() * sizeof(unsigned), hipMemcpyHostToDevice));
     maps[key] = dm;
@@ -1374,10 +1413,10 @@ struct JobCache {
   }
   Status upload_table(const std::vector<Part>& parts) {
     if (parts.size() > table_cap) {
-      if (table) HIP_TRY(hipFree(table));
+      if (table && !arena) HIP_TRY(hipFree(table));
       table = nullptr;
       table_cap = 0;
-      HIP_TRY(hipMalloc(&table, parts.size() * sizeof(Part)));
+      TRY(device_alloc((void**)&table, parts.size() * sizeof(Part)));
       table_cap = parts.size();
     }
     if (!parts.empty())
@@ -1458,6 +1497,11 @@ struct DeviceState {
   std::unique_ptr<JobCache> cache;  // maps of this plan's own runs
   std::vector<void*> allocs;
   long long bytes = 0;
+  // allocation-free plans: buffers carved from the device scratch array, the
+  // stream shared by all such plans of the device
+  Arena arena_store;
+  Arena* arena = nullptr;
+  bool own_stream = true;
   // timing
   bool timing = false;
   using Rec = TimedLaunch;
@@ -1476,13 +1520,19 @@ struct DeviceState {
     }
     for (auto e : pool) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 
   template <typename T>
   Status alloc(T** p, size_t count) {
     void* q = nullptr;
     const size_t nb = count * sizeof(T);
+    if (arena) {
+      TRY(arena->take(&q, nb));
+      bytes += (long long)nb;
+      *p = (T*)q;
+      return Status::Ok();
+    }
     hipError_t e = hipMalloc(&q, nb ? nb : 4);
     if (e != hipSuccess)
       return Status::Err(e == hipErrorOutOfMemory ? FWIFLOW_ERR_MEMORY : FWIFLOW_ERR_HIP,
@@ -1495,6 +1545,11 @@ struct DeviceState {
   template <typename T>
   void release(T*& p, size_t count) {
     if (!p) return;
+    if (arena) {  // arena pieces are dropped with the plan
+      bytes -= (long long)(count * sizeof(T));
+      p = nullptr;
+      return;
+    }
     auto it = std::find(allocs.begin(), allocs.end(), (void*)p);
     if (it != allocs.end()) allocs.erase(it);
     (void)hipFree(p);
@@ -1571,10 +1626,43 @@ struct DeviceState {
   }
 };
 
+// Address of the scratch array on each device and the stream of the
+// allocation-free plans there, looked up once per device with that device
+// current and kept until the process ends.
+namespace {
+
+struct ScratchDevice {
+  bool ready = false;
+  char* ptr = nullptr;
+  hipStream_t stream = nullptr;
+};
+std::mutex g_scratch_mu;
+std::vector<ScratchDevice> g_scratch_dev;
+
+Status scratch_device(int dev, char** base, hipStream_t* stream) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1);
+  ScratchDevice& sd = g_scratch_dev[dev];
+  if (!sd.ready) {
+    HIP_TRY(hipSetDevice(dev));
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    HIP_TRY(hipStreamCreate(&sd.stream));
+    sd.ptr = (char*)ptr;
+    sd.ready = true;
+  }
+  *base = sd.ptr;
+  *stream = sd.stream;
+  return Status::Ok();
+}
+
+}  // namespace
+
 Plan::Plan() = default;
 Plan::~Plan() = default;
 
-Status Plan::create(const std::string& para_fname, int gpu, int group_size, const int* shot_ids) {
+Status Plan::create(const std::string& para_fname, int gpu, int group_size, const int* shot_ids,
+                    bool in_scratch) {
   TRY(parse_para(para_fname, para));
   TRY(parse_survey(para, group_size, shot_ids, survey));
   gpu_id = gpu;
@@ -1603,8 +1691,19 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.fold_inject = env_int("FWIFLOW_FOLD_INJECT", 1) != 0;
   d.part_table = env_int("FWIFLOW_PART_TABLE", 0) != 0;
   d.graphs = env_int("FWIFLOW_GRAPHS", 0) != 0;
+  if (in_scratch) {
+    // every buffer from the device's scratch array, no graphs (instantiating
+    // one allocates), the device's shared stream
+    char* base = nullptr;
+    TRY(scratch_device(gpu, &base, &d.stream));
+    d.own_stream = false;
+    d.arena_store.base = base;
+    d.arena_store.cap = FWIFLOW_SCRATCH_BYTES;
+    d.arena = &d.arena_store;
+    d.graphs = false;
+  }
   HIP_TRY(hipSetDevice(gpu));
-  HIP_TRY(hipStreamCreate(&d.stream));
+  if (!d.stream) HIP_TRY(hipStreamCreate(&d.stream));
   Geo& g = d.g;
   g.nz = para.nz;
   g.nx = para.nx;
@@ -2323,7 +2422,20 @@ Status Runner::run(int calc_id) {
   gb = std::min(gb, cap);
   // memory plan of a gradient run: rows per sweep S and checkpoint interval K
   const int steps = std::max(1, nSteps - 1);  // forward steps 0 .. nSteps-2
-  const double budget = c0.ring_budget;
+  double budget = c0.ring_budget;
+  if (withAdj && c0.arena && M.size() == 1) {
+    // scratch-backed plan: ring + checkpoints get what the scratch array has
+    // left after the gradient run's other buffers (adjoint fields, C-PML
+    // adjoint memory variables, accumulators, misfit partials, maps)
+    const DeviceState& d = c0;
+    const double N = (double)d.g.N, F = (double)d.F();
+    const double other = 4.0 * ((d.adj ? 0 : 5 * F) + (d.madj ? 0 : 8 * F) +
+                                (d.acc ? 0 : 5 * N) + (d.grad ? 0 : 3 * N) +
+                                (d.stf_grad ? 0 : (double)d.g.ns * nSteps) +
+                                (d.illum_on && !d.illum ? N : 0)) +
+                         8.0 * rows * kPartials + (1 << 20);
+    budget = std::min(budget, std::max(1.0, (double)d.arena->free_bytes() - other));
+  }
   K = withAdj ? c0.ckpt_every : 0;
   if (K >= steps) K = 0;
   S = rows;
@@ -2616,6 +2728,7 @@ Status Plan::run(int calc_id) {
   if (!d_->cache) {
     d_->cache.reset(new JobCache());
     d_->cache->dev = d_->dev;
+    d_->cache->arena = d_->arena;
   }
   Runner r({Member{this, d_.get()}}, *d_->cache);
   return r.run(calc_id);
@@ -2635,6 +2748,8 @@ Status Job::add(Plan* p) {
     return Status::Err(FWIFLOW_ERR_ARG, "at most " + std::to_string(kMaxPartsTable) + " plans per job");
   for (Plan* q : plans_)
     if (q == p) return Status::Err(FWIFLOW_ERR_ARG, "plan already in the job");
+  if (p->d_->arena)
+    return Status::Err(FWIFLOW_ERR_ARG, "a plan in the device scratch cannot join a job");
   if (!plans_.empty()) {
     const DeviceState& a = *plans_[0]->d_;
     if (p->d_->dev != a.dev) return Status::Err(FWIFLOW_ERR_ARG, "plans of a job must share the GPU");

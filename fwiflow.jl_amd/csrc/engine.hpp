@@ -18,7 +18,10 @@ class Plan {
  public:
   Plan();
   ~Plan();
-  Status create(const std::string& para_fname, int gpu_id, int group_size, const int* shot_ids);
+  // in_scratch: every device buffer carved from the device's fixed scratch
+  // array (FWIFLOW_SCRATCH_BYTES) instead of hipMalloc
+  Status create(const std::string& para_fname, int gpu_id, int group_size, const int* shot_ids,
+                bool in_scratch = false);
   Status set_model(const double* Lambda, const double* Mu, const double* Den, bool zfast);
   Status set_stf(const double* stf);
   Status set_obs(const float* obs);

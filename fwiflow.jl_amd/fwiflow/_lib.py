@@ -23,7 +23,7 @@ STATUS_CODE = {v: k for k, v in STATUS.items()}
 
 # every symbol include/fwiflow.h declares
 EXPORTS = [
-    "fwiflow_cufd", "fwiflow_check", "fwiflow_host_tables", "fwiflow_last_error",
+    "fwiflow_cufd", "fwiflow_cufd_noalloc", "fwiflow_check", "fwiflow_host_tables", "fwiflow_last_error",
     "fwiflow_version", "fwiflow_plan_create", "fwiflow_plan_destroy", "fwiflow_plan_set_model",
     "fwiflow_plan_set_model_zfast", "fwiflow_plan_set_stf", "fwiflow_plan_set_obs",
     "fwiflow_plan_load_obs", "fwiflow_plan_run", "fwiflow_plan_misfit",
@@ -64,6 +64,7 @@ def load():
     L = C.CDLL(LIB_PATH)
     L.fwiflow_cufd.argtypes = [dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_int, C.c_int, C.c_int, ip,
                                C.c_char_p]
+    L.fwiflow_cufd_noalloc.argtypes = L.fwiflow_cufd.argtypes
     L.fwiflow_check.argtypes = [C.c_int, C.c_int, ip, C.c_char_p, dp, dp, dp]
     L.fwiflow_host_tables.argtypes = [C.c_char_p, fp, fp, fp, fp]
     L.fwiflow_last_error.restype = C.c_char_p

@@ -28,8 +28,10 @@ def _args(lam, mu, den, stf, shot_ids):
     return lam, mu, den, stf, i32(shot_ids)
 
 
-def cufd(lam, mu, den, stf, calc_id, gpu_id, shot_ids, para_fname):
-    """Direct binding of ``fwiflow_cufd`` (libCUFD.cu:34-38)."""
+def cufd(lam, mu, den, stf, calc_id, gpu_id, shot_ids, para_fname, noalloc=False):
+    """Direct binding of ``fwiflow_cufd`` (libCUFD.cu:34-38); ``noalloc=True``
+    calls ``fwiflow_cufd_noalloc``, which takes every device buffer from the
+    library's fixed per-GPU scratch array instead of allocating."""
     L = _lib.load()
     lam, mu, den, stf, ids = _args(lam, mu, den, stf, shot_ids)
     misfit = np.zeros(1)
@@ -38,9 +40,10 @@ def cufd(lam, mu, den, stf, calc_id, gpu_id, shot_ids, para_fname):
     gD = np.zeros_like(lam) if calc_id == 1 else None
     gS = np.zeros_like(stf) if calc_id == 1 else None
     D = _lib.C.c_double
-    rc = L.fwiflow_cufd(ptr(misfit, D), ptr(gL, D), ptr(gM, D), ptr(gD, D), ptr(gS, D),
-                        ptr(lam, D), ptr(mu, D), ptr(den, D), ptr(stf, D), int(calc_id),
-                        int(gpu_id), len(ids), ptr(ids, _lib.C.c_int), str(para_fname).encode())
+    fn = L.fwiflow_cufd_noalloc if noalloc else L.fwiflow_cufd
+    rc = fn(ptr(misfit, D), ptr(gL, D), ptr(gM, D), ptr(gD, D), ptr(gS, D),
+            ptr(lam, D), ptr(mu, D), ptr(den, D), ptr(stf, D), int(calc_id),
+            int(gpu_id), len(ids), ptr(ids, _lib.C.c_int), str(para_fname).encode())
     check(rc)
     return float(misfit[0]), gL, gM, gD, gS
 

@@ -61,6 +61,34 @@ int fwiflow_cufd(double* misfit, double* grad_Lambda, double* grad_Mu, double* g
                  const double* stf, int calc_id, int gpu_id, int group_size,
                  const int* shot_ids, const char* para_fname);
 
+/*
+ * Device scratch of fwiflow_cufd_noalloc on each GPU, in bytes: a fixed
+ * array in the library's device code, held by every GPU the library runs on
+ * from the code object's load to the end of the process.
+ */
+#define FWIFLOW_SCRATCH_BYTES (128ull << 20)
+
+/*
+ * fwiflow_cufd without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, files and results as fwiflow_cufd; every device buffer of the
+ * call (model, wavefields, seismograms, boundary ring or checkpoints) is
+ * carved from the FWIFLOW_SCRATCH_BYTES array of the GPU, and the call makes
+ * no hipMalloc, hipFree or graph.  A gradient run whose boundary ring does
+ * not fit in what is left keeps checkpoints instead (the interval chosen to
+ * fit), so its gradients can differ from fwiflow_cufd's within the FP32
+ * tolerance of that scheme.  A call whose buffers exceed the array returns
+ * FWIFLOW_ERR_MEMORY naming the shortfall.  Calls on one GPU share its
+ * array and its one stream: calls that overlap on one GPU (from several
+ * threads) race; the array keeps the last call's contents.
+ */
+int fwiflow_cufd_noalloc(double* misfit, double* grad_Lambda, double* grad_Mu,
+                         double* grad_Den, double* grad_stf, const double* Lambda,
+                         const double* Mu, const double* Den, const double* stf, int calc_id,
+                         int gpu_id, int group_size, const int* shot_ids,
+                         const char* para_fname);
+
 /* Host-only validation of a cufd call: parses the JSON files, checks calc_id,
  * shot ids, source/receiver positions and the Courant number exactly as
  * fwiflow_cufd does before touching a GPU.  Needs no GPU.  No counterpart in

@@ -6,9 +6,9 @@
 // Device data layout (all FP32, z fastest so a wave64 row is 64 consecutive
 // depth cells, one 256-byte coalesced access):
 //   model      lam, mu, ave_mu, byc_a, byc_b          [NX][NZ]
-//   wavefield  vz, vx, szz, sxx, sxz                  [component][shot][NX][NZ]
-//   adjoint    same 5 components                      [component][shot][NX][NZ]
-//   C-PML memory variables (8)                        [variable][shot][NX][NZ]
+//   wavefield  vz, vx, szz, sxx, sxz                  [shot][component][NX][NZ]
+//   adjoint    same 5 components                      [shot][component][NX][NZ]
+//   C-PML memory variables (8)                        [shot][variable][NX][NZ]
 //   boundary ring, 5 layers x 4 sides, 5 components   [shot][step][component][ring slot]
 //   seismograms syn/obs/res, time-major               [nSteps][all receivers]
 //   gradient accumulators (5, summed over shots)      [NX][NZ]

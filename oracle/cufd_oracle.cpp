@@ -68,6 +68,25 @@ struct Prm {
   explicit Prm(const Geom& g) : dz((T)g.dz), dx((T)g.dx), dt((T)g.dt), f0((T)g.f0) {}
 };
 
+// ---------------------------------------------------------------- planted mistakes
+// Mistakes a restatement of the reference could plausibly make, planted on
+// request so the oracle's own tests can show that they would catch them
+// (0: none, the reference as written).  Set by oracle_set_mutation.
+enum Mutation {
+  kMutNone = 0,
+  kMutVelGate = 1,         // el_velocity: right x-PML gate of el_stress (x > nx-nPml-1)
+  kMutAdjVelASwap = 2,     // el_velocity_adj vx: a_x_half in place of a_x
+  kMutAdjVelKSwap = 3,     // el_velocity_adj vx: K_x_half in place of K_x
+  kMutAdjVelDropPsi = 4,   // el_velocity_adj vx: a_x * d(psi_xx)/dx dropped
+  kMutAdjStressDropPsi = 5,  // el_stress_adj sxz: a_x * d(phi_xz_x)/dx dropped
+  kMutAdjStressASwap = 6,  // el_stress_adj sxz: a_x_half in place of a_x
+};
+// Not a mistake, a variant (a flag combined with the above): el_stress_adj
+// with the PML gates that the reference comments out (el_stress_adj.cu:66-72,
+// 89-96) restored, so its memory variables stay zero in the interior.
+constexpr int kVarAdjStressGated = 0x100;
+static int g_mutation = kMutNone;
+
 // ---------------------------------------------------------------- C-PML
 // utilities.cu:242-358 (cpmlInit).  CpAve is forced to 3000 (utilities.cu:259).
 template <typename T>
@@ -518,6 +537,7 @@ void el_velocity_fwd(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpm
   T *mszz = s.m_dszz_dz.data(), *msxzx = s.m_dsxz_dx.data(), *msxzz = s.m_dsxz_dz.data(),
     *msxx = s.m_dsxx_dx.data();
   const T *ba = m.byc_a.data(), *bb = m.byc_b.data();
+  const int xr = (g_mutation & 0xff) == kMutVelGate ? nx - nPml - 1 : nx - nPml;
   OMP_FOR
   for (int x = 2; x <= nx - 3; x++) {
     for (int z = 2; z <= nz - nPad - 3; z++) {
@@ -528,7 +548,7 @@ void el_velocity_fwd(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpm
         mszz[i] = pm.b_z_half[z] * mszz[i] + pm.a_z_half[z] * dszz_dz;
         dszz_dz = dszz_dz / pm.K_z_half[z] + mszz[i];
       }
-      if (x < nPml || x > nx - nPml) {
+      if (x < nPml || x > xr) {
         msxzx[i] = pm.b_x[x] * msxzx[i] + pm.a_x[x] * dsxz_dx;
         dsxz_dx = dsxz_dx / pm.K_x[x] + msxzx[i];
       }
@@ -539,7 +559,7 @@ void el_velocity_fwd(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpm
         msxzz[i] = pm.b_z[z] * msxzz[i] + pm.a_z[z] * dsxz_dz;
         dsxz_dz = dsxz_dz / pm.K_z[z] + msxzz[i];
       }
-      if (x < nPml || x > nx - nPml) {
+      if (x < nPml || x > xr) {
         msxx[i] = pm.b_x_half[x] * msxx[i] + pm.a_x_half[x] * dsxx_dx;
         dsxx_dx = dsxx_dx / pm.K_x_half[x] + msxx[i];
       }
@@ -671,9 +691,11 @@ void el_velocity_adj(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpm
       T dsxx_dx = (-c1 * (sxx[i + nz] - sxx[i]) + c2 * (sxx[i + 2 * nz] - sxx[i - nz])) / dx;
       T dpsixz_dz = (-c1 * (mvxz[i] - mvxz[i - 1]) + c2 * (mvxz[i + 1] - mvxz[i - 2])) / dz;
       T dsxz_dz = (-c1 * (sxz[i] - sxz[i - 1]) + c2 * (sxz[i + 1] - sxz[i - 2])) / dz;
-      vx[i] += (pm.a_x[x] * dpsixx_dx + lambda * dszz_dx / pm.K_x[x] * dt +
-                (lambda + 2.0 * muv) * dsxx_dx / pm.K_x[x] * dt + pm.a_z_half[z] * dpsixz_dz +
-                amu[i] / pm.K_z_half[z] * dsxz_dz * dt);
+      const T ax = (g_mutation & 0xff) == kMutAdjVelASwap ? pm.a_x_half[x] : pm.a_x[x];
+      const T kx = (g_mutation & 0xff) == kMutAdjVelKSwap ? pm.K_x_half[x] : pm.K_x[x];
+      const T psi_xx = (g_mutation & 0xff) == kMutAdjVelDropPsi ? (T)0 : ax * dpsixx_dx;
+      vx[i] += (psi_xx + lambda * dszz_dx / kx * dt + (lambda + 2.0 * muv) * dsxx_dx / kx * dt +
+                pm.a_z_half[z] * dpsixz_dz + amu[i] / pm.K_z_half[z] * dsxz_dz * dt);
       if (x < nPml || x > nx - nPml - 1) msxx[i] = pm.b_x_half[x] * msxx[i] + bb[i] * vx[i] * dt;
       if (z < nPml || (z > nz - nPml - nPad - 1))
         msxzz[i] = pm.b_z[z] * msxzz[i] + bb[i] * vx[i] * dt;
@@ -698,7 +720,7 @@ void el_velocity_adj(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpm
 template <typename T>
 void el_stress_adj(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpml<T>& pm,
                    State<T>& a) {
-  const int nz = c.g.nz, nx = c.g.nx, nPad = c.g.nPad, nthreads = c.nthreads;
+  const int nz = c.g.nz, nx = c.g.nx, nPml = c.g.nPml, nPad = c.g.nPad, nthreads = c.nthreads;
   const T c1 = 9.0 / 8.0, c2 = 1.0 / 24.0;
   const T dz = p.dz, dx = p.dx, dt = p.dt;
   const T *vz = a.vz.data(), *vx = a.vx.data();
@@ -719,10 +741,14 @@ void el_stress_adj(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpml<
       T dvz_dx = (-c1 * (vz[i + nz] - vz[i]) + c2 * (vz[i + 2 * nz] - vz[i - nz])) / dx;
       T dphi_xz_z_dz = (-c1 * (msxzz[i + 1] - msxzz[i]) + c2 * (msxzz[i + 2] - msxzz[i - 1])) / dz;
       T dvx_dz = (-c1 * (vx[i + 1] - vx[i]) + c2 * (vx[i + 2] - vx[i - 1])) / dz;
-      sxz[i] += pm.a_x[x] * dphi_xz_x_dx + dvz_dx / pm.K_x[x] * ba[i] * dt +
-                pm.a_z[z] * dphi_xz_z_dz + dvx_dz / pm.K_z[z] * bb[i] * dt;
-      mvzx[i] = pm.b_x_half[x] * mvzx[i] + sxz[i] * amu[i] * dt;
-      mvxz[i] = pm.b_z_half[z] * mvxz[i] + sxz[i] * amu[i] * dt;
+      const T ax = (g_mutation & 0xff) == kMutAdjStressASwap ? pm.a_x_half[x] : pm.a_x[x];
+      const T phi_xz_x = (g_mutation & 0xff) == kMutAdjStressDropPsi ? (T)0 : ax * dphi_xz_x_dx;
+      sxz[i] += phi_xz_x + dvz_dx / pm.K_x[x] * ba[i] * dt + pm.a_z[z] * dphi_xz_z_dz +
+                dvx_dz / pm.K_z[z] * bb[i] * dt;
+      const bool gated = (g_mutation & kVarAdjStressGated) != 0;
+      const bool xin = x < nPml || x > nx - nPml - 1, zin = z < nPml || z > nz - nPml - nPad - 1;
+      if (!gated || xin) mvzx[i] = pm.b_x_half[x] * mvzx[i] + sxz[i] * amu[i] * dt;
+      if (!gated || zin) mvxz[i] = pm.b_z_half[z] * mvxz[i] + sxz[i] * amu[i] * dt;
       T dphi_xx_x_dx =
           (-c1 * (msxx[i] - msxx[i - nz]) + c2 * (msxx[i + nz] - msxx[i - 2 * nz])) / dx;
       T dvx_dx = (-c1 * (vx[i] - vx[i - nz]) + c2 * (vx[i + nz] - vx[i - 2 * nz])) / dx;
@@ -730,8 +756,10 @@ void el_stress_adj(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpml<
       T dvz_dz = (-c1 * (vz[i] - vz[i - 1]) + c2 * (vz[i + 1] - vz[i - 2])) / dz;
       sxx[i] += pm.a_x_half[x] * dphi_xx_x_dx + bb[i] * dvx_dx / pm.K_x_half[x] * dt;
       szz[i] += pm.a_z_half[z] * dphi_zz_z_dz + ba[i] * dvz_dz / pm.K_z_half[z] * dt;
-      mvxx[i] = pm.b_x[x] * mvxx[i] + lambda * szz[i] * dt + (lambda + 2.0 * muv) * sxx[i] * dt;
-      mvzz[i] = pm.b_z[z] * mvzz[i] + (lambda + 2.0 * muv) * szz[i] * dt + lambda * sxx[i] * dt;
+      if (!gated || xin)
+        mvxx[i] = pm.b_x[x] * mvxx[i] + lambda * szz[i] * dt + (lambda + 2.0 * muv) * sxx[i] * dt;
+      if (!gated || zin)
+        mvzz[i] = pm.b_z[z] * mvzz[i] + (lambda + 2.0 * muv) * szz[i] * dt + lambda * sxx[i] * dt;
     }
   }
 }
@@ -1079,6 +1107,10 @@ int run_all(const Ctx& c, int scheme, int ckpt_every, int calc_id, int group_siz
 }  // namespace
 
 extern "C" {
+
+// Plant one of the Mutation mistakes, optionally with the kVarAdjStressGated
+// flag (0 removes both), for the oracle's tests.
+void oracle_set_mutation(int m) { g_mutation = m; }
 
 typedef struct {
   int nz, nx, nPml, nPad, nSteps;

@@ -64,6 +64,7 @@ def lib():
                                   C.c_double, dp]
         L.oracle_model.argtypes = [C.c_int, C.POINTER(Geom), dp, dp, dp, dp]
         L.oracle_window.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dp]
+        L.oracle_set_mutation.argtypes = [C.c_int]
         L.oracle_ring.restype = C.c_int
         L.oracle_ring.argtypes = [C.POINTER(Geom), C.POINTER(C.c_longlong)]
         _lib = L
@@ -72,6 +73,31 @@ def lib():
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+MUTATIONS = {"velocity_gate_of_stress": 1, "adj_vel_a_half": 2, "adj_vel_K_half": 3,
+             "adj_vel_drop_psi": 4, "adj_stress_drop_psi": 5, "adj_stress_a_half": 6,
+             "adj_stress_gated": 0x100}
+
+
+class mutation:
+    """Context manager planting named mistakes in the oracle (for the
+    oracle's own tests: checks that must fail under them), optionally with
+    the one named variant, ``adj_stress_gated`` (the reference's
+    commented-out adjoint memory-variable gates restored)."""
+
+    def __init__(self, *names):
+        self.code = 0
+        for n in names:
+            if n is not None:
+                self.code |= MUTATIONS[n]
+
+    def __enter__(self):
+        lib().oracle_set_mutation(self.code)
+        return self
+
+    def __exit__(self, *a):
+        lib().oracle_set_mutation(0)
 
 
 def geom(nz, nx, nPml, nPad, nSteps, dz, dx, dt, f0) -> Geom:

@@ -132,3 +132,55 @@ def test_learned_descent_with_fallback_reduces_misfit(workspace):
     assert x.shape == (4,) + cp0.shape
     t = target_from_update(np.ones(cp0.shape), fwi.mask)
     assert t.shape == (3,) + cp0.shape and np.all(t[1:] == 0)
+
+
+@pytest.mark.timeout(900)
+def test_config5_learned_lbfgs_on_a_held_out_model(workspace):
+    """The learned preconditioner at its own scale: the 134x384 model family
+    (224x448 with the 32-cell C-PML, nt 2000, 30 shots, elastic top rows).
+    Training pairs (raw gradients + illumination -> accepted update) are
+    logged by the L-BFGS control driver on two training seeds; the network
+    trains on them with the HIP kernels; on a held-out seed the L-BFGS driver
+    with the network as its initial inverse Hessian takes steps that each
+    lower the operator's misfit.  With every network parameter zero the
+    network gives no direction, so every step takes the fallback (the
+    illumination-scaled gradient), and the misfit still falls."""
+    import bench_precond as B
+    from fwiflow import precond
+
+    samples, mask = [], None
+    for seed in (100, 101):
+        fwi, obj, x0 = B.setup(seed, 2000, os.path.join(workspace, f"train{seed}"))
+        mask = fwi.mask
+
+        def on_pair(g, extras, update, J):
+            samples.append((precond.features(extras["gradients"], extras["illumination"], mask),
+                            precond.target_from_update(update, mask)))
+        try:
+            r = precond.learned_lbfgs(obj, x0, mask, net=None, maxiter=4, on_pair=on_pair)
+        finally:
+            obj.close()
+        say("training seed", seed, "misfit", r["history"][0][1] if r["history"] else None,
+            "->", r["misfit"], "pairs", len(samples))
+    assert mask.shape == (224, 448) and len(samples) >= 4
+    fwi, obj, x0 = B.setup(1100, 2000, os.path.join(workspace, "heldout"))
+    try:
+        J0 = obj(x0)[0]
+        with CNN(224, 448, seed=0) as net:
+            hist = train(net, samples, mask.astype(np.float32), epochs=3, lr=1e-3, seed=0)
+            say("training loss", hist)
+            assert np.isfinite(hist[-1]) and hist[-1] < hist[0]
+            rl = precond.learned_lbfgs(obj, x0, fwi.mask, net=net, maxiter=4)
+            net.set_params(np.zeros(net.num_params, dtype=np.float32))
+            rz = precond.learned_lbfgs(obj, x0, fwi.mask, net=net, maxiter=2)
+    finally:
+        obj.close()
+    for name, res in (("trained", rl), ("zero", rz)):
+        used = [h[2] for h in res["history"]]
+        ms = [J0] + [h[1] for h in res["history"]]
+        say("held-out", name, J0, "->", res["misfit"], "steps", used,
+            "evaluations", res["evaluations"])
+        assert res["iterations"] >= 1 and res["misfit"] < J0
+        assert all(b < a for a, b in zip(ms, ms[1:]))  # every accepted step descends
+        assert res["x"].min() >= 1500.0 and res["x"].max() <= 5500.0
+    assert all(h[2] in ("fallback", "restart") for h in rz["history"])

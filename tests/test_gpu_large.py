@@ -186,7 +186,7 @@ def test_config4_time_lapse_parity(workspace, grid):
     d = float(np.linalg.norm(data[True] - data[False]) / np.linalg.norm(data[False]))
     want = float(fixture(f"c4_{nz}x{nx}_timelapse")["monitor_rel"])
     say("monitor vs baseline", d, "oracle", want)
-    assert abs(d - want) <= 1e-3 * want
+    assert abs(d - want) <= 1e-2 * want
 
 
 @pytest.mark.timeout(600)

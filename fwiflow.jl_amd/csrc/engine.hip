@@ -32,6 +32,7 @@
 // Reference kernels restated here: el_stress.cu:50-131, el_velocity.cu:45-117,
 // el_velocity_adj.cu:22-108, el_stress_adj.cu:22-104, utilities.cu:125-205,
 // 361-424, 506-593, 653-747; driver sequence libCUFD.cu:157-463.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1433,7 +1434,7 @@ struct DeviceState {
   int R = 0;
   std::vector<int> roff;  // first receiver of each shot (+ total)
   // engine settings (environment variables read at plan creation)
-  int tile_x = 4;          // x extent of a workgroup tile (blockDim.y)
+  int tile_x = 2;          // x extent of a workgroup tile (blockDim.y)
   bool xcd = true;         // XCD-banded workgroup order
   bool balance = true;     // C-PML columns shared evenly by the XCDs (update-region launches)
   bool mix = true;         // backward pair halves interleaved (pair a)
@@ -1567,14 +1568,24 @@ struct DeviceState {
     (void)hipEventCreate(&e);
     return e;
   }
+  // a sampled step launch passes these to hipExtLaunchKernel, which stamps
+  // them when the kernel itself starts and ends (no launch gap inside)
+  hipEvent_t kernel_start = nullptr, kernel_end = nullptr;
   template <typename F>
   void launch(const char* name, bool sample, F&& f) {
-    if (timing && sample) {
-      Rec r{name, take_event(), take_event(), sink == nullptr};
+    if (timing && sample && !sink) {
+      Rec r{name, take_event(), take_event(), true};
+      kernel_start = r.a;
+      kernel_end = r.b;
+      f();
+      kernel_start = kernel_end = nullptr;
+      pending.push_back(r);
+    } else if (timing && sample) {
+      Rec r{name, take_event(), take_event(), false};
       (void)hipEventRecord(r.a, stream);
       f();
       (void)hipEventRecord(r.b, stream);
-      (sink ? *sink : pending).push_back(r);
+      sink->push_back(r);
     } else {
       f();
     }
@@ -1679,7 +1690,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d_.reset(new DeviceState());
   DeviceState& d = *d_;
   d.dev = gpu;
-  d.tile_x = std::max(1, std::min(4, env_int("FWIFLOW_TILE_X", 4)));  // <= 256 threads
+  d.tile_x = std::max(1, std::min(4, env_int("FWIFLOW_TILE_X", 2)));  // <= 256 threads
   d.xcd = env_int("FWIFLOW_XCD_REMAP", 1) != 0;
   d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
@@ -2205,6 +2216,16 @@ Status Runner::replay(const std::string& key, const std::function<void()>& body)
   return Status::Ok();
 }
 
+// a sampled launch (timing on) goes through hipExtLaunchKernel with the
+// plan's kernel_start / kernel_end events
+#define STEP_LAUNCH_ONE(K, a)                                                               \
+  do {                                                                                      \
+    if (d0->kernel_start)                                                                   \
+      hipExtLaunchKernelGGL(K, grid, blk, 0, st, d0->kernel_start, d0->kernel_end, 0, a,    \
+                            s.map, it);                                                     \
+    else                                                                                    \
+      hipLaunchKernelGGL(K, grid, blk, 0, st, a, s.map, it);                                \
+  } while (0)
 #define STEP_LAUNCH(KER)                                                                    \
   do {                                                                                      \
     if (!s.nwg) break;                                                                      \
@@ -2212,15 +2233,15 @@ Status Runner::replay(const std::string& key, const std::function<void()>& body)
     if (s.single) {                                                                         \
       const Parts<1> a{*s.single};                                                          \
       if (U == 2)                                                                           \
-        hipLaunchKernelGGL((KER<1, 2>), grid, blk, 0, st, a, s.map, it);                    \
+        STEP_LAUNCH_ONE((KER<1, 2>), a);                                                    \
       else                                                                                  \
-        hipLaunchKernelGGL((KER<1, 1>), grid, blk, 0, st, a, s.map, it);                    \
+        STEP_LAUNCH_ONE((KER<1, 1>), a);                                                    \
     } else {                                                                                \
       const Parts<0> a{s.table};                                                            \
       if (U == 2)                                                                           \
-        hipLaunchKernelGGL((KER<0, 2>), grid, blk, 0, st, a, s.map, it);                    \
+        STEP_LAUNCH_ONE((KER<0, 2>), a);                                                    \
       else                                                                                  \
-        hipLaunchKernelGGL((KER<0, 1>), grid, blk, 0, st, a, s.map, it);                    \
+        STEP_LAUNCH_ONE((KER<0, 1>), a);                                                    \
     }                                                                                       \
   } while (0)
 
@@ -2229,6 +2250,7 @@ void Runner::fwd_vel(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_fwd_vel
 void Runner::bwd_a(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_a); }
 void Runner::bwd_b(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_b); }
 #undef STEP_LAUNCH
+#undef STEP_LAUNCH_ONE
 
 GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
   GroupParts gp;

@@ -96,13 +96,27 @@ def FWIExample():
 
 
 def try_pad(fwi, a):
-    """FWI.jl:216-223 (input already of size (nz, nx) or padded)."""
+    """FWI.jl:216-223: an input of the padded size is returned as is, any
+    other is padded.  The reference's padding (FWI.jl:193-204) computes a
+    bilinear resize of an input that is not (nz, nx) but pads the input
+    itself, so the result keeps the input's size plus the C-PML and pad
+    cells; so does this one, and the operator calls below reject an array
+    that is then not of the grid's size (``_on_grid``)."""
     a = np.asarray(a, dtype=np.float64)
     if a.shape == (fwi.nz_pad, fwi.nx_pad):
         return a
-    if a.shape != (fwi.nz, fwi.nx):
-        raise ValueError("resizing (tf.image.resize_bilinear) is not supported; pass (nz, nx)")
     return padding(a, fwi.nPml, fwi.nPad)
+
+
+def _on_grid(fwi, *arrays):
+    """The operator reads nz_pad x nx_pad values of each model array (the
+    C ABI takes no sizes): refuse any other shape before the call."""
+    for a in arrays:
+        if a.shape != (fwi.nz_pad, fwi.nx_pad):
+            raise ValueError(f"model array of shape {a.shape} after padding; the grid is "
+                             f"{fwi.nz_pad}x{fwi.nx_pad} (pass (nz, nx) = ({fwi.nz}, {fwi.nx}) "
+                             "or the padded size; inputs are not resized)")
+    return arrays
 
 
 def _stf_rows(fwi, stf_array):
@@ -114,7 +128,7 @@ def _stf_rows(fwi, stf_array):
 
 def compute_observation(fwi, cp, cs, rho, stf_array, shot_ids=None, gpu_id=0):
     """FWI.jl:109-135.  Returns data[g, nSteps, nrec] (float64 of the f32 files)."""
-    cp_pad, cs_pad, rho_pad = (try_pad(fwi, a) for a in (cp, cs, rho))
+    cp_pad, cs_pad, rho_pad = _on_grid(fwi, *(try_pad(fwi, a) for a in (cp, cs, rho)))
     stf = _stf_rows(fwi, stf_array)
     lam, mu = velocity_to_moduli(cp_pad, cs_pad, rho_pad)
     if shot_ids is None:
@@ -130,13 +144,13 @@ def compute_observation(fwi, cp, cs, rho, stf_array, shot_ids=None, gpu_id=0):
 
 
 def _masked(fwi, cp, cs, rho, is_masked, cp_ref, cs_ref, rho_ref):
-    cp_pad, cs_pad, rho_pad = (try_pad(fwi, a) for a in (cp, cs, rho))
+    cp_pad, cs_pad, rho_pad = _on_grid(fwi, *(try_pad(fwi, a) for a in (cp, cs, rho)))
     if is_masked:
         return cp_pad, cs_pad, rho_pad
     if cp_ref is None or cs_ref is None or rho_ref is None:
         # FWI.jl:174-176 blends with the reference model, which must be given
         raise ValueError("is_masked=False needs cp_ref, cs_ref and rho_ref")
-    refs = [try_pad(fwi, a) for a in (cp_ref, cs_ref, rho_ref)]
+    refs = _on_grid(fwi, *(try_pad(fwi, a) for a in (cp_ref, cs_ref, rho_ref)))
     m, mn = fwi.mask, fwi.mask_neg
     return (cp_pad * m + refs[0] * mn, cs_pad * m + refs[1] * mn, rho_pad * m + refs[2] * mn)
 

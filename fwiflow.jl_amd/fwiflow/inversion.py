@@ -33,7 +33,7 @@ from collections import OrderedDict
 import numpy as np
 from scipy.optimize import minimize
 
-from .fwi import _stf_rows, try_pad
+from .fwi import _on_grid, _stf_rows, try_pad
 from .ops import Plan
 from .utils import moduli_to_velocity_grad, velocity_to_moduli
 
@@ -67,6 +67,7 @@ class PlanObjective:
         self.cp_ref = try_pad(fwi, cp_ref)
         self.cs_ref = self.cs if cs_ref is None else try_pad(fwi, cs_ref)
         self.rho_ref = self.rho if rho_ref is None else try_pad(fwi, rho_ref)
+        _on_grid(fwi, self.cs, self.rho, self.cp_ref, self.cs_ref, self.rho_ref)
         self.mask = fwi.mask
         self.plan = Plan(fwi.para_path, self.ids, gpu_id)
         self.plan.set_stf(stf)

@@ -193,8 +193,10 @@ def test_config4_time_lapse_parity(workspace, grid):
 def test_config4_surveys_in_one_job(workspace):
     """The six config-4 surveys (three grids, baseline and monitor; two shots
     each, 3000 steps) run as one job, every time step one launch across
-    grids: misfits, data and gradients equal each survey's plan run alone,
-    bit for bit."""
+    grids: misfits and data equal each survey's plan run alone bit for bit,
+    gradients to 1e-6 (a job launch covers fewer shots of each survey than
+    the survey's own launches, and a launch sums its shots' gradient
+    contributions before adding them to the accumulators)."""
     plans, alone = [], []
     try:
         for nz, nx in C4_GRIDS:
@@ -218,7 +220,8 @@ def test_config4_surveys_in_one_job(workspace):
                 say("job", P.misfit(), "alone", J0)
                 assert P.misfit() == J0
                 assert np.array_equal(P.data(), d0)
-                assert all(np.array_equal(a, b) for a, b in zip(P.gradients(), g0))
+                for a, b in zip(P.gradients(), g0):
+                    assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
     finally:
         for P in plans:
             P.close()

@@ -81,12 +81,17 @@ def test_fwi_geometry_mask_and_padding(tmp_path):
     assert m[8:8 + 10].max() == 0.0 and m[:, :8].max() == 0.0 and m[8 + 40:].max() == 0.0
     np.testing.assert_array_equal(fwi.mask_neg, 1.0 - m)
     assert (tmp_path / "para_file.json").exists() and (tmp_path / "survey_file.json").exists()
-    # try_pad: padded input passes through, (nz, nx) is padded, other sizes refused
+    # try_pad: padded input passes through, (nz, nx) is padded; another size
+    # is padded unresized (FWI.jl:193-204 pads the input, not its resize),
+    # and the operator calls refuse the result
     a = np.ones((40, 32))
     assert F.try_pad(fwi, a).shape == m.shape
     p = np.zeros(m.shape)
-    assert F.try_pad(fwi, p) is not None and F.try_pad(fwi, p).shape == m.shape
-    with pytest.raises(ValueError):
-        F.try_pad(fwi, np.ones((20, 16)))
+    assert F.try_pad(fwi, p) is p
+    q = F.try_pad(fwi, np.arange(20 * 16, dtype=float).reshape(20, 16))
+    assert q.shape == (20 + 16 + 8, 16 + 16)
+    np.testing.assert_array_equal(q[8:28, 8:24], np.arange(20 * 16).reshape(20, 16))
+    with pytest.raises(ValueError, match="after padding"):
+        F._on_grid(fwi, q)
     # a 1-row source time function is repeated for every shot (FWI.jl:118-120)
     assert F._stf_rows(fwi, np.ones(100)).shape == (2, 100)

@@ -47,7 +47,8 @@ KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_str
 # workload's shot groups: profiles/stencil_kernels_608x1780_counters.md.
 # Reported as roofline.traffic when the dominant kernel and its mean shots
 # per launch match the profiled run.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {("bwd_pair_a", 2.91): 231.5e6, ("bwd_pair_b", 2.91): 226.9e6,
+                    ("stress_fwd", 4.92): 204.4e6, ("vel_fwd", 4.92): 180.9e6}
 # kernels of the backward sweep (their launches cover the backward shot groups)
 BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "bwd_pair_a", "bwd_pair_b")
 # C-PML memory-variable traffic per PML cell-direction (2 variables RW = 16 B)
@@ -67,6 +68,9 @@ def parse():
                    help="checkpoint interval of the backward sweep (0: automatic)")
     p.add_argument("--memory-budget", type=float, default=None,
                    help="device bytes for the boundary ring and checkpoints (default 32e9)")
+    p.add_argument("--autotune", action="store_true",
+                   help="time launch shapes on a 200-step record first (fwiflow.tune) and "
+                        "run with the fastest")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 1 GiB device-to-device copy, 4 times (the "
                         "reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -138,9 +142,24 @@ def host_cores():
 
 
 def cpu_model():
+    """`lscpu`'s model name (with its CPU family / model / stepping numbers,
+    which identify the part where a virtual machine shortens the name);
+    /proc/cpuinfo when lscpu is absent."""
+    import subprocess
+
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        f = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        f = {k.strip(): v.strip() for k, v in f.items()}
+        if f.get("Model name"):
+            ids = ", ".join(f"{k.lower()} {f[k]}" for k in ("CPU family", "Model", "Stepping")
+                            if k in f)
+            return f"{f['Model name']} ({ids})" if ids else f["Model name"]
+    except (OSError, subprocess.SubprocessError, ValueError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
                 if line.startswith("model name"):
                     return line.split(":", 1)[1].strip()
     except OSError:
@@ -202,6 +221,15 @@ def main():
     P.set_obs(P.data())
     lam0, mu0, den0 = models.moduli(c, "start")
     P.set_model(lam0, mu0, den0)
+    tuned = None
+    if a.autotune:
+        from fwiflow import tune
+
+        t0 = time.perf_counter()
+        tuned = tune.autotune(para, ids, lam0, mu0, den0, stf, steps=200,
+                              path=os.path.join(ws, "launch.json"), log=log)
+        P.set_launch(tuned["tile_x"], tuned["shots_fwd"], tuned["shots_bwd"])
+        log(f"bench: autotuned launch {P.launch()} in {time.perf_counter() - t0:.1f} s")
     # timing is on from the warmup so that the time loops the engine captures
     # as graphs there (with their event records) are the ones the timed steps
     # replay; set_timing(True) again clears the warmup's totals
@@ -219,7 +247,8 @@ def main():
     ms_per_step = elapsed / a.steps * 1e3
     value = a.nshots * a.steps / elapsed
 
-    # per-kernel HIP-event timings (every 8th step is bracketed)
+    # per-kernel HIP-event timings: every 64th launch of each kind is launched
+    # with hipExtLaunchKernel and events that the kernel itself stamps
     kt = {}
     for name in list(KERNEL_BYTES) + list(KERNEL_PARTS):
         ms, n = P.timing(name)
@@ -278,6 +307,8 @@ def main():
             "unit": "GB/s",
         },
         "kernel_avg_ms": {k: round(v[0], 4) for k, v in kt.items()},
+        "launch": dict(zip(("tile_x", "shots_fwd", "shots_bwd"), P.launch()),
+                       autotuned=tuned is not None),
         "launch_groups": {"forward": groups_f, "backward": groups_b},
         "device_bytes": P.device_bytes(),
         "memory_plan": dict(zip(("checkpoint_every", "shots_per_sweep", "ring_bytes",

@@ -240,6 +240,12 @@ int fwiflow_plan_launch_groups(fwiflow_plan* plan, int* forward, int* backward) 
 int fwiflow_plan_set_memory_budget(fwiflow_plan* plan, double bytes) {
   PLAN_CALL(plan->impl.set_memory_budget(bytes));
 }
+int fwiflow_plan_set_launch(fwiflow_plan* plan, int tile_x, int shots_fwd, int shots_bwd) {
+  PLAN_CALL(plan->impl.set_launch(tile_x, shots_fwd, shots_bwd));
+}
+int fwiflow_plan_launch(fwiflow_plan* plan, int* tile_x, int* shots_fwd, int* shots_bwd) {
+  PLAN_CALL(plan->impl.launch(tile_x, shots_fwd, shots_bwd));
+}
 int fwiflow_plan_memory_plan(fwiflow_plan* plan, int* ckpt_every, int* shots_per_sweep,
                              long long* ring_bytes, long long* checkpoint_bytes) {
   PLAN_CALL(plan->impl.memory_plan(ckpt_every, shots_per_sweep, ring_bytes, checkpoint_bytes));

@@ -1434,7 +1434,7 @@ struct DeviceState {
   int R = 0;
   std::vector<int> roff;  // first receiver of each shot (+ total)
   // engine settings (environment variables read at plan creation)
-  int tile_x = 2;          // x extent of a workgroup tile (blockDim.y)
+  int tile_x = 4;          // x extent of a workgroup tile (blockDim.y)
   bool xcd = true;         // XCD-banded workgroup order
   bool balance = true;     // C-PML columns shared evenly by the XCDs (update-region launches)
   bool mix = true;         // backward pair halves interleaved (pair a)
@@ -1690,7 +1690,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d_.reset(new DeviceState());
   DeviceState& d = *d_;
   d.dev = gpu;
-  d.tile_x = std::max(1, std::min(4, env_int("FWIFLOW_TILE_X", 2)));  // <= 256 threads
+  d.tile_x = std::max(1, std::min(4, env_int("FWIFLOW_TILE_X", 4)));  // <= 256 threads
   d.xcd = env_int("FWIFLOW_XCD_REMAP", 1) != 0;
   d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
@@ -1965,6 +1965,26 @@ Status Plan::set_memory_budget(double bytes) {
   if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
   if (!(bytes > 0)) return Status::Err(FWIFLOW_ERR_ARG, "memory budget must be positive");
   d_->ring_budget = bytes;
+  return Status::Ok();
+}
+
+Status Plan::set_launch(int tile_x, int shots_fwd, int shots_bwd) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (tile_x < 1 || tile_x > 4) return Status::Err(FWIFLOW_ERR_ARG, "tile width must be 1..4");
+  if (shots_fwd < -1 || shots_bwd < -1)
+    return Status::Err(FWIFLOW_ERR_ARG, "shots per launch must be >= -1");
+  d_->tile_x = tile_x;
+  d_->shot_group = shots_fwd;
+  d_->shot_group_bwd = shots_bwd;
+  if (d_->cache) d_->cache->clear_graphs();  // captured loops bake the old shape in
+  return Status::Ok();
+}
+
+Status Plan::launch(int* tile_x, int* shots_fwd, int* shots_bwd) const {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (tile_x) *tile_x = d_->tile_x;
+  if (shots_fwd) *shots_fwd = d_->shot_group;
+  if (shots_bwd) *shots_bwd = d_->shot_group_bwd;
   return Status::Ok();
 }
 

@@ -39,6 +39,10 @@ class Plan {
   // interval that minimises memory); k > 0 checkpoints every k steps
   Status set_checkpoint(int k);
   Status set_memory_budget(double bytes);  // ring + checkpoints of a gradient run
+  // launch shape: tile width in columns (1-4), shots per forward / backward
+  // launch (0: all, -1: automatic); replaces the environment defaults
+  Status set_launch(int tile_x, int shots_fwd, int shots_bwd);
+  Status launch(int* tile_x, int* shots_fwd, int* shots_bwd) const;
   // the last gradient run's checkpoint interval (0: ring of every step), shots
   // per sweep, and device bytes of the ring and of the checkpoints
   Status memory_plan(int* k, int* shots_per_sweep, long long* ring_bytes,

@@ -12,9 +12,11 @@ from .fwi import (FWI, FWIExample, compute_misfit, compute_misfit_and_gradient,
                   compute_observation, make_fwi)
 from .inversion import PlanObjective, lbfgs
 from .precond import CNN, learned_descent, learned_lbfgs
+from .tune import apply_tuned, autotune
 from ._lib import FwiflowError, version
 
 __all__ = [
+    "apply_tuned", "autotune",
     "Job", "Plan", "check_call", "cufd", "fwi_obs_op", "fwi_op", "fwi_op_grad", "host_tables",
     "moduli_to_velocity_grad", "npad", "padding", "padding_adjoint", "paraGen", "sourceGene",
     "surveyGen", "velocity_to_moduli", "FWI", "FWIExample", "compute_misfit",

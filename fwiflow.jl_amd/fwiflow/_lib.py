@@ -33,6 +33,7 @@ EXPORTS = [
     "fwiflow_plan_timing", "fwiflow_plan_device_bytes", "fwiflow_plan_shots_per_launch",
     "fwiflow_plan_set_illumination", "fwiflow_plan_illumination",
     "fwiflow_plan_launch_groups", "fwiflow_plan_set_memory_budget", "fwiflow_plan_memory_plan",
+    "fwiflow_plan_set_launch", "fwiflow_plan_launch",
     "fwiflow_job_create", "fwiflow_job_destroy", "fwiflow_job_add", "fwiflow_job_run",
     "fwiflow_cnn_create", "fwiflow_cnn_destroy", "fwiflow_cnn_num_params",
     "fwiflow_cnn_get_params", "fwiflow_cnn_set_params", "fwiflow_cnn_forward",
@@ -95,6 +96,8 @@ def load():
     L.fwiflow_plan_set_memory_budget.argtypes = [vp, C.c_double]
     L.fwiflow_plan_memory_plan.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                            C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.fwiflow_plan_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.fwiflow_plan_launch.argtypes = [vp, ip, ip, ip]
     L.fwiflow_job_create.argtypes = [C.POINTER(vp)]
     L.fwiflow_job_destroy.argtypes = [vp]
     L.fwiflow_job_add.argtypes = [vp, vp]

@@ -226,6 +226,19 @@ class Plan:
         """Device bytes a gradient run may hold for the boundary ring and checkpoints."""
         check(self._L.fwiflow_plan_set_memory_budget(self._h, float(nbytes)))
 
+    def set_launch(self, tile_x, shots_fwd=-1, shots_bwd=-1):
+        """Launch shape: tile width 1-4, shots per forward / backward launch
+        (0: all, -1: automatic)."""
+        check(self._L.fwiflow_plan_set_launch(self._h, int(tile_x), int(shots_fwd),
+                                              int(shots_bwd)))
+
+    def launch(self):
+        """(tile width, shots per forward launch, shots per backward launch)."""
+        t, f, b = _lib.C.c_int(0), _lib.C.c_int(0), _lib.C.c_int(0)
+        check(self._L.fwiflow_plan_launch(self._h, _lib.C.byref(t), _lib.C.byref(f),
+                                          _lib.C.byref(b)))
+        return int(t.value), int(f.value), int(b.value)
+
     def memory_plan(self):
         """(checkpoint interval, shots per sweep, ring bytes, checkpoint bytes) of
         the last gradient run; interval 0 is the ring of every step."""

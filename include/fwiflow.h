@@ -180,6 +180,14 @@ int fwiflow_plan_set_checkpoint(fwiflow_plan* plan, int ckpt_every);
 /* Device bytes a gradient run may hold for the boundary ring and the
  * checkpoints (default 32e9).  It sets the shots per sweep. */
 int fwiflow_plan_set_memory_budget(fwiflow_plan* plan, double bytes);
+/* Launch shape of the plan's runs: tile width in columns (1-4; a workgroup
+ * is 64 cells x tile_x), shots per forward and per backward launch (0: all
+ * of the plan's shots, -1: automatic, the Infinity-Cache rule).  Replaces
+ * the FWIFLOW_TILE_X / FWIFLOW_SHOT_GROUP(_BWD) defaults; what the autotuner
+ * (fwiflow/tune.py) chooses and caches per grid and shot count.  No
+ * counterpart in the reference (its launch shape is fixed). */
+int fwiflow_plan_set_launch(fwiflow_plan* plan, int tile_x, int shots_fwd, int shots_bwd);
+int fwiflow_plan_launch(fwiflow_plan* plan, int* tile_x, int* shots_fwd, int* shots_bwd);
 /* The memory plan of the last gradient run: checkpoint interval (0: ring of
  * every step), shots per sweep, device bytes of the ring and of the
  * checkpoints.  Any pointer may be NULL. */

@@ -677,3 +677,32 @@ def test_irregular_survey_parity(oracle_mod, workspace):
     say({k: f"{v:.2e}" for k, v in out.items()})
     for k, v in out.items():
         assert v <= (SEIS_TOL if k.startswith("misfit") else GRAD_TOL), (k, v)
+
+
+def test_autotuner_measures_caches_and_applies(workspace):
+    """fwiflow.tune.autotune times launch shapes around the automatic one on
+    a shortened record of the caller's problem, caches the fastest per
+    (grid, shot count), and a new plan given that shape reproduces the
+    automatic plan's data bit for bit and gradients to the summation order
+    of a launch's shots."""
+    from fwiflow import tune
+
+    c, para, stf, ids, m0 = _grad_setup(workspace, nshots=4)
+    path = os.path.join(workspace, "launch.json")
+    e = tune.autotune(para, ids, *m0, stf, steps=60, repeats=1, path=path, log=say)
+    say("tuned", {k: v for k, v in e.items() if k != "candidates"})
+    assert e["tile_x"] in (4, 2) and 1 <= e["shots_fwd"] <= 4 and 1 <= e["shots_bwd"] <= 4
+    assert len(e["candidates"]) >= 2 and e["seconds"] == min(e["candidates"].values())
+    assert tune.lookup(para, len(ids), path) == e
+    d0, g0, J0, _ = _run_plan(para, ids, *m0, stf)
+    with fwiflow.Plan(para, ids) as P:
+        assert tune.apply_tuned(P, para, len(ids), path) == e
+        assert P.launch() == (e["tile_x"], e["shots_fwd"], e["shots_bwd"])
+        P.set_model(*m0)
+        P.set_stf(stf)
+        P.load_obs()
+        P.run(2)
+        assert np.array_equal(P.data(), d0)
+        P.run(1)
+        for a, b in zip(P.gradients(), g0):
+            assert rel(a, b) < 1e-5

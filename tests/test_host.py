@@ -216,3 +216,29 @@ def test_bench_byte_model_matches_the_stated_per_unit_figure():
     assert bench.algorithmic_bytes_per_shot_gradient(c, 4000) == pytest.approx(766.272e9, rel=1e-6)
     one = bench.kernel_launch_bytes("bwd_pair_a", c, 1.0)
     assert bench.kernel_launch_bytes("bwd_pair_a", c, 64 / 22) == pytest.approx(one * 64 / 22)
+
+
+def test_launch_tuning_cache_round_trip(workspace):
+    """The autotuner's cache: key per (arch, grid, C-PML, shot count), entries
+    kept across stores, and apply_tuned sets the cached shape on a plan."""
+    import json
+
+    from fwiflow import tune
+
+    c = models.case_small(nz=40, nx=60, nSteps=100, nshots=3)
+    para, _ = models.write_case(c, workspace)
+    path = os.path.join(workspace, "tune", "launch.json")
+    key = tune.cache_key(para, 3)
+    assert key == f"gfx950/{c.NZ}x{c.NX}/pml{c.nPml}/shots3"
+    assert tune.lookup(para, 3, path) is None
+    tune.store(key, {"tile_x": 2, "shots_fwd": 3, "shots_bwd": 1}, path)
+    tune.store("other", {"tile_x": 4, "shots_fwd": 1, "shots_bwd": 1}, path)
+    assert json.load(open(path))[key]["shots_bwd"] == 1
+
+    class FakePlan:
+        def set_launch(self, *a):
+            self.shape = a
+
+    fp = FakePlan()
+    assert tune.apply_tuned(fp, para, 3, path)["tile_x"] == 2 and fp.shape == (2, 3, 1)
+    assert tune.apply_tuned(FakePlan(), para, 4, path) is None

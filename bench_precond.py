@@ -61,6 +61,12 @@ def parse():
     p.add_argument("--driver", choices=["lbfgs", "descent"], default="lbfgs")
     p.add_argument("--teacher", choices=["plain", "control"], default="control",
                    help="optimiser whose accepted updates are the training pairs")
+    p.add_argument("--teacher-maxiter", type=int, default=None,
+                   help="iterations of each training inversion (default: --maxiter)")
+    p.add_argument("--save-net", default=None, help="write the trained parameters (.npy)")
+    p.add_argument("--load-net", default=None,
+                   help="skip training and evaluate the parameters in this .npy file")
+    p.add_argument("--train-only", action="store_true", help="stop after training")
     return p.parse_args()
 
 
@@ -87,7 +93,8 @@ def main():
     mask = None
     samples = []
     plain_train = []
-    for k in range(a.train_models):
+    train_models = 0 if a.load_net else a.train_models
+    for k in range(train_models):
         seed = a.seed0 + k
         ws = tempfile.mkdtemp(prefix=f"fwiflow_pc_{seed}_")
         fwi, obj, x0 = setup(seed, a.nt, ws)
@@ -97,23 +104,43 @@ def main():
             x = precond.features(extras["gradients"], extras["illumination"], mask)
             samples.append((x, precond.target_from_update(update, mask)))
 
+        tmax = a.teacher_maxiter or a.maxiter
         if a.teacher == "plain":
-            r = inversion.lbfgs(obj, x0, maxiter=a.maxiter, on_pair=on_pair)
+            r = inversion.lbfgs(obj, x0, maxiter=tmax, on_pair=on_pair)
         else:
-            r = precond.learned_lbfgs(obj, x0, mask, net=None, maxiter=a.maxiter,
-                                      on_pair=on_pair)
+            r = precond.learned_lbfgs(obj, x0, mask, net=None, maxiter=tmax, on_pair=on_pair)
         obj.close()
         plain_train.append((seed, r["iterations"], r["evaluations"], r["seconds"]))
         log(f"train model {seed}: {r['iterations']} iterations, {r['evaluations']} evaluations, "
             f"{r['seconds']:.1f} s, misfit {r['history'][0][1] if r['history'] else 0:.4g} -> "
             f"{r['misfit']:.4g}; pairs {len(samples)}")
+    if a.load_net:
+        fwi0, obj0, _ = setup(a.seed0, a.nt, tempfile.mkdtemp(prefix="fwiflow_pc_mask_"))
+        mask = fwi0.mask
+        obj0.close()
     H, W = mask.shape
     net = precond.CNN(H, W, seed=0)
     t0 = time.perf_counter()
-    hist = precond.train(net, samples, mask.astype(np.float32), epochs=a.epochs, lr=a.lr,
-                         log=log)
+    if a.load_net:
+        net.set_params(np.load(a.load_net).astype(np.float32))
+        hist = [float("nan")]
+        log(f"loaded network parameters from {a.load_net}")
+    else:
+        hist = precond.train(net, samples, mask.astype(np.float32), epochs=a.epochs, lr=a.lr,
+                             log=log)
+        log(f"trained on {len(samples)} pairs in {time.perf_counter() - t0:.1f} s, "
+            f"loss {hist[0]:.4f} -> {hist[-1]:.4f}")
     t_train = time.perf_counter() - t0
-    log(f"trained on {len(samples)} pairs in {t_train:.1f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    if a.save_net:
+        np.save(a.save_net, net.get_params())
+    if a.train_only:
+        print(json.dumps({"training": {"pairs": len(samples), "seconds": round(t_train, 1),
+                                       "loss": hist, "models": train_models},
+                          "plain_training_runs": [{"seed": s_, "iterations": i, "evaluations": e,
+                                                   "seconds": round(t, 1)}
+                                                  for s_, i, e, t in plain_train]}), flush=True)
+        net.close()
+        return
     evals = []
     for k in range(a.eval_models):
         seed = a.seed0 + 1000 + k

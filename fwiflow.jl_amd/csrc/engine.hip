@@ -1439,6 +1439,8 @@ struct DeviceState {
   bool balance = true;     // C-PML columns shared evenly by the XCDs (update-region launches)
   bool mix = true;         // backward pair halves interleaved (pair a)
   bool mix_b = true;       // ... (pair b)
+  bool pair_a = true;      // reverse stress + adjoint velocity in one launch
+  bool pair_b = true;      // adjoint stress + reverse velocity in one launch
   int unroll = 1;          // shots whose loads a backward thread issues together
   int unroll_fwd = 2;      // ... a forward thread
   int shot_group = -1;     // shots per forward launch (0: all, -1: automatic)
@@ -1695,6 +1697,8 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.balance = env_int("FWIFLOW_XCD_BALANCE", 1) != 0 && d.xcd;
   d.mix = env_int("FWIFLOW_PAIR_MIX", 1) != 0;
   d.mix_b = env_int("FWIFLOW_PAIR_MIX_B", 1) != 0;
+  d.pair_a = env_int("FWIFLOW_PAIR_A", 1) != 0;
+  d.pair_b = env_int("FWIFLOW_PAIR_B", 1) != 0;
   d.unroll = env_int("FWIFLOW_SHOT_UNROLL", 1) >= 2 ? 2 : 1;
   d.unroll_fwd = env_int("FWIFLOW_SHOT_UNROLL_FWD", 2) >= 2 ? 2 : 1;
   d.shot_group = std::max(-1, env_int("FWIFLOW_SHOT_GROUP", -1));
@@ -2369,6 +2373,8 @@ Status Runner::maps_of(const GroupParts& gp, const char* kind, DevMap& out) {
         L = interleave(a, b);
       else
         L = a, append(L, b);
+    } else if (k == "R") {
+      L = rng(kRevStress);
     } else if (k == "AV") {
       L = upd(kAdjVel);
     } else if (k == "AS") {
@@ -2646,14 +2652,15 @@ Status Runner::run(int calc_id) {
         TRY(d.zero_shots(d.adj, kWaveComps, s0, n, st));
         TRY(d.zero_shots(ckpt ? d.madj : d.mem, kMemComps, s0, n, st));
       }
-      DevMap mA, mB, mAV, mAS, mRV, mRS, mRW;
+      DevMap mA, mB, mAV, mAS, mRV, mRS, mRW, mR;
       TRY(maps_of(gp, "A", mA));
+      TRY(maps_of(gp, "R", mR));
       TRY(maps_of(gp, "B", mB));
       TRY(maps_of(gp, "AV", mAV));
       TRY(maps_of(gp, "AS", mAS));
       TRY(maps_of(gp, "RV", mRV));
       const StepLaunch sA = step(gp, mA), sB = step(gp, mB), sAV = step(gp, mAV),
-                       sAS = step(gp, mAS), sRV = step(gp, mRV);
+                       sAS = step(gp, mAS), sRV = step(gp, mRV), sR = step(gp, mR);
       StepLaunch sRS, sRW;
       if (ckpt) {
         TRY(maps_of(RG[j][q], "S", mRS));
@@ -2683,18 +2690,27 @@ Status Runner::run(int calc_id) {
           bwd_b(sRV, t1, Ub);         // rev_vel(t1-1)
         for (int it = t1 - 1; it >= t0; it--) {
           const bool smp = it % sample_every == 0;
-          d0->launch("bwd_pair_a", smp, [&] { bwd_a(sA, it, Ub); });
+          if (d0->pair_a) {
+            d0->launch("bwd_pair_a", smp, [&] { bwd_a(sA, it, Ub); });
+          } else {  // the halves as launches of their own (same kernel, one kind each)
+            d0->launch("rev_stress", smp, [&] { bwd_a(sR, it, Ub); });
+            d0->launch("adj_vel", smp, [&] { bwd_a(sAV, it, Ub); });
+          }
           inject(it);
-          if (it > t0)
+          if (it > t0 && d0->pair_b) {
             d0->launch("bwd_pair_b", smp, [&] { bwd_b(sB, it, Ub); });
-          else
+          } else if (it > t0) {
+            d0->launch("adj_stress", smp, [&] { bwd_b(sAS, it, Ub); });
+            d0->launch("rev_vel", smp, [&] { bwd_b(sRV, it, Ub); });
+          } else {
             bwd_b(sAS, it, Ub);
+          }
         }
       };
       if (!ckpt) {
         // the whole backward sweep of the group, from the initial adjoint
         // step at t = nSteps-1 (libCUFD.cu:353-373)
-        TRY(replay(graph_key("B", gp, {&mA, &mB, &mAV, &mAS, &mRV}), [&] {
+        TRY(replay(graph_key("B", gp, {&mA, &mB, &mAV, &mAS, &mRV, &mR}), [&] {
           bwd_a(sAV, nSteps - 1, Ub);
           inject(nSteps - 1);
           if (nSteps < 2)

@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libfwiflow_hip.so")
+# FWIFLOW_LIBRARY names another build of the library (e.g. an experimental one)
+LIB_PATH = os.environ.get("FWIFLOW_LIBRARY") or os.path.join(os.path.dirname(_HERE), "lib",
+                                                               "libfwiflow_hip.so")
 
 STATUS = {
     0: "FWIFLOW_OK", 1: "FWIFLOW_ERR_CALC_ID", 2: "FWIFLOW_ERR_IO", 3: "FWIFLOW_ERR_JSON",

@@ -441,6 +441,7 @@ VARIANTS = {
     "tile_x2_unbalanced": {"FWIFLOW_TILE_X": "2", "FWIFLOW_XCD_BALANCE": "0"},
     "no_xcd_remap": {"FWIFLOW_XCD_REMAP": "0"},
     "pairs_in_order": {"FWIFLOW_PAIR_MIX": "0", "FWIFLOW_PAIR_MIX_B": "0"},
+    "unpaired": {"FWIFLOW_PAIR_A": "0", "FWIFLOW_PAIR_B": "0"},
     "separate_injection": {"FWIFLOW_FOLD_INJECT": "0"},
     "part_table": {"FWIFLOW_PART_TABLE": "1"},
     "graphs": {"FWIFLOW_GRAPHS": "1"},
@@ -454,7 +455,8 @@ def test_launch_variants_match_default(workspace, variant):
     forward kernels, two shots per thread in the backward kernels,
     backward groups that differ from the forward ones, tiles 1 or 2
     columns wide, the workgroup map without XCD balancing or without XCD
-    remapping, the backward stages in order instead of interleaved, the
+    remapping, the backward stages in order instead of interleaved or as
+    four launches per step instead of two pairs, the
     residual injected by a launch of its own, the shot table read from
     device memory instead of kernel arguments, and time loops replayed
     from captured graphs.  Data and gradients within 1e-6 / 1e-5 relative
@@ -469,7 +471,8 @@ def test_launch_variants_match_default(workspace, variant):
         "grads", [rel(a, b) for a, b in zip(g1, g0)])
     assert np.array_equal(d1, d0)
     if variant in ("separate_injection", "part_table", "graphs", "no_xcd_remap",
-                   "pairs_in_order", "tile_x1", "tile_x2_unbalanced", "unroll1_fwd"):
+                   "pairs_in_order", "tile_x1", "tile_x2_unbalanced", "unroll1_fwd",
+                   "unpaired"):
         assert J1 == J0 and all(np.array_equal(a, b) for a, b in zip(g1, g0))
     assert abs(J1 - J0) <= 1e-12 * abs(J0)
     for a, b in zip(g1, g0):

@@ -117,6 +117,15 @@ def test_shot_id_out_of_range(workspace):
     assert _code(e) == "FWIFLOW_ERR_SHAPE"
 
 
+def test_empty_shot_list(workspace):
+    """An empty group is refused (the reference's loops would run no shot and
+    return uninitialised outputs)."""
+    c, para, stf, lam, mu, den = _small(workspace)
+    with pytest.raises(fwiflow.FwiflowError) as e:
+        fwiflow.check_call(2, [], para, lam, mu, den)
+    assert _code(e) == "FWIFLOW_ERR_ARG"
+
+
 def test_source_outside_grid(workspace):
     c, para, stf, lam, mu, den = _small(workspace)
     surveyGen([2], [c.nx + 40], c.rec_z, c.rec_x, json.loads(open(para).read())["survey_fname"])

@@ -68,6 +68,8 @@ def parse():
                    help="checkpoint interval of the backward sweep (0: automatic)")
     p.add_argument("--memory-budget", type=float, default=None,
                    help="device bytes for the boundary ring and checkpoints (default 32e9)")
+    p.add_argument("--abi", action="store_true",
+                   help="also time one end-to-end fwiflow_cufd gradient call (files, set-up)")
     p.add_argument("--autotune", action="store_true",
                    help="time launch shapes on a 200-step record first (fwiflow.tune) and "
                         "run with the fastest")
@@ -167,6 +169,20 @@ def cpu_model():
     return "unknown"
 
 
+def abi_seconds(c, ws, ids, models, fwiflow):
+    """One end-to-end `fwiflow_cufd` gradient call (calc_id 1) as the
+    reference's callers make it: JSON parsing, plan set-up and device
+    allocation, host model transposes, the observed data read from the 64
+    `Shot{id}.bin` files, the gradient, the host copies back, teardown."""
+    para, stf = models.write_case(c, os.path.join(ws, "abi"))
+    lam, mu, den = models.moduli(c, "true")
+    fwiflow.fwi_obs_op(lam, mu, den, stf, 0, ids, para)
+    lam0, mu0, den0 = models.moduli(c, "start")
+    t0 = time.perf_counter()
+    fwiflow.fwi_op_grad(lam0, mu0, den0, stf, 0, ids, para)
+    return time.perf_counter() - t0
+
+
 def cpu_baseline(c, ws, sample_steps):
     from fwiflow import models
     from oracle import oracle
@@ -182,8 +198,16 @@ def cpu_baseline(c, ws, sample_steps):
     oracle.cufd(lam0, mu0, den0, stf, 1, [0], para, precision=32, nthreads=cores)
     t = time.perf_counter() - t0
     per_grad = t * (c.nSteps / sample_steps)
+    # the 100x200 case's full gradient (1 shot, nt 1000), not extrapolated
+    c1 = models.case1()
+    p1, s1 = models.write_case(c1, os.path.join(ws, "cpu_c1"))
+    oracle.cufd(*models.moduli(c1, "true"), s1, 2, [0], p1, precision=32, nthreads=cores)
+    t1 = time.perf_counter()
+    oracle.cufd(*models.moduli(c1, "start"), s1, 1, [0], p1, precision=32, nthreads=cores)
+    t1 = time.perf_counter() - t1
     return dict(value=1.0 / per_grad, unit="shot-gradients/s", cores=cores, kind="port",
                 cpu=cpu_model(), nproc=os.cpu_count(),
+                c1_full_gradient_s=round(t1, 3),
                 sample=f"1 shot of the same grid, first {sample_steps} of {c.nSteps} steps "
                        f"(calc_id 1, FP32 reference arithmetic, OpenMP on {cores} threads), "
                        f"{t:.1f} s, scaled linearly to {c.nSteps} steps")
@@ -240,8 +264,11 @@ def main():
         log(f"bench: warmup {i} {time.perf_counter() - t0:.2f} s")
     P.set_timing(True)
     t0 = time.perf_counter()
+    step_s = []
     for i in range(a.steps):
+        t1 = time.perf_counter()
         P.run(1)  # synchronises the plan's stream before returning
+        step_s.append(time.perf_counter() - t1)
         log(f"bench: step {i} done at {time.perf_counter() - t0:.2f} s")
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / a.steps * 1e3
@@ -273,6 +300,7 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": round(ms_per_step, 2),
+        "ms_per_step_median": round(float(np.median(step_s)) * 1e3, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -315,6 +343,13 @@ def main():
                                  "checkpoint_bytes"), P.memory_plan())),
     }
     P.close()
+    if a.abi:
+        log("bench: one end-to-end fwiflow_cufd gradient call ...")
+        t = abi_seconds(c, ws, ids, models, fwiflow)
+        out["abi_end_to_end"] = {
+            "seconds": round(t, 2), "shot_gradients_per_s": round(a.nshots / t, 4),
+            "what": "one fwiflow_cufd calc_id 1 call: JSON, set-up and allocation, host "
+                    "transposes, 64 Shot files read, gradient, copies back"}
     if not a.no_cpu_baseline:
         log("bench: CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(c, ws, a.cpu_sample_steps)

@@ -68,6 +68,10 @@ def parse():
                    help="checkpoint interval of the backward sweep (0: automatic)")
     p.add_argument("--memory-budget", type=float, default=None,
                    help="device bytes for the boundary ring and checkpoints (default 32e9)")
+    p.add_argument("--workload", choices=["c3", "timelapse"], default="c3",
+                   help="c3: the 500x1700 benchmark (default); timelapse: the six config-4 "
+                        "surveys (200x600, 300x900, 450x1350, baseline and monitor, 16 shots "
+                        "each, nt 3000) as one job")
     p.add_argument("--abi", action="store_true",
                    help="also time one end-to-end fwiflow_cufd gradient call (files, set-up)")
     p.add_argument("--autotune", action="store_true",
@@ -213,6 +217,70 @@ def cpu_baseline(c, ws, sample_steps):
                        f"{t:.1f} s, scaled linearly to {c.nSteps} steps")
 
 
+def timelapse(a):
+    """One JSON line for the time-lapse workload: six surveys (three grids,
+    baseline and monitor, 16 shots each, nt 3000) whose gradients run as one
+    job (every time step one launch across the grids), against the same
+    plans run one after another."""
+    import fwiflow
+    from fwiflow import models
+
+    nt = 3000
+    ws = tempfile.mkdtemp(prefix="fwiflow_bench_tl_")
+    plans, cases = [], []
+    for nz, nx in ((200, 600), (300, 900), (450, 1350)):
+        for monitor in (False, True):
+            c = models.case4(nz, nx, monitor=monitor, nSteps=nt, nshots=16)
+            para, stf = models.write_case(c, os.path.join(ws, c.name))
+            P = fwiflow.Plan(para, list(range(16)))
+            P.set_model(*models.moduli(c, "true"))
+            P.set_stf(stf)
+            P.run(2)
+            P.set_obs(P.data())
+            P.set_model(*models.moduli(c, "start"))
+            plans.append(P)
+            cases.append(c)
+    log(f"bench: time-lapse surveys {[c.name for c in cases]}, nt {nt}")
+    J = fwiflow.Job(plans)
+    for _ in range(a.warmup):
+        J.run(1)
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        J.run(1)
+        log(f"bench: job step {i} done at {time.perf_counter() - t0:.2f} s")
+    elapsed = time.perf_counter() - t0
+    fused = J.fused()
+    J.close()
+    t1 = time.perf_counter()
+    for P in plans:
+        P.run(1)
+    separate = time.perf_counter() - t1
+    nshots = 16 * len(plans)
+    value = nshots * a.steps / elapsed
+    grad_bytes = sum(16 * algorithmic_bytes_per_shot_gradient(c, nt) for c in cases)
+    job_gbs = grad_bytes * a.steps / elapsed / 1e9
+    out = {
+        "metric": "shot-gradients/s, six time-lapse surveys in one job",
+        "value": round(value, 4), "unit": "shot-gradients/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 2), "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "config-4 time-lapse: 200x600, 300x900, 450x1350 grids "
+                               "(+32 C-PML), baseline and monitor, 16 shots each, nt 3000, "
+                               "calc_id 1, observed data resident in HBM",
+                   "grids": [[c.NZ, c.NX] for c in cases[::2]], "nt": nt, "shots": nshots},
+        "job_roofline": {"algorithmic_bytes_per_job_gradient": grad_bytes,
+                         "achieved": round(job_gbs, 1), "frac": round(job_gbs / HBM_PEAK_GBS, 4),
+                         "unit": "GB/s"},
+        "fused": fused,
+        "plans_one_after_another_ms": round(separate * 1e3, 2),
+        "job_speedup": round(separate / (elapsed / a.steps), 3),
+    }
+    for P in plans:
+        P.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -223,6 +291,9 @@ def main():
               "independent: each GPU would run a plan for its own shot range and the "
               "gradients would be summed once per evaluation)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "timelapse":
+        timelapse(a)
+        return
     import fwiflow
     from fwiflow import models
 

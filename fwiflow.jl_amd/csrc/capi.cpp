@@ -269,6 +269,12 @@ int fwiflow_job_run(fwiflow_job* job, int calc_id) {
   return report(job->impl.run(calc_id));
 }
 
+int fwiflow_job_fused(fwiflow_job* job, int* fused) {
+  if (!job || !fused) return report(Status::Err(FWIFLOW_ERR_ARG, "null argument"));
+  *fused = job->impl.fused() ? 1 : 0;
+  return report(Status::Ok());
+}
+
 #define CNN_CALL(expr)                                                      \
   do {                                                                      \
     if (!net) return report(Status::Err(FWIFLOW_ERR_ARG, "null network")); \

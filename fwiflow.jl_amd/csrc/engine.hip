@@ -2822,6 +2822,38 @@ Status Job::add(Plan* p) {
 Status Job::run(int calc_id) {
   if (plans_.empty()) return Status::Err(FWIFLOW_ERR_ARG, "empty job");
   HIP_TRY(hipSetDevice(plans_[0]->d_->dev));
+  // Fused launches (every launch covers a group of shots of every plan) or
+  // the plans one after another.  The fused groups are sized by the
+  // Infinity-Cache rule on the plans' summed bytes per shot, so a fused
+  // backward launch holds fewer shots of each plan than that plan's own
+  // launches would, and every launch re-reads every plan's model and
+  // gradient accumulators for those few shots.  Fusing pays only where the
+  // plans are small enough that their own launches would be no larger;
+  // FWIFLOW_JOB_FUSE = 1 / 0 forces either.
+  int fuse = env_int("FWIFLOW_JOB_FUSE", -1);
+  if (fuse < 0 && plans_.size() > 1) {
+    auto group = [](double bytes_per_row, int rows) {
+      return std::max(1, std::min(rows, (int)std::lround(224e6 / bytes_per_row)));
+    };
+    double sum = 0;
+    int rows = 0, solo = 1 << 30;
+    for (Plan* p : plans_) {
+      const DeviceState& d = *p->d_;
+      sum += d.bwd_bytes_per_shot();
+      rows = std::max(rows, d.g.ns);
+      solo = std::min(solo, d.shot_group_bwd > 0 ? std::min(d.shot_group_bwd, d.g.ns)
+                                                 : group(d.bwd_bytes_per_shot(), d.g.ns));
+    }
+    const int fused = plans_[0]->d_->shot_group_bwd > 0
+                          ? std::min(plans_[0]->d_->shot_group_bwd, rows)
+                          : group(sum, rows);
+    fuse = fused >= solo ? 1 : 0;
+  }
+  last_fused_ = fuse != 0 || plans_.size() == 1;
+  if (!last_fused_) {
+    for (Plan* p : plans_) TRY(p->run(calc_id));
+    return Status::Ok();
+  }
   if (!cache_) {
     cache_.reset(new JobCache());
     cache_->dev = plans_[0]->d_->dev;
@@ -2833,6 +2865,7 @@ Status Job::run(int calc_id) {
 }
 
 int Job::size() const { return (int)plans_.size(); }
+bool Job::fused() const { return last_fused_; }
 
 Status Plan::misfit(double* out) {
   if (!d_ || (last_calc != 0 && last_calc != 1))

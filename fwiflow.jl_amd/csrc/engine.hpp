@@ -76,8 +76,10 @@ class Job {
   Status add(Plan* p);
   Status run(int calc_id);
   int size() const;
+  bool fused() const;  // the last run's launches covered every plan
 
  private:
+  bool last_fused_ = true;
   std::vector<Plan*> plans_;
   std::unique_ptr<JobCache> cache_;
 };

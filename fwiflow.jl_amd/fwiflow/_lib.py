@@ -37,6 +37,7 @@ EXPORTS = [
     "fwiflow_plan_launch_groups", "fwiflow_plan_set_memory_budget", "fwiflow_plan_memory_plan",
     "fwiflow_plan_set_launch", "fwiflow_plan_launch",
     "fwiflow_job_create", "fwiflow_job_destroy", "fwiflow_job_add", "fwiflow_job_run",
+    "fwiflow_job_fused",
     "fwiflow_cnn_create", "fwiflow_cnn_destroy", "fwiflow_cnn_num_params",
     "fwiflow_cnn_get_params", "fwiflow_cnn_set_params", "fwiflow_cnn_forward",
     "fwiflow_cnn_gradients", "fwiflow_cnn_train_step",
@@ -104,6 +105,7 @@ def load():
     L.fwiflow_job_destroy.argtypes = [vp]
     L.fwiflow_job_add.argtypes = [vp, vp]
     L.fwiflow_job_run.argtypes = [vp, C.c_int]
+    L.fwiflow_job_fused.argtypes = [vp, ip]
     L.fwiflow_cnn_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint]
     L.fwiflow_cnn_destroy.argtypes = [vp]
     L.fwiflow_cnn_num_params.argtypes = [vp, ip]

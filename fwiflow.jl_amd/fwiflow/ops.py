@@ -250,10 +250,12 @@ class Plan:
 
 
 class Job:
-    """Several plans run together (fwiflow_job_* in include/fwiflow.h): every
-    time-step launch covers a group of shots of every plan, e.g. baseline and
-    monitor surveys on grids of different sizes.  The plans must share the GPU
-    and nSteps and must outlive the job; results are read from each plan."""
+    """Several plans run together (fwiflow_job_* in include/fwiflow.h), e.g.
+    baseline and monitor surveys on grids of different sizes: fused (every
+    time-step launch covers a group of shots of every plan) where that keeps
+    launches at least as large as the plans' own, else one plan after
+    another.  The plans must share the GPU and nSteps and must outlive the
+    job; results are read from each plan."""
 
     def __init__(self, plans=()):
         L = _lib.load()
@@ -270,6 +272,13 @@ class Job:
 
     def run(self, calc_id):
         check(self._L.fwiflow_job_run(self._h, int(calc_id)))
+
+    def fused(self):
+        """Whether the last run fused every plan into each launch (else the
+        plans ran one after another)."""
+        f = _lib.C.c_int(0)
+        check(self._L.fwiflow_job_fused(self._h, _lib.C.byref(f)))
+        return bool(f.value)
 
     def close(self):
         if self._h:

@@ -220,10 +220,16 @@ int fwiflow_plan_illumination(fwiflow_plan* plan, double* illum);
 
 /* ------------------------------------------------------------------------
  * Job: several plans run together on one GPU, e.g. the baseline and
- * monitor surveys of a time-lapse study on grids of different sizes.  Every
- * time-step launch covers a group of shots of every plan (a workgroup map
- * assigns each workgroup a plan and a tile), so the job takes as many
- * launches per step as one plan.  The plans must share gpu_id and nSteps;
+ * monitor surveys of a time-lapse study on grids of different sizes.  Fused,
+ * every time-step launch covers a group of shots of every plan (a workgroup
+ * map assigns each workgroup a plan and a tile), so the job takes as many
+ * launches per step as one plan.  A fused launch holds fewer shots of each
+ * plan than the plan's own launches (its groups are sized for the Infinity
+ * Cache on the plans' summed bytes), so the job fuses only when that is no
+ * fewer than the smallest plan's own group, and otherwise runs the plans one
+ * after another (FWIFLOW_JOB_FUSE = 1 / 0 forces either);
+ * fwiflow_job_fused reports what the last run did.  The plans must share
+ * gpu_id and nSteps;
  * each keeps its own model, data, misfit and gradients, read with the plan
  * calls above after fwiflow_job_run.  No reference counterpart: its callers
  * run one fwi_op per survey (docs/codes/src_fwi_coupled/
@@ -235,6 +241,7 @@ int fwiflow_job_create(fwiflow_job** job);
 int fwiflow_job_destroy(fwiflow_job* job);
 int fwiflow_job_add(fwiflow_job* job, fwiflow_plan* plan);
 int fwiflow_job_run(fwiflow_job* job, int calc_id);
+int fwiflow_job_fused(fwiflow_job* job, int* fused);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner network.  No reference counterpart: FwiFlow's plain

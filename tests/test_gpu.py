@@ -507,6 +507,7 @@ def test_job_of_two_grids_equals_separate_plans(workspace):
             P.load_obs()
         with fwiflow.Job(plans) as J:
             J.run(1)
+            assert J.fused()  # both plans' own launches hold all their shots
             for P, (J0, g0, d0) in zip(plans, sep):
                 say("job misfit", P.misfit(), "plan", J0)
                 assert P.misfit() == J0

@@ -190,13 +190,19 @@ def test_config4_time_lapse_parity(workspace, grid):
 
 
 @pytest.mark.timeout(600)
-def test_config4_surveys_in_one_job(workspace):
+@pytest.mark.parametrize("fuse", ["-1", "1"], ids=["automatic", "fused"])
+def test_config4_surveys_in_one_job(workspace, fuse):
     """The six config-4 surveys (three grids, baseline and monitor; two shots
-    each, 3000 steps) run as one job, every time step one launch across
-    grids: misfits and data equal each survey's plan run alone bit for bit,
-    gradients to 1e-6 (a job launch covers fewer shots of each survey than
-    the survey's own launches, and a launch sums its shots' gradient
-    contributions before adding them to the accumulators)."""
+    each, 3000 steps) run as one job: misfits and data equal each survey's
+    plan run alone bit for bit, gradients to 1e-6 (a fused launch covers
+    fewer shots of each survey than the survey's own launches, and a launch
+    sums its shots' gradient contributions before adding them to the
+    accumulators).  Fused (every time step one launch across the grids, a
+    backward group of one shot per survey by the cache rule on the summed
+    bytes) and automatic (here: the surveys one after another, since each
+    survey alone fills launches of both its shots)."""
+    old = os.environ.get("FWIFLOW_JOB_FUSE")
+    os.environ["FWIFLOW_JOB_FUSE"] = fuse
     plans, alone = [], []
     try:
         for nz, nx in C4_GRIDS:
@@ -216,6 +222,7 @@ def test_config4_surveys_in_one_job(workspace):
                 alone.append((P.misfit(), P.data(), P.gradients()))
         with fwiflow.Job(plans) as J:
             J.run(1)
+            assert J.fused() == (fuse == "1")
             for P, (J0, d0, g0) in zip(plans, alone):
                 say("job", P.misfit(), "alone", J0)
                 assert P.misfit() == J0
@@ -223,6 +230,10 @@ def test_config4_surveys_in_one_job(workspace):
                 for a, b in zip(P.gradients(), g0):
                     assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
     finally:
+        if old is None:
+            os.environ.pop("FWIFLOW_JOB_FUSE", None)
+        else:
+            os.environ["FWIFLOW_JOB_FUSE"] = old
         for P in plans:
             P.close()
 

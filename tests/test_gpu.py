@@ -710,3 +710,44 @@ def test_autotuner_measures_caches_and_applies(workspace):
         P.run(1)
         for a, b in zip(P.gradients(), g0):
             assert rel(a, b) < 1e-5
+
+
+def test_reference_gradtest_taylor(workspace):
+    """The reference's own gradient test (deps/CustomOps/FWI/gradtest.jl:
+    400x400, nSteps 2001, one central source, a 13x13 receiver mesh, the
+    misfit as a function of mu at the mu of Vp 3200, a positive random
+    direction off the C-PML; restated in tests/golden/make_gradtest.py) on
+    the GPU.  The reference only plots |f(m + g v) - f(m)| = O(g) and
+    |f(m + g v) - f(m) - g <v, grad f>| = O(g^2) (gradtest.jl:137-171); here
+    both slopes are asserted, and <v, grad f>, the differences and the
+    remainders are held to the oracle's (FP32 reference arithmetic for the
+    directional derivative, FP64 for the differences, which the GPU sums in
+    double)."""
+    sys.path.insert(0, GOLDEN)
+    import make_gradtest as G
+
+    fx = np.load(os.path.join(GOLDEN, "gradtest.npz"))
+    para, lam, mu, den, mu2, stf, v = G.setup(workspace)
+    fwiflow.fwi_obs_op(lam, mu, den, stf, 0, [0], para)
+    with fwiflow.Plan(para, [0]) as P:
+        P.set_stf(stf)
+        P.load_obs()
+
+        def f(m, calc=0):
+            P.set_model(lam, m, den)
+            P.run(calc)
+            return P.misfit()
+
+        y0 = f(mu2, 1)
+        vg = float(np.sum(v * P.gradients()[1]))
+        s = np.array([f(mu2 + g * v) - y0 for g in fx["gamma"]])
+    w = s - fx["gamma"] * vg
+    say("gradtest y0", y0, "oracle", float(fx["y64"]), "<v,g>", vg, "oracle FP32",
+        float(fx["vg32"]), "s", s, "oracle FP64", fx["s64"], "w", w, "oracle FP64", fx["w64"])
+    assert abs(y0 - fx["y64"]) <= 1e-4 * fx["y64"]
+    assert abs(vg - fx["vg32"]) <= 1e-3 * abs(fx["vg32"])
+    assert np.all(np.abs(s[:2] - fx["s64"][:2]) <= 1e-4 * np.abs(fx["s64"][:2]))
+    slope_s = np.log10(s[0] / s[1])
+    slope_w = np.log10(w[0] / w[1])
+    assert abs(slope_s - 1.0) < 0.05 and abs(slope_w - 2.0) < 0.1, (slope_s, slope_w)
+    assert np.all(np.abs(w[:2] - fx["w64"][:2]) <= 0.02 * np.abs(fx["w64"][:2]))

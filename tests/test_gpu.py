@@ -751,3 +751,57 @@ def test_reference_gradtest_taylor(workspace):
     slope_w = np.log10(w[0] / w[1])
     assert abs(slope_s - 1.0) < 0.05 and abs(slope_w - 2.0) < 0.1, (slope_s, slope_w)
     assert np.all(np.abs(w[:2] - fx["w64"][:2]) <= 0.02 * np.abs(fx["w64"][:2]))
+
+
+@pytest.mark.parametrize("mode", ["sweeps_of_one_shot", "checkpoints_7"])
+def test_conditioning_with_sweeps_and_checkpoints(oracle_mod, workspace, mode):
+    """All three conditioning stages (windows and weights, band-pass, source
+    update) in a gradient run split into sweeps of one shot, or with
+    checkpoints every 7 steps: residual, updated source rows and misfit equal
+    the single-sweep run's (the stages work shot by shot); gradients against
+    the oracle (its scheme 2 for the checkpoints: the forward segments are
+    recomputed with the original source rows, the reversal subtracts the
+    updated ones, libCUFD.cu:326-329,390-394)."""
+    c, para, stf, cond = _cond_setup(workspace, if_win=True, filter_para=[2.0, 5.0, 25.0, 40.0],
+                                     if_src_update=True)
+    ids = list(range(len(c.src_x)))
+    g = oracle_mod.geom(c.NZ, c.NX, c.nPml, c.nPad, c.nSteps, c.dz, c.dx, c.dt, c.f0)
+    src = [(z + c.nPml, x + c.nPml) for z, x in zip(c.src_z, c.src_x)]
+    recs = [(c.rec_z + c.nPml, c.rec_x + c.nPml)] * len(src)
+    obs = oracle_mod.run(g, 2, *models.moduli(c, "true"), stf, src, recs, precision=32)["syn"]
+    m0 = models.moduli(c, "start")
+
+    # sweeps of one shot need backward groups of one shot (the whole ring of a
+    # group must fit, else the run switches to checkpoints)
+    group = {"FWIFLOW_SHOT_GROUP": "1"} if mode == "sweeps_of_one_shot" else {}
+
+    def run(budget=None, k=None):
+        with env(**group), fwiflow.Plan(para, ids) as P:
+            P.set_model(*m0)
+            P.set_stf(stf)
+            P.set_obs(obs)
+            if budget is not None:
+                P.set_memory_budget(budget)
+            if k is not None:
+                P.set_checkpoint(k)
+            P.run(1)
+            return P.misfit(), P.residual(), P.source(), P.gradients(), P.memory_plan()
+
+    J0, r0, s0, g0, p0 = run()
+    if mode == "sweeps_of_one_shot":
+        J1, r1, s1, g1, p1 = run(budget=1.2 * p0[2] / len(ids))
+        assert p1[0] == 0 and p1[1] == 1
+        o = oracle_mod.run(g, 1, *m0, stf, src, recs, obs, precision=32, cond=cond)
+    else:
+        J1, r1, s1, g1, p1 = run(k=7)
+        assert p1[0] == 7
+        o = oracle_mod.run(g, 1, *m0, stf, src, recs, obs, precision=32, cond=cond, scheme=2,
+                           ckpt_every=7)
+    say(mode, "plan", p1, "misfit", J1, J0)
+    assert np.array_equal(r1, r0) and np.array_equal(s1, s0)
+    assert J1 == pytest.approx(J0, rel=1e-12)
+    for name, a, b in zip(("lam", "mu", "den"), g1[:3], (o["grad_lambda"], o["grad_mu"],
+                                                          o["grad_den"])):
+        e = rel(a, b)
+        say(mode, name, "rel to oracle", e)
+        assert e < GRAD_TOL

@@ -22,13 +22,15 @@
 //               conversions promote exactly where the CUDA source promotes.
 //               Compiled with -ffp-contract=off (no FMA contraction).
 //   T = double: an FP64 run of the same scheme.
-// Two backward schemes:
+// Backward schemes:
 //   scheme 0  : the reference's: a 5-layer boundary ring of all 5 wavefield
 //               components is stored for every step and the forward wavefield
 //               is re-materialised by running the update backwards in time.
 //   scheme 1  : exact: full forward state checkpoints every `ckpt_every` steps,
 //               segments recomputed forward and replayed backwards.  In exact
 //               arithmetic both schemes give the same gradient.
+//   scheme 3  : no reference counterpart: the exact adjoint (transpose of the
+//               forward step) on the forward states of scheme 1.
 //   scheme 2  : scheme 0 plus interior resets: the 5 components on the interior
 //               rectangle [nPml, NZ-nPad-1-nPml] x [nPml, NX-1-nPml] are saved
 //               before forward step it and written back after reverse step it,
@@ -764,6 +766,151 @@ void el_stress_adj(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpml<
   }
 }
 
+// ---------------------------------------------------------------- exact adjoint
+// No reference counterpart: the transpose of the forward time step above
+// (el_stress_fwd, add_source, el_velocity_fwd, recording), taken operation
+// by operation, so that its gradient is the derivative of the discrete
+// misfit in every setting (C-PML, heterogeneous media, Vs = 0 layers).  The
+// adjoint variables are dJ/d(state): a stencil difference f -> D f becomes
+// its transpose acting on the coefficient-weighted adjoint (coefficients
+// inside the derivative), and a C-PML term d~ = d/K + psi', psi' = b psi +
+// a d becomes d_adj = g/K + a (g + P), P <- b (g + P) for the adjoint P of
+// psi.  Forward gates (including el_velocity's x > nx-nPml) are kept.
+
+// f_adj[j] += (D^T B)[j] for D = D+ (f[k+1]-f[k], c2: f[k+2]-f[k-1]) along a
+// stride (1: z, nz: x) over the update region's rows k (B zero elsewhere)
+template <typename T>
+inline T dplus_T(const T* B, size_t j, size_t st, T c1, T c2, T h) {
+  return (c1 * (B[j - st] - B[j]) - c2 * (B[j - 2 * st] - B[j + st])) / h;
+}
+// D = D- (f[k]-f[k-1], c2: f[k+1]-f[k-2])
+template <typename T>
+inline T dminus_T(const T* B, size_t j, size_t st, T c1, T c2, T h) {
+  return (c1 * (B[j] - B[j + st]) - c2 * (B[j - st] - B[j + 2 * st])) / h;
+}
+
+// one C-PML adjoint: g = dJ/d(d~) at a point with a gated memory variable
+template <typename T>
+inline T pml_T(bool gated, T g, T K, T a, T b, T& P) {
+  if (!gated) return g;
+  const T tot = g + P;
+  P = b * tot;
+  return g / K + a * tot;
+}
+
+// Transpose of el_velocity_fwd at step it: adds dJ/d(sigma) to a's
+// stresses from a's velocities; rho terms of the gradient (interior, as the
+// reference's) from the forward stresses sigma^{it+1} in w.
+template <typename T>
+void exact_velocity_T(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpml<T>& pm,
+                      State<T>& a, Wave<T> w, T* den_grad) {
+  const int nz = c.g.nz, nx = c.g.nx, nPml = c.g.nPml, nPad = c.g.nPad, nthreads = c.nthreads;
+  const int nzA = nz - nPad;
+  const size_t n = (size_t)nz * nx;
+  const T c1 = 9.0 / 8.0, c2 = 1.0 / 24.0;
+  const T dz = p.dz, dx = p.dx, dt = p.dt;
+  const T *ba = m.byc_a.data(), *bb = m.byc_b.data();
+  std::vector<T> Bz(n, 0), Bzx(n, 0), Bxz(n, 0), Bx(n, 0), ga(n, 0), gb(n, 0);
+  const int xr = (g_mutation & 0xff) == kMutVelGate ? nx - nPml - 1 : nx - nPml;
+  OMP_FOR
+  for (int x = 2; x <= nx - 3; x++) {
+    for (int z = 2; z <= nzA - 3; z++) {
+      const size_t i = (size_t)x * nz + z;
+      const bool pz = z < nPml || z > nz - nPml - nPad - 1, px = x < nPml || x > xr;
+      const T gz = dt * ba[i] * a.vz[i], gx = dt * bb[i] * a.vx[i];
+      Bz[i] = pml_T(pz, gz, pm.K_z_half[z], pm.a_z_half[z], pm.b_z_half[z], a.m_dszz_dz[i]);
+      Bzx[i] = pml_T(px, gz, pm.K_x[x], pm.a_x[x], pm.b_x[x], a.m_dsxz_dx[i]);
+      Bxz[i] = pml_T(pz, gx, pm.K_z[z], pm.a_z[z], pm.b_z[z], a.m_dsxz_dz[i]);
+      Bx[i] = pml_T(px, gx, pm.K_x_half[x], pm.a_x_half[x], pm.b_x_half[x], a.m_dsxx_dx[i]);
+      if (z >= nPml && z <= nzA - 1 - nPml && x >= nPml && x <= nx - 1 - nPml) {
+        const T dszz_dz = (c1 * (w.szz[i + 1] - w.szz[i]) - c2 * (w.szz[i + 2] - w.szz[i - 1])) / dz;
+        const T dsxz_dx = (c1 * (w.sxz[i] - w.sxz[i - nz]) - c2 * (w.sxz[i + nz] - w.sxz[i - 2 * nz])) / dx;
+        const T dsxz_dz = (c1 * (w.sxz[i] - w.sxz[i - 1]) - c2 * (w.sxz[i + 1] - w.sxz[i - 2])) / dz;
+        const T dsxx_dx = (c1 * (w.sxx[i + nz] - w.sxx[i]) - c2 * (w.sxx[i + 2 * nz] - w.sxx[i - nz])) / dx;
+        // d(ba)/d(rho) = -ba^2/2 for both rho cells of the average
+        ga[i] = a.vz[i] * dt * (dszz_dz + dsxz_dx) * (-ba[i] * ba[i] / 2);
+        gb[i] = a.vx[i] * dt * (dsxz_dz + dsxx_dx) * (-bb[i] * bb[i] / 2);
+      }
+    }
+  }
+  OMP_FOR
+  for (int x = 2; x <= nx - 3; x++) {
+    for (int z = 2; z <= nzA - 3; z++) {
+      const size_t i = (size_t)x * nz + z;
+      a.szz[i] += dplus_T(Bz.data(), i, 1, c1, c2, dz);
+      a.sxz[i] += dminus_T(Bzx.data(), i, (size_t)nz, c1, c2, dx) +
+                  dminus_T(Bxz.data(), i, 1, c1, c2, dz);
+      a.sxx[i] += dplus_T(Bx.data(), i, (size_t)nz, c1, c2, dx);
+    }
+  }
+  for (int x = 2; x <= nx - 3; x++)
+    for (int z = 2; z <= nzA - 3; z++) {
+      const size_t i = (size_t)x * nz + z;
+      den_grad[i] += ga[i] + gb[i];
+      den_grad[i + 1] += ga[i];
+      den_grad[i + nz] += gb[i];
+    }
+}
+
+// Transpose of el_stress_fwd at step it: adds dJ/d(v) to a's velocities from
+// a's stresses; lambda / mu terms of the gradient (interior; x1e6 for MPa)
+// from the forward velocities v^it in w, the mu spray onto the 4 cells of
+// the harmonic average unconditionally.
+template <typename T>
+void exact_stress_T(const Ctx& c, const Prm<T>& p, const Model<T>& m, const Cpml<T>& pm,
+                    State<T>& a, Wave<T> w, T* lam_grad, T* mu_grad) {
+  const int nz = c.g.nz, nx = c.g.nx, nPml = c.g.nPml, nPad = c.g.nPad, nthreads = c.nthreads;
+  const int nzA = nz - nPad;
+  const size_t n = (size_t)nz * nx;
+  const T c1 = 9.0 / 8.0, c2 = 1.0 / 24.0;
+  const T dz = p.dz, dx = p.dx, dt = p.dt;
+  const T *lam = m.lam.data(), *mu = m.mu.data(), *amu = m.ave_mu.data();
+  std::vector<T> Azz(n, 0), Axx(n, 0), Axz(n, 0), Azx(n, 0), gs(n, 0);
+  OMP_FOR
+  for (int x = 2; x <= nx - 3; x++) {
+    for (int z = 2; z <= nzA - 3; z++) {
+      const size_t i = (size_t)x * nz + z;
+      const bool pz = z < nPml || z > nz - nPml - nPad - 1, px = x < nPml || x > nx - nPml - 1;
+      const T l2m = lam[i] + 2.0 * mu[i];
+      const T Gzz = dt * (l2m * a.szz[i] + lam[i] * a.sxx[i]);
+      const T Gxx = dt * (lam[i] * a.szz[i] + l2m * a.sxx[i]);
+      const T Gs = dt * amu[i] * a.sxz[i];
+      Azz[i] = pml_T(pz, Gzz, pm.K_z[z], pm.a_z[z], pm.b_z[z], a.m_dvz_dz[i]);
+      Axx[i] = pml_T(px, Gxx, pm.K_x[x], pm.a_x[x], pm.b_x[x], a.m_dvx_dx[i]);
+      Axz[i] = pml_T(pz, Gs, pm.K_z_half[z], pm.a_z_half[z], pm.b_z_half[z], a.m_dvx_dz[i]);
+      Azx[i] = pml_T(px, Gs, pm.K_x_half[x], pm.a_x_half[x], pm.b_x_half[x], a.m_dvz_dx[i]);
+      if (z >= nPml && z <= nzA - 1 - nPml && x >= nPml && x <= nx - 1 - nPml) {
+        const T dvz_dz = (c1 * (w.vz[i] - w.vz[i - 1]) - c2 * (w.vz[i + 1] - w.vz[i - 2])) / dz;
+        const T dvx_dx = (c1 * (w.vx[i] - w.vx[i - nz]) - c2 * (w.vx[i + nz] - w.vx[i - 2 * nz])) / dx;
+        const T dvx_dz = (c1 * (w.vx[i + 1] - w.vx[i]) - c2 * (w.vx[i + 2] - w.vx[i - 1])) / dz;
+        const T dvz_dx = (c1 * (w.vz[i + nz] - w.vz[i]) - c2 * (w.vz[i + 2 * nz] - w.vz[i - nz])) / dx;
+        lam_grad[i] += (a.szz[i] + a.sxx[i]) * (dvz_dz + dvx_dx) * dt * kMEGA;
+        mu_grad[i] += 2.0 * (a.szz[i] * dvz_dz + a.sxx[i] * dvx_dx) * dt * kMEGA;
+        gs[i] = a.sxz[i] * (dvx_dz + dvz_dx) * dt * kMEGA;  // dJ/d(amu), per MPa
+      }
+    }
+  }
+  OMP_FOR
+  for (int x = 2; x <= nx - 3; x++) {
+    for (int z = 2; z <= nzA - 3; z++) {
+      const size_t i = (size_t)x * nz + z;
+      a.vz[i] += dminus_T(Azz.data(), i, 1, c1, c2, dz) +
+                 dplus_T(Azx.data(), i, (size_t)nz, c1, c2, dx);
+      a.vx[i] += dminus_T(Axx.data(), i, (size_t)nz, c1, c2, dx) +
+                 dplus_T(Axz.data(), i, 1, c1, c2, dz);
+    }
+  }
+  // harmonic average: d(amu)/d(mu_k) = amu^2 / (4 mu_k^2)
+  for (int x = 2; x <= nx - 3; x++)
+    for (int z = 2; z <= nzA - 3; z++) {
+      const size_t i = (size_t)x * nz + z;
+      if (gs[i] == 0 || amu[i] == 0) continue;
+      const double q = (double)amu[i] * (double)amu[i] / 4.0;
+      for (size_t k : {i, i + 1, i + (size_t)nz, i + (size_t)nz + 1})
+        mu_grad[k] += (T)((double)gs[i] * q / ((double)mu[k] * (double)mu[k]));
+    }
+}
+
 // ---------------------------------------------------------------- ring
 // Boundary.cu:17-41 and from_bnd/to_bnd (utilities.cu:361-424): for every step
 // the 5 layers at rows/columns nPml-2 .. nPml+2 of each of the 4 sides.
@@ -883,6 +1030,8 @@ struct Runner {
     std::vector<T> src(nSteps);
     for (int it = 0; it < nSteps; it++) src[it] = (T)sh.stf[it];
     window_default<T>(nSteps, 1, p.dt, (T)0.001, src.data());
+    std::vector<T> src_window(nSteps, (T)1);  // d(src)/d(stf), for the exact adjoint
+    window_default<T>(nSteps, 1, p.dt, (T)0.001, src_window.data());
     const std::vector<T> src_fwd = src;  // the source update replaces src for the reversal
 
     State<T> s(n);
@@ -892,7 +1041,7 @@ struct Runner {
     std::vector<T> rbuf;  // scheme 0: [it][field][ring]
     std::vector<std::vector<T>> ckpt;  // scheme 1
     const int k = std::max(1, ckpt_every);
-    if (withAdj && scheme != 1) rbuf.assign((size_t)nSteps * 5 * rl, (T)0);
+    if (withAdj && (scheme == 0 || scheme == 2)) rbuf.assign((size_t)nSteps * 5 * rl, (T)0);
     std::vector<std::vector<T>> interior;  // scheme 2: [checkpoint][5 * NZ*NX]
     for (int it = 0; it <= nSteps - 2; it++) {
       if (withAdj && scheme == 2 && it > 0 && it % k == 0) {
@@ -901,13 +1050,13 @@ struct Runner {
         for (auto* v : {&s.vz, &s.vx, &s.szz, &s.sxx, &s.sxz}) cp.insert(cp.end(), v->begin(), v->end());
         interior.push_back(std::move(cp));
       }
-      if (withAdj && scheme != 1) {
+      if (withAdj && (scheme == 0 || scheme == 2)) {
         T* dst = rbuf.data() + (size_t)it * 5 * rl;
         const T* f[5] = {s.szz.data(), s.sxz.data(), s.sxx.data(), s.vz.data(), s.vx.data()};
         for (int q = 0; q < 5; q++)
           for (size_t j = 0; j < rl; j++) dst[q * rl + j] = f[q][ring.idx[j]];
       }
-      if (withAdj && scheme == 1 && it % k == 0) {
+      if (withAdj && (scheme == 1 || scheme == 3) && it % k == 0) {
         std::vector<T> cp;
         cp.reserve(13 * n);
         for (auto* v : s.all()) cp.insert(cp.end(), v->begin(), v->end());
@@ -987,11 +1136,13 @@ struct Runner {
       const size_t i = (size_t)sh.x_src * nz + sh.z_src;
       stf_grad[it] = -(a.szz[i] + kRSXXZZ * a.sxx[i]) * p.dt;
     };
-    el_velocity_adj(c, p, m, pm, a);
-    res_injection(nSteps - 1);
-    el_stress_adj(c, p, m, pm, a);
+    if (scheme != 3) {
+      el_velocity_adj(c, p, m, pm, a);
+      res_injection(nSteps - 1);
+      el_stress_adj(c, p, m, pm, a);
+    }
 
-    if (scheme != 1) {
+    if (scheme == 0 || scheme == 2) {
       auto reset = [&](int it) {  // scheme 2: interior state before forward step it
         const std::vector<T>& cp = interior[(size_t)(it / k) - 1];
         const int nzA = g.nz - g.nPad, P0 = g.nPml;
@@ -1059,6 +1210,23 @@ struct Runner {
           T* szz = sig.data() + (q * 3 + 0) * n;
           T* sxx = sig.data() + (q * 3 + 1) * n;
           T* sxz = sig.data() + (q * 3 + 2) * n;
+          if (scheme == 3) {
+            // transpose of forward step it: recording of sample it+1 (the
+            // residual is -dJ/d(data)), velocity update, source, stress update
+            for (int r = 0; r < sh.nrec; r++) {
+              const size_t i = (size_t)sh.x_rec[r] * nz + sh.z_rec[r];
+              const T f = -res[(size_t)r * nSteps + it + 1];
+              a.szz[i] += f;
+              a.sxx[i] += kRSXXZZ * f;
+            }
+            exact_velocity_T(c, p, m, pm, a, Wave<T>{vz, vx, szz, sxx, sxz}, den_grad.data());
+            const size_t is = (size_t)sh.x_src * nz + sh.z_src;
+            stf_grad[it] = (T)std::pow(1500.0, 2) * p.dt * (a.szz[is] + kRSXXZZ * a.sxx[is]) *
+                           src_window[it];
+            exact_stress_T(c, p, m, pm, a, Wave<T>{vz, vx, szz, sxx, sxz}, lam_grad.data(),
+                           mu_grad.data());
+            continue;
+          }
           source_grad(it);
           el_velocity_rev(c, p, m, Wave<T>{vz, vx, szz, sxx, sxz}, false, a.vz.data(),
                           a.vx.data(), den_grad.data(), ga, gb);

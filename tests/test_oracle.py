@@ -514,3 +514,42 @@ def test_reference_adjoint_grows_with_a_zero_shear_layer(oracle_mod, workspace):
             rms[(water, nt)] = float(np.sqrt(np.mean(gL ** 2)))
     assert rms[(True, 1200)] > 1e3 * rms[(True, 400)], rms
     assert rms[(False, 1200)] < 1e2 * rms[(False, 400)], rms
+
+
+@pytest.mark.parametrize("water", [False, True], ids=["heterogeneous", "zero_shear_layer"])
+def test_exact_adjoint_equals_central_differences(oracle_mod, workspace, water):
+    """Scheme 3 (no reference counterpart): the transpose of the forward time
+    step, on the forward states of the exact recompute scheme.  Heterogeneous
+    model, waves in the C-PML, 400 steps; also with Vs = 0 in the top 5
+    rows, where the reference's adjoint is unstable: the lambda, mu, rho and
+    source-function gradients equal central differences of the FP64 misfit
+    to 1e-7 in smooth random directions (the reference's scheme on the same
+    directions: 4%-15%, and its stf gradient has neither the 1500^2 scale nor
+    the source window's chain rule, utilities.cu:582-593)."""
+    c = models.case_small(nz=40, nx=60, nSteps=400, nshots=2)
+    if water:
+        for a in (c.cs, c.cs0):
+            a[:5] = 0.0
+    para, stf, ids = _setup(oracle_mod, c, workspace)
+    m0 = list(models.moduli(c, "start"))
+    NZ, NX = m0[0].shape
+    v = np.zeros((NZ, NX))
+    top = c.nPml + (7 if water else 4)  # mu = 0 rows are not differentiable in mu
+    v[top:NZ - c.nPad - c.nPml - 4, c.nPml + 4:NX - c.nPml - 4] = 1
+    rng = np.random.default_rng(5)
+    _, gL, gM, gD, gS, _ = oracle_mod.cufd(*m0, stf, 1, ids, para, precision=64, scheme=3,
+                                           ckpt_every=50)
+    J = lambda *m: oracle_mod.cufd(*m, stf, 0, ids, para, precision=64)[0]  # noqa: E731
+    h = 1e-3
+    for k, (g, sc) in enumerate(((gL, 1.0), (gM, 1.0), (gD, 10.0))):
+        d = gaussian_filter(rng.standard_normal((NZ, NX)), 3) * v
+        d *= sc / np.abs(d).max()
+        mp = [a + h * d if q == k else a for q, a in enumerate(m0)]
+        mm = [a - h * d if q == k else a for q, a in enumerate(m0)]
+        fd = (J(*mp) - J(*mm)) / (2 * h)
+        assert np.sum(g * d) == pytest.approx(fd, rel=1e-7), k
+    ds = np.zeros_like(stf)
+    ds[:, 20:-20] = rng.standard_normal((stf.shape[0], stf.shape[1] - 40))
+    Js = lambda s: oracle_mod.cufd(*m0, s, 0, ids, para, precision=64)[0]  # noqa: E731
+    fd = (Js(stf + h * ds) - Js(stf - h * ds)) / (2 * h)
+    assert np.sum(gS * ds) == pytest.approx(fd, rel=1e-7)

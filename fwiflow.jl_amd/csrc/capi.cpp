@@ -206,6 +206,9 @@ int fwiflow_plan_dims(fwiflow_plan* plan, int* dims) {
 int fwiflow_plan_set_checkpoint(fwiflow_plan* plan, int ckpt_every) {
   PLAN_CALL(plan->impl.set_checkpoint(ckpt_every));
 }
+int fwiflow_plan_set_adjoint(fwiflow_plan* plan, int mode) {
+  PLAN_CALL(plan->impl.set_adjoint(mode));
+}
 int fwiflow_plan_set_timing(fwiflow_plan* plan, int enable) {
   PLAN_CALL(plan->impl.set_timing(enable != 0));
 }

@@ -207,6 +207,11 @@ struct Part {
   int nrec;         // receivers of the part's shots
   int gzU;          // tiles along z of the update-region tiling
   int gzR;          // tiles along z of the ring-rectangle tiling
+  // exact adjoint (the transpose of the forward step; no reference
+  // counterpart): the reversal kernels then keep no gradient sums
+  int exact;
+  float* xt;             // transposed-stencil operands [group shot][8][cell]
+  const float* srcwin;   // d(source amplitude)/d(stf) of each step
 };
 
 // Kernel argument: one part by value, or a table of parts in device memory.
@@ -594,6 +599,7 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
   const float ba = P.m.ba[i], bb = P.m.bb[i];
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
   const int ns = g.ns;
+  const bool sums = !P.exact;
   float ga = 0.f, gb = 0.f;
   for (int s0 = 0; s0 < ns; s0 += U) {
     RevVelIn r[U];
@@ -609,7 +615,7 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
         q.sxzm2 = sxc[0], q.sxzm1 = sxc[1], q.sxz0 = sxc[2], q.sxzp1 = sxc[3];
         q.sxzmx = lx(W, sw, kSxz, C, -1), q.sxzpx = lx(W, sw, kSxz, C, 1), q.sxzm2x = lx(W, sw, kSxz, C, -2);
         q.sxxpx = lx(W, sw, kSxx, C, 1), q.sxx0 = lz(W, sw, kSxx, C, 0), q.sxxp2x = lx(W, sw, kSxx, C, 2), q.sxxmx = lx(W, sw, kSxx, C, -1);
-        q.avz = lz(A, sw, kVz, C, 0), q.avx = lz(A, sw, kVx, C, 0);
+        if (sums) q.avz = lz(A, sw, kVz, C, 0), q.avx = lz(A, sw, kVx, C, 0);
       }
       if (slot < 0) {
         q.vz = lz(W, sw, kVz, C, 0), q.vx = lz(W, sw, kVx, C, 0);
@@ -631,8 +637,10 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
         const float dsxz_dz = dfw(q.sxz0, q.sxzm1, q.sxzp1, q.sxzm2, dz, rdz);
         const float dsxx_dx = dfw(q.sxxpx, q.sxx0, q.sxxp2x, q.sxxmx, dx, rdx);
         const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
-        ga += q.avz * fz;
-        gb += q.avx * fx;
+        if (sums) {
+          ga += q.avz * fz;
+          gb += q.avx * fx;
+        }
         if (slot < 0) {
           vz = vz - fz * ba * dt;
           vx = vx - fx * bb * dt;
@@ -642,7 +650,7 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
       st(W, sw, kVx, C, vx);
     }
   }
-  if (in) {
+  if (in && sums) {
     P.acc[3 * g.N + i] += ga;
     P.acc[4 * g.N + i] += gb;
   }
@@ -680,6 +688,7 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
   const double dt64 = (double)dt;
   const int ns = g.ns;
+  const bool sums = !P.exact;
   float gl = 0.f, gm = 0.f, gs = 0.f;
   for (int s0 = 0; s0 < ns; s0 += U) {
     RevStressIn r[U];
@@ -695,7 +704,8 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
         q.vzpx = lx(W, sw, kVz, C, 1), q.vzp2x = lx(W, sw, kVz, C, 2), q.vzmx = lx(W, sw, kVz, C, -1);
         q.vxm1 = vxc[0], q.vx0 = vxc[1], q.vxp1 = vxc[2], q.vxp2 = vxc[3];
         q.vxmx = lx(W, sw, kVx, C, -1), q.vxpx = lx(W, sw, kVx, C, 1), q.vxm2x = lx(W, sw, kVx, C, -2);
-        q.aszz = lz(A, sw, kSzz, C, 0), q.asxx = lz(A, sw, kSxx, C, 0), q.asxz = lz(A, sw, kSxz, C, 0);
+        if (sums)
+          q.aszz = lz(A, sw, kSzz, C, 0), q.asxx = lz(A, sw, kSxx, C, 0), q.asxz = lz(A, sw, kSxz, C, 0);
       }
       if (slot < 0) {
         q.szz = lz(W, sw, kSzz, C, 0), q.sxx = lz(W, sw, kSxx, C, 0), q.sxz = lz(W, sw, kSxz, C, 0);
@@ -718,9 +728,11 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
         const float dvx_dx = dfw(q.vx0, q.vxmx, q.vxpx, q.vxm2x, dx, rdx);
         const float dvx_dz = dfw(q.vxp1, q.vx0, q.vxp2, q.vxm1, dz, rdz);
         const float dvz_dx = dfw(q.vzpx, q.vz0, q.vzp2x, q.vzmx, dx, rdx);
-        gl += (q.aszz + q.asxx) * (dvz_dz + dvx_dx);
-        gm += q.aszz * dvz_dz + q.asxx * dvx_dx;
-        gs += q.asxz * (dvx_dz + dvz_dx);
+        if (sums) {
+          gl += (q.aszz + q.asxx) * (dvz_dz + dvx_dx);
+          gm += q.aszz * dvz_dz + q.asxx * dvx_dx;
+          gs += q.asxz * (dvx_dz + dvz_dx);
+        }
         if (slot < 0) {
           if (z == q.sz && x == q.sx) {
             const float amp = P.src.amp[(size_t)s * g.nSteps + it];
@@ -737,7 +749,7 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
       st(W, sw, kSxz, C, sxz);
     }
   }
-  if (in) {
+  if (in && sums) {
     P.acc[i] += gl;
     P.acc[g.N + i] += gm;
     P.acc[2 * g.N + i] += gs;
@@ -986,6 +998,208 @@ __device__ __forceinline__ void adj_stress(const Part& P, int tile, int it) {
   }
 }
 
+// ------------------------------------------------------------ exact adjoint
+// No reference counterpart: the transpose of the forward time step
+// (el_stress_fwd, add_source, el_velocity_fwd, recording), taken operation by
+// operation, so that the gradient is the derivative of the discrete misfit
+// also where waves enter the C-PML and in zero-shear layers.  The adjoint
+// state keeps the reference's sign (the residual is injected with +, so it is
+// -dJ/d(state)) and the gradient sums keep the meaning of the reference
+// mode's, so k_finalize turns both into (lambda, mu, rho) gradients.
+//
+// A stencil difference f -> D f of the forward step becomes its transpose
+// acting on the coefficient-weighted adjoint, and a C-PML term
+// d~ = d / K + psi', psi' = b psi + a d becomes, for the adjoint P of psi,
+// d_adj = g / K + a (g + P), P <- b (g + P) (only where the forward updates
+// psi, with the forward's gates, including el_velocity.cu:56,71's
+// x > nx - nPml).  Each transposed update runs as two passes over the
+// update region: the first forms the weighted operands of every cell
+// (xt, zero outside the update region), the second applies the transposed
+// stencils to them.  The second pass of one update and the first of the next
+// touch only their own cell's adjoint, so they share a launch: k_exact_a
+// applies the transposed stress update of step it+1 to v_adj and forms the
+// velocity operands of step it; k_exact_b applies those to sigma_adj (with
+// the residual of sample it+1) and forms the stress operands of step it.
+// xt holds two operand sets per shot (slots 0-3: stress update, 4-7:
+// velocity update).
+__device__ __forceinline__ float pml_t(bool gated, float g, float K, float a, float b, float& P) {
+  if (!gated) return g;
+  const float tot = g + P;
+  P = b * tot;
+  return g / K + a * tot;
+}
+// (D+)^T and (D-)^T along a stride st (1: z, nz: x), for
+// D+ f = (c1 (f[k+1] - f[k]) - c2 (f[k+2] - f[k-1])) / h and
+// D- f = (c1 (f[k] - f[k-1]) - c2 (f[k+1] - f[k-2])) / h
+__device__ __forceinline__ float dpt(const float* B, long long j, long long st, float h) {
+  return (kC1 * (B[j - st] - B[j]) - kC2 * (B[j - 2 * st] - B[j + st])) / h;
+}
+__device__ __forceinline__ float dmt(const float* B, long long j, long long st, float h) {
+  return (kC1 * (B[j] - B[j + st]) - kC2 * (B[j - st] - B[j + 2 * st])) / h;
+}
+
+// Transposed stress update of step it+1 (unless `first`), then the velocity
+// operands of step it: B = C-PML-adjoint(dt b v_adj) per derivative, and the
+// density sums acc_a += vz_adj (dszz/dz + dsxz/dx), acc_b += vx_adj (dsxz/dz +
+// dsxx/dx) from the forward stresses sigma^{it+1} held in w.
+__global__ __launch_bounds__(256) void k_exact_a(Part P, int first) {
+  const Geo& g = P.g;
+  const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int x = wave_col((int)(blockIdx.y * blockDim.y));
+  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  const int nz = g.nz;
+  const long long N = g.N, i = (long long)x * nz + z;
+  const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
+  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
+  const bool pxv = x < g.nPml || x > g.nx - g.nPml;  // el_velocity's gate
+  const float ba = P.m.ba[i], bb = P.m.bb[i];
+  const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  float ga = 0.f, gb = 0.f;
+  for (int s = 0; s < g.ns; ++s) {
+    float* a = P.a + (size_t)s * kWaveComps * N;
+    float* m = P.p + (size_t)s * kMemComps * N;
+    float* xs = P.xt + (size_t)s * 8 * N;  // stress operands Azz, Axx, Axz, Azx
+    float* xv = xs + 4 * N;                // velocity operands Bz, Bzx, Bxz, Bx
+    float vz = a[kVz * N + i], vx = a[kVx * N + i];
+    if (!first) {
+      vz = vz + (dmt(xs, i, 1, dz) + dpt(xs + 3 * N, i, nz, dx));
+      vx = vx + (dmt(xs + N, i, nz, dx) + dpt(xs + 2 * N, i, 1, dz));
+      a[kVz * N + i] = vz;
+      a[kVx * N + i] = vx;
+    }
+    const float gz = dt * ba * vz, gx = dt * bb * vx;
+    if (pz) {
+      float q = m[kMszz * N + i];
+      xv[i] = pml_t(true, gz, P.c.Kzh[z], P.c.azh[z], P.c.bzh[z], q);
+      m[kMszz * N + i] = q;
+      q = m[kMsxzz * N + i];
+      xv[2 * N + i] = pml_t(true, gx, P.c.Kz[z], P.c.az[z], P.c.bz[z], q);
+      m[kMsxzz * N + i] = q;
+    } else {
+      xv[i] = gz;
+      xv[2 * N + i] = gx;
+    }
+    if (pxv) {
+      float q = m[kMsxzx * N + i];
+      xv[N + i] = pml_t(true, gz, P.c.Kx[x], P.c.ax[x], P.c.bx[x], q);
+      m[kMsxzx * N + i] = q;
+      q = m[kMsxx * N + i];
+      xv[3 * N + i] = pml_t(true, gx, P.c.Kxh[x], P.c.axh[x], P.c.bxh[x], q);
+      m[kMsxx * N + i] = q;
+    } else {
+      xv[N + i] = gz;
+      xv[3 * N + i] = gx;
+    }
+    if (in) {
+      const float* w = P.w + (size_t)s * kWaveComps * N;
+      const float* szz = w + kSzz * N;
+      const float* sxx = w + kSxx * N;
+      const float* sxz = w + kSxz * N;
+      const float dszz_dz = dfw(szz[i + 1], szz[i], szz[i + 2], szz[i - 1], dz, rdz);
+      const float dsxz_dx = dfw(sxz[i], sxz[i - nz], sxz[i + nz], sxz[i - 2 * nz], dx, rdx);
+      const float dsxz_dz = dfw(sxz[i], sxz[i - 1], sxz[i + 1], sxz[i - 2], dz, rdz);
+      const float dsxx_dx = dfw(sxx[i + nz], sxx[i], sxx[i + 2 * nz], sxx[i - nz], dx, rdx);
+      ga += vz * (dszz_dz + dsxz_dx);
+      gb += vx * (dsxz_dz + dsxx_dx);
+    }
+  }
+  if (in) {
+    P.acc[3 * N + i] += ga;
+    P.acc[4 * N + i] += gb;
+  }
+}
+
+// Residual of sample it+1 (surveys with the folded injection) and the
+// transposed velocity update of step it into sigma_adj; the stf gradient of
+// step it; the stress operands of step it: A = C-PML-adjoint(dt C sigma_adj)
+// per derivative; and the sums acc_l += (szz_adj + sxx_adj)(dvz/dz + dvx/dx),
+// acc_m += szz_adj dvz/dz + sxx_adj dvx/dx, acc_s += sxz_adj (dvx/dz + dvz/dx)
+// from the forward velocities v^it held in w.
+__global__ __launch_bounds__(256) void k_exact_b(Part P, int it) {
+  const Geo& g = P.g;
+  const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int x = wave_col((int)(blockIdx.y * blockDim.y));
+  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  const int nz = g.nz;
+  const long long N = g.N, i = (long long)x * nz + z;
+  const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
+  const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
+  const bool pxs = x < g.nPml || x > g.nx - g.nPml - 1;
+  const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i];
+  const float l2m = (float)((double)lam + 2.0 * (double)mu);
+  const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  const bool inj_row = P.inj.res && z == P.inj.z;
+  const float* res_row = P.inj.res ? P.inj.res + (size_t)(it + 1) * g.R : nullptr;
+  float gl = 0.f, gm = 0.f, gs = 0.f;
+  for (int s = 0; s < g.ns; ++s) {
+    float* a = P.a + (size_t)s * kWaveComps * N;
+    float* m = P.p + (size_t)s * kMemComps * N;
+    float* xs = P.xt + (size_t)s * 8 * N;
+    const float* xv = xs + 4 * N;
+    float szz = a[kSzz * N + i], sxx = a[kSxx * N + i], sxz = a[kSxz * N + i];
+    if (inj_row) {
+      const int k = P.inj.at[(size_t)(g.s0 + s) * g.nx + x];
+      if (k >= 0) {
+        const float v = res_row[k];
+        szz = szz + v;
+        sxx = (float)((double)sxx + kRS * (double)v);
+      }
+    }
+    szz = szz + dpt(xv, i, 1, dz);
+    sxz = sxz + (dmt(xv + N, i, nz, dx) + dmt(xv + 2 * N, i, 1, dz));
+    sxx = sxx + dpt(xv + 3 * N, i, nz, dx);
+    a[kSzz * N + i] = szz;
+    a[kSxx * N + i] = sxx;
+    a[kSxz * N + i] = sxz;
+    if (P.stf_grad && z == P.src.zx[2 * s] && x == P.src.zx[2 * s + 1])
+      P.stf_grad[(size_t)s * g.nSteps + it] =
+          (float)(-(double)kSrcScale * (double)dt * ((double)szz + kRS * (double)sxx) *
+                  (double)P.srcwin[it]);
+    const float Gzz = dt * (l2m * szz + lam * sxx);
+    const float Gxx = dt * (lam * szz + l2m * sxx);
+    const float Gs = dt * amu * sxz;
+    if (pz) {
+      float q = m[kMvzz * N + i];
+      xs[i] = pml_t(true, Gzz, P.c.Kz[z], P.c.az[z], P.c.bz[z], q);
+      m[kMvzz * N + i] = q;
+      q = m[kMvxz * N + i];
+      xs[2 * N + i] = pml_t(true, Gs, P.c.Kzh[z], P.c.azh[z], P.c.bzh[z], q);
+      m[kMvxz * N + i] = q;
+    } else {
+      xs[i] = Gzz;
+      xs[2 * N + i] = Gs;
+    }
+    if (pxs) {
+      float q = m[kMvxx * N + i];
+      xs[N + i] = pml_t(true, Gxx, P.c.Kx[x], P.c.ax[x], P.c.bx[x], q);
+      m[kMvxx * N + i] = q;
+      q = m[kMvzx * N + i];
+      xs[3 * N + i] = pml_t(true, Gs, P.c.Kxh[x], P.c.axh[x], P.c.bxh[x], q);
+      m[kMvzx * N + i] = q;
+    } else {
+      xs[N + i] = Gxx;
+      xs[3 * N + i] = Gs;
+    }
+    if (in) {
+      const float* w = P.w + (size_t)s * kWaveComps * N;
+      const float* vzw = w + kVz * N;
+      const float* vxw = w + kVx * N;
+      const float dvz_dz = dfw(vzw[i], vzw[i - 1], vzw[i + 1], vzw[i - 2], dz, rdz);
+      const float dvx_dx = dfw(vxw[i], vxw[i - nz], vxw[i + nz], vxw[i - 2 * nz], dx, rdx);
+      const float dvx_dz = dfw(vxw[i + 1], vxw[i], vxw[i + 2], vxw[i - 1], dz, rdz);
+      const float dvz_dx = dfw(vzw[i + nz], vzw[i], vzw[i + 2 * nz], vzw[i - nz], dx, rdx);
+      gl += (szz + sxx) * (dvz_dz + dvx_dx);
+      gm += szz * dvz_dz + sxx * dvx_dx;
+      gs += sxz * (dvx_dz + dvz_dx);
+    }
+  }
+  if (in) {
+    P.acc[i] += gl;
+    P.acc[N + i] += gm;
+    P.acc[2 * N + i] += gs;
+  }
+}
+
 // ------------------------------------------------------------ step kernels
 template <int NP>
 __device__ __forceinline__ const Part& part_of(const Parts<NP>& ps, unsigned e) {
@@ -1116,9 +1330,12 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int n, double
 // Chain rule from the accumulated derivatives to (lambda, mu, rho), once per
 // run.  Deterministic gather form of the reference's atomicAdd sprays
 // (el_stress.cu:113-124, el_velocity.cu:104-112), with the same target
-// conditions (including the always-true `gidx+1<=gidx<=nx-1-nPml`).
+// conditions (including the always-true `gidx+1<=gidx<=nx-1-nPml`).  With
+// `exact` (the exact adjoint) every cell of a harmonic or arithmetic average
+// receives its share: the conditions that drop the last interior row and
+// column's neighbours are not applied.
 __global__ __launch_bounds__(256) void k_finalize(Geo g, Mdl m, const float* __restrict__ acc,
-                                                  float* __restrict__ grad) {
+                                                  float* __restrict__ grad, int exact) {
   const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int x = (int)blockIdx.y;
   if (z >= g.nz) return;
@@ -1148,9 +1365,10 @@ __global__ __launch_bounds__(256) void k_finalize(Geo g, Mdl m, const float* __r
   float t = 0.f;
   bool any = false;
   if (inI(z, x) && m.amu[i] != 0.f) t += shear(z, x), any = true;
-  if (inI(z - 1, x) && z <= z1 && m.amu[i - 1] != 0.f) t += shear(z - 1, x), any = true;
+  const bool zc = exact || z <= z1, xc = exact || x <= x1;
+  if (inI(z - 1, x) && zc && m.amu[i - 1] != 0.f) t += shear(z - 1, x), any = true;
   if (inI(z, x - 1) && m.amu[i - nz] != 0.f) t += shear(z, x - 1), any = true;
-  if (inI(z - 1, x - 1) && z <= z1 && x <= x1 && m.amu[i - nz - 1] != 0.f)
+  if (inI(z - 1, x - 1) && zc && xc && m.amu[i - nz - 1] != 0.f)
     t += shear(z - 1, x - 1), any = true;
   if (any) {
     const float mu = m.mu[i];
@@ -1159,7 +1377,7 @@ __global__ __launch_bounds__(256) void k_finalize(Geo g, Mdl m, const float* __r
   auto ga = [&](long long j) { const float b = m.ba[j]; return acc_a[j] * dt * (0.5f * b * b); };
   auto gb = [&](long long j) { const float b = m.bb[j]; return acc_b[j] * dt * (0.5f * b * b); };
   if (inI(z, x)) gd += ga(i) + gb(i);
-  if (inI(z - 1, x) && z <= z1) gd += ga(i - 1);
+  if (inI(z - 1, x) && zc) gd += ga(i - 1);
   if (inI(z, x - 1)) gd += gb(i - nz);
   grad[i] = gl;
   grad[N + i] = gm;
@@ -1450,6 +1668,7 @@ struct DeviceState {
   bool graphs = false;     // shot groups' time loops captured and replayed as graphs
                            // (measured no faster at 608x1780; timed runs stay eager)
   int ckpt_every = 0;      // 0: boundary ring every step; k > 0: checkpoints every k steps
+  int adjoint = 0;         // 0: the reference's adjoint; 1: the exact adjoint
   double ring_budget = 32e9;  // device bytes a gradient run may hold for ring + checkpoints
   // model + tables
   float *lam = nullptr, *mu = nullptr, *den = nullptr, *amu = nullptr, *ba = nullptr,
@@ -1458,6 +1677,7 @@ struct DeviceState {
   float* win2 = nullptr;  // nSteps
   int* src = nullptr;     // [ns][2]
   float* srcamp = nullptr;  // [ns][nSteps] windowed source rows
+  float* srcwin = nullptr;  // [nSteps] the source rows' window (exact adjoint's stf gradient)
   int* rec = nullptr;     // [R][3] (z, x, plan shot)
   int* rec_at = nullptr;  // [ns][nx] receiver of each column (one-row surveys)
   int rec_row = -1;
@@ -1468,6 +1688,8 @@ struct DeviceState {
   float *fwd = nullptr, *mem = nullptr, *adj = nullptr, *madj = nullptr;
   float* ring = nullptr;
   size_t ring_floats = 0;
+  float* xt = nullptr;  // exact adjoint: operands [backward group shot][8][cell]
+  size_t xt_floats = 0;
   float* snap = nullptr;
   size_t snap_floats = 0;
   float *syn = nullptr, *obs = nullptr, *res = nullptr, *stage = nullptr;
@@ -1706,6 +1928,7 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   d.fold_inject = env_int("FWIFLOW_FOLD_INJECT", 1) != 0;
   d.part_table = env_int("FWIFLOW_PART_TABLE", 0) != 0;
   d.graphs = env_int("FWIFLOW_GRAPHS", 0) != 0;
+  d.adjoint = env_int("FWIFLOW_EXACT_ADJOINT", 0) != 0 ? 1 : 0;
   if (in_scratch) {
     // every buffer from the device's scratch array, no graphs (instantiating
     // one allocates), the device's shared stream
@@ -1757,6 +1980,10 @@ Status Plan::create(const std::string& para_fname, int gpu, int group_size, cons
   window_squared(g.nSteps, para.dt, 0.005f, w2.data());
   TRY(d.alloc(&d.win2, w2.size()));
   HIP_TRY(hipMemcpy(d.win2, w2.data(), w2.size() * sizeof(float), hipMemcpyHostToDevice));
+  // the source rows' window (Src_Rec.cu:132-143): d(source amplitude)/d(stf)
+  window_squared(g.nSteps, para.dt, 0.001f, w2.data());
+  TRY(d.alloc(&d.srcwin, w2.size()));
+  HIP_TRY(hipMemcpy(d.srcwin, w2.data(), w2.size() * sizeof(float), hipMemcpyHostToDevice));
   std::vector<int> src(2 * g.ns), rec(3 * (size_t)d.R);
   size_t r = 0;
   d.roff.assign(g.ns + 1, 0);
@@ -1964,6 +2191,15 @@ Status Plan::set_checkpoint(int k) {
   d_->ckpt_every = k;
   return Status::Ok();
 }
+
+Status Plan::set_adjoint(int mode) {
+  if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
+  if (mode != 0 && mode != 1) return Status::Err(FWIFLOW_ERR_ARG, "adjoint mode must be 0 or 1");
+  d_->adjoint = mode;
+  return Status::Ok();
+}
+
+int Plan::adjoint() const { return d_ ? d_->adjoint : 0; }
 
 Status Plan::set_memory_budget(double bytes) {
   if (!d_) return Status::Err(FWIFLOW_ERR_ARG, "plan not created");
@@ -2179,6 +2415,8 @@ class Runner {
   void fwd_vel(const StepLaunch& s, int it, int U);
   void bwd_a(const StepLaunch& s, int it, int U);
   void bwd_b(const StepLaunch& s, int it, int U);
+  void exact_a(const Part& P, int first);
+  void exact_b(const Part& P, int it);
   Status snapshot(const GroupParts& gp, int sg0, int j, bool save);
   // run `body` (launches only, no memsets) as a graph captured on first use
   Status replay(const std::string& key, const std::function<void()>& body);
@@ -2276,6 +2514,20 @@ void Runner::bwd_b(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_b); }
 #undef STEP_LAUNCH
 #undef STEP_LAUNCH_ONE
 
+// exact-adjoint launches: 64 x 4 workgroups over the update region
+#define EXACT_LAUNCH(K, arg)                                                                \
+  do {                                                                                      \
+    const dim3 grid((unsigned)((P.g.nzA + kBZ - 1) / kBZ), (unsigned)((P.g.nx + 3) / 4));    \
+    const dim3 b4(kBZ, 4);                                                                  \
+    if (d0->kernel_start)                                                                   \
+      hipExtLaunchKernelGGL(K, grid, b4, 0, st, d0->kernel_start, d0->kernel_end, 0, P, arg); \
+    else                                                                                    \
+      hipLaunchKernelGGL(K, grid, b4, 0, st, P, arg);                                       \
+  } while (0)
+void Runner::exact_a(const Part& P, int first) { EXACT_LAUNCH(k_exact_a, first); }
+void Runner::exact_b(const Part& P, int it) { EXACT_LAUNCH(k_exact_b, it); }
+#undef EXACT_LAUNCH
+
 GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
   GroupParts gp;
   gp.key = std::to_string(rr.r0) + ":" + std::to_string(rr.r1);
@@ -2308,6 +2560,9 @@ GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
     P.nrec = d.roff[s1] - d.roff[s0];
     P.gzU = d.gzU();
     P.gzR = d.gzR();
+    P.exact = d.adjoint;
+    P.xt = d.xt;
+    P.srcwin = d.srcwin;
     gp.parts.push_back(P);
     gp.member.push_back((int)mi);
     gp.key += "|" + std::to_string(mi) + ":" + std::to_string(s0) + "-" + std::to_string(s1);
@@ -2425,6 +2680,11 @@ Status Runner::run(int calc_id) {
       return Status::Err(FWIFLOW_ERR_ARG, "plans of a job must share nSteps");
     if (m.d->tile_x != d0->tile_x)
       return Status::Err(FWIFLOW_ERR_ARG, "plans of a job must share the tile width");
+    if (m.d->adjoint && M.size() > 1)
+      return Status::Err(FWIFLOW_ERR_UNSUPPORTED, "the exact adjoint runs one plan at a time");
+    if (m.d->adjoint && calc_id == 1 && m.plan->para.if_src_update)
+      return Status::Err(FWIFLOW_ERR_UNSUPPORTED,
+                         "the exact adjoint does not take the source update (if_src_update)");
     m.d->illum_valid = false;
     m.plan->last_calc = -1;
   }
@@ -2531,6 +2791,15 @@ Status Runner::run(int calc_id) {
       if (!d.acc) TRY(d.alloc(&d.acc, 5 * N));
       if (!d.grad) TRY(d.alloc(&d.grad, 3 * N));
       if (!d.stf_grad) TRY(d.alloc(&d.stf_grad, (size_t)d.g.ns * nSteps));
+      if (d.adjoint) {
+        const size_t xf = (size_t)std::min(gb, d.g.ns) * 8 * N;
+        if (d.xt_floats < xf) {
+          d.release(d.xt, d.xt_floats);
+          d.xt_floats = 0;
+          TRY(d.alloc(&d.xt, xf));
+          d.xt_floats = xf;
+        }
+      }
       const int rows_m = std::min(S, d.g.ns);
       const size_t rf = (size_t)rows_m * (ckpt ? K : steps) * d.ring_step_floats();
       if (d.ring_floats < rf || d.ring_floats > 2 * rf + (1u << 20)) {
@@ -2681,6 +2950,34 @@ Status Runner::run(int calc_id) {
                                d.res + (size_t)t * d.R);
         }
       };
+      // exact adjoint, reverse steps t1-1 .. t0 of a segment: the forward
+      // state enters as (v, sigma) after step t1-1; at reverse step it it
+      // holds (v^it, sigma^{it+1}).  No initial adjoint step: sample 0 is
+      // not recorded, and sample it+1 enters at reverse step it.
+      const Part* xp = gp.parts.empty() ? nullptr : &gp.parts[0];
+      auto xsegment = [&](int t0, int t1) {
+        bwd_b(sRV, t1, Ub);  // rev_vel(t1-1)
+        for (int it = t1 - 1; it >= t0; it--) {
+          const bool smp = it % sample_every == 0;
+          d0->launch("exact_a", smp, [&] { exact_a(*xp, it == nSteps - 2 ? 1 : 0); });
+          inject(it + 1);
+          d0->launch("exact_b", smp, [&] { exact_b(*xp, it); });
+          d0->launch("rev_stress", smp, [&] { bwd_a(sR, it, Ub); });
+          if (it > t0) d0->launch("rev_vel", smp, [&] { bwd_b(sRV, it, Ub); });
+        }
+      };
+      if (d0->adjoint) {
+        for (size_t p = 0; p < gp.parts.size(); p++) {
+          DeviceState& d = *M[gp.member[p]].d;
+          HIP_TRY(hipMemsetAsync(d.xt, 0, (size_t)gp.parts[p].g.ns * 8 * d.g.N * sizeof(float), st));
+        }
+      }
+      if (d0->adjoint && !ckpt) {
+        if (nSteps >= 2)
+          TRY(replay(graph_key("X", gp, {&mR, &mRV}), [&] { xsegment(0, steps); }));
+        HIP_TRY(hipGetLastError());
+        continue;
+      }
       // reverse steps t1-1 .. t0 of one segment; `last` is the segment that
       // ends at nSteps-1
       auto segment = [&](int t0, int t1, bool last) {
@@ -2723,11 +3020,15 @@ Status Runner::run(int calc_id) {
       }
       // checkpoints: each segment is recomputed from its checkpoint, then
       // reversed with its ring
-      bwd_a(sAV, nSteps - 1, Ub);
-      inject(nSteps - 1);
-      if (nSteps < 2) {
-        bwd_b(sAS, nSteps - 1, Ub);
-        continue;
+      if (d0->adjoint) {
+        if (nSteps < 2) continue;
+      } else {
+        bwd_a(sAV, nSteps - 1, Ub);
+        inject(nSteps - 1);
+        if (nSteps < 2) {
+          bwd_b(sAS, nSteps - 1, Ub);
+          continue;
+        }
       }
       const int nseg = (steps + K - 1) / K;
       for (int jj = nseg - 1; jj >= 0; jj--) {
@@ -2745,7 +3046,10 @@ Status Runner::run(int calc_id) {
           fwd_stress(sRS, it, Uf);
           fwd_vel(sRW, it, Uf);
         }
-        segment(t0, t1, jj == nseg - 1);
+        if (d0->adjoint)
+          xsegment(t0, t1);
+        else
+          segment(t0, t1, jj == nseg - 1);
       }
       HIP_TRY(hipGetLastError());
     }
@@ -2763,7 +3067,7 @@ Status Runner::run(int calc_id) {
     }
     if (withAdj)
       hipLaunchKernelGGL(k_finalize, dim3((d.g.nz + 255) / 256, d.g.nx), dim3(256), 0, st, d.g,
-                         d.mdl(), d.acc, d.grad);
+                         d.mdl(), d.acc, d.grad, d.adjoint);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
@@ -2849,6 +3153,9 @@ Status Job::run(int calc_id) {
                           : group(sum, rows);
     fuse = fused >= solo ? 1 : 0;
   }
+  // the exact adjoint runs one plan at a time
+  for (Plan* p : plans_)
+    if (p->d_->adjoint) fuse = 0;
   last_fused_ = fuse != 0 || plans_.size() == 1;
   if (!last_fused_) {
     for (Plan* p : plans_) TRY(p->run(calc_id));

@@ -38,6 +38,8 @@ class Plan {
   // when the sweep's ring fits the memory budget, else checkpoints at the
   // interval that minimises memory); k > 0 checkpoints every k steps
   Status set_checkpoint(int k);
+  Status set_adjoint(int mode);  // 0: the reference's adjoint, 1: the exact adjoint
+  int adjoint() const;
   Status set_memory_budget(double bytes);  // ring + checkpoints of a gradient run
   // launch shape: tile width in columns (1-4), shots per forward / backward
   // launch (0: all, -1: automatic); replaces the environment defaults

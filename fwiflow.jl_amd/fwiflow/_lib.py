@@ -31,7 +31,8 @@ EXPORTS = [
     "fwiflow_plan_load_obs", "fwiflow_plan_run", "fwiflow_plan_misfit",
     "fwiflow_plan_gradients", "fwiflow_plan_data", "fwiflow_plan_write_data",
     "fwiflow_plan_residual", "fwiflow_plan_source",
-    "fwiflow_plan_dims", "fwiflow_plan_set_checkpoint", "fwiflow_plan_set_timing",
+    "fwiflow_plan_dims", "fwiflow_plan_set_checkpoint", "fwiflow_plan_set_adjoint",
+    "fwiflow_plan_set_timing",
     "fwiflow_plan_timing", "fwiflow_plan_device_bytes", "fwiflow_plan_shots_per_launch",
     "fwiflow_plan_set_illumination", "fwiflow_plan_illumination",
     "fwiflow_plan_launch_groups", "fwiflow_plan_set_memory_budget", "fwiflow_plan_memory_plan",
@@ -89,6 +90,7 @@ def load():
     L.fwiflow_plan_source.argtypes = [vp, fp]
     L.fwiflow_plan_dims.argtypes = [vp, ip]
     L.fwiflow_plan_set_checkpoint.argtypes = [vp, C.c_int]
+    L.fwiflow_plan_set_adjoint.argtypes = [vp, C.c_int]
     L.fwiflow_plan_set_timing.argtypes = [vp, C.c_int]
     L.fwiflow_plan_timing.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_longlong)]
     L.fwiflow_plan_device_bytes.argtypes = [vp, C.POINTER(C.c_longlong)]

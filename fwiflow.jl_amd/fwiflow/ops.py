@@ -184,6 +184,14 @@ class Plan:
     def set_checkpoint(self, k):
         check(self._L.fwiflow_plan_set_checkpoint(self._h, int(k)))
 
+    def set_adjoint(self, mode):
+        """'reference' (default) or 'exact': the transpose of the forward
+        step, whose gradients are the derivatives of the discrete misfit."""
+        modes = {"reference": 0, "exact": 1}
+        if mode not in modes:
+            raise ValueError(f"adjoint mode {mode!r}: expected one of {sorted(modes)}")
+        check(self._L.fwiflow_plan_set_adjoint(self._h, modes[mode]))
+
     def set_timing(self, on=True):
         check(self._L.fwiflow_plan_set_timing(self._h, 1 if on else 0))
 

@@ -177,6 +177,18 @@ int fwiflow_plan_dims(fwiflow_plan* plan, int* dims);
  * instead of the reversed one.  No reference counterpart (the reference's
  * scheme is the ring of every step for one shot at a time). */
 int fwiflow_plan_set_checkpoint(fwiflow_plan* plan, int ckpt_every);
+/* Adjoint of the plan's gradient runs.  mode 0 (default): the reference's
+ * (el_velocity_adj.cu, el_stress_adj.cu and the sprays of el_stress.cu /
+ * el_velocity.cu, as cufd computes them).  mode 1: the exact adjoint, the
+ * transpose of the forward time step, whose gradients (lambda, mu, rho and
+ * stf) are the derivatives of the discrete misfit also where waves enter the
+ * C-PML and in zero-shear layers; the stf gradient then includes the source
+ * scale 1500^2 dt and window of add_source.  No counterpart in the reference.
+ * Mode 1 runs one plan at a time (a job runs its plans one after another),
+ * refuses if_src_update, and holds 8 more arrays per backward-launch shot.
+ * FWIFLOW_EXACT_ADJOINT=1 makes it the default of new plans (and of
+ * fwiflow_cufd). */
+int fwiflow_plan_set_adjoint(fwiflow_plan* plan, int mode);
 /* Device bytes a gradient run may hold for the boundary ring and the
  * checkpoints (default 32e9).  It sets the shots per sweep. */
 int fwiflow_plan_set_memory_budget(fwiflow_plan* plan, double bytes);

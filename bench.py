@@ -39,6 +39,11 @@ KERNEL_BYTES = {
     "rev_stress": 44.0,   # read 2 velocity, 3 stress, 3 adjoint stress, write 3 stress
     "adj_vel": 28.0,      # read 3 adjoint stress, 2 adjoint velocity, write 2
     "adj_stress": 32.0,   # read 2 adjoint velocity, 3 adjoint stress, write 3
+    # exact adjoint (--adjoint exact), each with the reverse step of its field
+    "exact_a": 76.0,      # read 2 adjoint velocity, 4 stress operands, 3 stress, 2 velocity;
+                          # write 2 adjoint velocity, 4 velocity operands, 2 velocity
+    "exact_b": 88.0,      # read 3 adjoint stress, 4 velocity operands, 2 velocity, 3 stress;
+                          # write 3 adjoint stress, 4 stress operands, 3 stress
 }
 # kernels that run two of the above as one grid (disjoint workgroup ranges)
 KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_stress", "rev_vel")}
@@ -50,9 +55,11 @@ KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_str
 MEASURED_TRAFFIC = {("bwd_pair_a", 2.91): 231.5e6, ("bwd_pair_b", 2.91): 226.9e6,
                     ("stress_fwd", 4.92): 204.4e6, ("vel_fwd", 4.92): 180.9e6}
 # kernels of the backward sweep (their launches cover the backward shot groups)
-BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "bwd_pair_a", "bwd_pair_b")
+BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "bwd_pair_a", "bwd_pair_b",
+            "exact_a", "exact_b")
 # C-PML memory-variable traffic per PML cell-direction (2 variables RW = 16 B)
-KERNEL_PML = {"stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0}
+KERNEL_PML = {"stress_fwd": 16.0, "vel_fwd": 16.0, "adj_vel": 16.0, "adj_stress": 16.0,
+              "exact_a": 16.0, "exact_b": 16.0}
 
 
 def parse():
@@ -72,6 +79,9 @@ def parse():
                    help="c3: the 500x1700 benchmark (default); timelapse: the six config-4 "
                         "surveys (200x600, 300x900, 450x1350, baseline and monitor, 16 shots "
                         "each, nt 3000) as one job")
+    p.add_argument("--adjoint", choices=["reference", "exact"], default="reference",
+                   help="the reference's adjoint (default, the benchmark) or the exact one "
+                        "(the transpose of the forward step)")
     p.add_argument("--abi", action="store_true",
                    help="also time one end-to-end fwiflow_cufd gradient call (files, set-up)")
     p.add_argument("--autotune", action="store_true",
@@ -303,6 +313,7 @@ def main():
     ids = list(range(a.nshots))
     log(f"bench: grid {c.NZ}x{c.NX} nt {c.nSteps} shots {a.nshots}")
     P = fwiflow.Plan(para, ids)
+    P.set_adjoint(a.adjoint)
     if a.checkpoint:
         P.set_checkpoint(a.checkpoint)
     if a.memory_budget:
@@ -384,6 +395,7 @@ def main():
             "grid": [c.NZ, c.NX],
             "nt": c.nSteps,
             "shots": a.nshots,
+            "adjoint": a.adjoint,
         },
         "gcell_updates_per_s": round(g["active"] * (c.nSteps - 1) * value / 1e9, 3),
         "roofline": {

@@ -66,11 +66,17 @@ def parse():
     p.add_argument("--save-net", default=None, help="write the trained parameters (.npy)")
     p.add_argument("--load-net", default=None,
                    help="skip training and evaluate the parameters in this .npy file")
+    p.add_argument("--init-net", default=None,
+                   help="start training from these parameters (.npy) instead of a fresh "
+                        "network: training in stages over several runs")
     p.add_argument("--train-only", action="store_true", help="stop after training")
+    p.add_argument("--adjoint", choices=["reference", "exact"], default="reference",
+                   help="gradient of every objective: the reference's adjoint or the exact "
+                        "one (the transpose of the forward step)")
     return p.parse_args()
 
 
-def setup(seed, nt, ws):
+def setup(seed, nt, ws, adjoint="reference"):
     import fwiflow
     from fwiflow import models
 
@@ -79,7 +85,7 @@ def setup(seed, nt, ws):
     fwi = fwiflow.make_fwi(c.nz, c.nx, c.dz, c.dx, nt, c.dt, c.src_z, c.src_x, c.rec_z, c.rec_x,
                            nPml=c.nPml, f0=c.f0, WORKSPACE=ws)
     fwiflow.compute_observation(fwi, c.cp, c.cs, c.rho, stf)
-    obj = fwiflow.PlanObjective(fwi, c.cs, c.rho, stf, cp_ref=c.cp0)
+    obj = fwiflow.PlanObjective(fwi, c.cs, c.rho, stf, cp_ref=c.cp0, adjoint=adjoint)
     x0 = fwiflow.padding(c.cp0, fwi.nPml, fwi.nPad)
     return fwi, obj, x0
 
@@ -97,7 +103,7 @@ def main():
     for k in range(train_models):
         seed = a.seed0 + k
         ws = tempfile.mkdtemp(prefix=f"fwiflow_pc_{seed}_")
-        fwi, obj, x0 = setup(seed, a.nt, ws)
+        fwi, obj, x0 = setup(seed, a.nt, ws, a.adjoint)
         mask = fwi.mask
 
         def on_pair(g, extras, update, J):
@@ -126,6 +132,9 @@ def main():
         hist = [float("nan")]
         log(f"loaded network parameters from {a.load_net}")
     else:
+        if a.init_net:
+            net.set_params(np.load(a.init_net).astype(np.float32))
+            log(f"training continues from {a.init_net}")
         hist = precond.train(net, samples, mask.astype(np.float32), epochs=a.epochs, lr=a.lr,
                              log=log)
         log(f"trained on {len(samples)} pairs in {time.perf_counter() - t0:.1f} s, "
@@ -145,7 +154,7 @@ def main():
     for k in range(a.eval_models):
         seed = a.seed0 + 1000 + k
         ws = tempfile.mkdtemp(prefix=f"fwiflow_pc_{seed}_")
-        fwi, obj, x0 = setup(seed, a.nt, ws)
+        fwi, obj, x0 = setup(seed, a.nt, ws, a.adjoint)
         J0 = obj(x0)[0]
         rp = inversion.lbfgs(obj, x0, maxiter=a.maxiter)
         log(f"eval {seed}: plain {rp['misfit']:.4g} in {rp['seconds']:.1f} s, "
@@ -192,7 +201,8 @@ def main():
         "control_value": (lambda v: v if np.isfinite(v) else None)(float(np.median(
             [e["control_ratio"] if e["control_ratio"] is not None else np.inf
              for e in evals]))) if evals else None,
-        "config": {"driver": a.driver, "teacher": a.teacher, "grid": [int(H), int(W)],
+        "config": {"driver": a.driver, "teacher": a.teacher, "adjoint": a.adjoint,
+                   "grid": [int(H), int(W)],
                    "nt": a.nt, "shots": 30,
                    "train_models":
                    a.train_models, "eval_models": a.eval_models, "maxiter": a.maxiter,

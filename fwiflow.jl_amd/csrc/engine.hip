@@ -207,9 +207,7 @@ struct Part {
   int nrec;         // receivers of the part's shots
   int gzU;          // tiles along z of the update-region tiling
   int gzR;          // tiles along z of the ring-rectangle tiling
-  // exact adjoint (the transpose of the forward step; no reference
-  // counterpart): the reversal kernels then keep no gradient sums
-  int exact;
+  // exact adjoint (the transpose of the forward step; no reference counterpart)
   float* xt;             // transposed-stencil operands [group shot][8][cell]
   const float* srcwin;   // d(source amplitude)/d(stf) of each step
 };
@@ -599,7 +597,6 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
   const float ba = P.m.ba[i], bb = P.m.bb[i];
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
   const int ns = g.ns;
-  const bool sums = !P.exact;
   float ga = 0.f, gb = 0.f;
   for (int s0 = 0; s0 < ns; s0 += U) {
     RevVelIn r[U];
@@ -615,7 +612,7 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
         q.sxzm2 = sxc[0], q.sxzm1 = sxc[1], q.sxz0 = sxc[2], q.sxzp1 = sxc[3];
         q.sxzmx = lx(W, sw, kSxz, C, -1), q.sxzpx = lx(W, sw, kSxz, C, 1), q.sxzm2x = lx(W, sw, kSxz, C, -2);
         q.sxxpx = lx(W, sw, kSxx, C, 1), q.sxx0 = lz(W, sw, kSxx, C, 0), q.sxxp2x = lx(W, sw, kSxx, C, 2), q.sxxmx = lx(W, sw, kSxx, C, -1);
-        if (sums) q.avz = lz(A, sw, kVz, C, 0), q.avx = lz(A, sw, kVx, C, 0);
+        q.avz = lz(A, sw, kVz, C, 0), q.avx = lz(A, sw, kVx, C, 0);
       }
       if (slot < 0) {
         q.vz = lz(W, sw, kVz, C, 0), q.vx = lz(W, sw, kVx, C, 0);
@@ -637,10 +634,8 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
         const float dsxz_dz = dfw(q.sxz0, q.sxzm1, q.sxzp1, q.sxzm2, dz, rdz);
         const float dsxx_dx = dfw(q.sxxpx, q.sxx0, q.sxxp2x, q.sxxmx, dx, rdx);
         const float fz = dszz_dz + dsxz_dx, fx = dsxz_dz + dsxx_dx;
-        if (sums) {
-          ga += q.avz * fz;
-          gb += q.avx * fx;
-        }
+        ga += q.avz * fz;
+        gb += q.avx * fx;
         if (slot < 0) {
           vz = vz - fz * ba * dt;
           vx = vx - fx * bb * dt;
@@ -650,7 +645,7 @@ __device__ __forceinline__ void rev_vel(const Part& P, int tile, int it) {
       st(W, sw, kVx, C, vx);
     }
   }
-  if (in && sums) {
+  if (in) {
     P.acc[3 * g.N + i] += ga;
     P.acc[4 * g.N + i] += gb;
   }
@@ -688,7 +683,6 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
   const double dt64 = (double)dt;
   const int ns = g.ns;
-  const bool sums = !P.exact;
   float gl = 0.f, gm = 0.f, gs = 0.f;
   for (int s0 = 0; s0 < ns; s0 += U) {
     RevStressIn r[U];
@@ -704,8 +698,7 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
         q.vzpx = lx(W, sw, kVz, C, 1), q.vzp2x = lx(W, sw, kVz, C, 2), q.vzmx = lx(W, sw, kVz, C, -1);
         q.vxm1 = vxc[0], q.vx0 = vxc[1], q.vxp1 = vxc[2], q.vxp2 = vxc[3];
         q.vxmx = lx(W, sw, kVx, C, -1), q.vxpx = lx(W, sw, kVx, C, 1), q.vxm2x = lx(W, sw, kVx, C, -2);
-        if (sums)
-          q.aszz = lz(A, sw, kSzz, C, 0), q.asxx = lz(A, sw, kSxx, C, 0), q.asxz = lz(A, sw, kSxz, C, 0);
+        q.aszz = lz(A, sw, kSzz, C, 0), q.asxx = lz(A, sw, kSxx, C, 0), q.asxz = lz(A, sw, kSxz, C, 0);
       }
       if (slot < 0) {
         q.szz = lz(W, sw, kSzz, C, 0), q.sxx = lz(W, sw, kSxx, C, 0), q.sxz = lz(W, sw, kSxz, C, 0);
@@ -728,11 +721,9 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
         const float dvx_dx = dfw(q.vx0, q.vxmx, q.vxpx, q.vxm2x, dx, rdx);
         const float dvx_dz = dfw(q.vxp1, q.vx0, q.vxp2, q.vxm1, dz, rdz);
         const float dvz_dx = dfw(q.vzpx, q.vz0, q.vzp2x, q.vzmx, dx, rdx);
-        if (sums) {
-          gl += (q.aszz + q.asxx) * (dvz_dz + dvx_dx);
-          gm += q.aszz * dvz_dz + q.asxx * dvx_dx;
-          gs += q.asxz * (dvx_dz + dvz_dx);
-        }
+        gl += (q.aszz + q.asxx) * (dvz_dz + dvx_dx);
+        gm += q.aszz * dvz_dz + q.asxx * dvx_dx;
+        gs += q.asxz * (dvx_dz + dvz_dx);
         if (slot < 0) {
           if (z == q.sz && x == q.sx) {
             const float amp = P.src.amp[(size_t)s * g.nSteps + it];
@@ -749,7 +740,7 @@ __device__ __forceinline__ void rev_stress(const Part& P, int tile, int it) {
       st(W, sw, kSxz, C, sxz);
     }
   }
-  if (in && sums) {
+  if (in) {
     P.acc[i] += gl;
     P.acc[g.N + i] += gm;
     P.acc[2 * g.N + i] += gs;
@@ -1016,127 +1007,199 @@ __device__ __forceinline__ void adj_stress(const Part& P, int tile, int it) {
 // update region: the first forms the weighted operands of every cell
 // (xt, zero outside the update region), the second applies the transposed
 // stencils to them.  The second pass of one update and the first of the next
-// touch only their own cell's adjoint, so they share a launch: k_exact_a
-// applies the transposed stress update of step it+1 to v_adj and forms the
-// velocity operands of step it; k_exact_b applies those to sigma_adj (with
-// the residual of sample it+1) and forms the stress operands of step it.
-// xt holds two operand sets per shot (slots 0-3: stress update, 4-7:
-// velocity update).
-__device__ __forceinline__ float pml_t(bool gated, float g, float K, float a, float b, float& P) {
-  if (!gated) return g;
+// touch only their own cell's adjoint, so they share a launch, and so does
+// the reverse re-materialisation step that reads the same forward field:
+//   k_exact_a(it): reverse velocity step it (v^{it+1} -> v^it from
+//     sigma^{it+1}), transposed stress update of step it+1 into v_adj,
+//     velocity operands of step it and the density sums from sigma^{it+1};
+//   k_exact_b(it): residual of sample it+1 (folded injection), transposed
+//     velocity update of step it into sigma_adj, stf gradient, stress
+//     operands of step it and the lambda / mu sums from v^it, reverse stress
+//     step it (sigma^{it+1} -> sigma^it from v^it).
+// Each reversal writes only its cell's own forward field, which the adjoint
+// half of its launch does not read.  xt holds two operand sets per shot
+// (slots 0-3: Azz, Axx, Axz, Azx of the stress update; 4-7: Bz, Bzx, Bxz, Bx
+// of the velocity update).
+__device__ __forceinline__ float pml_t(float g, const Kdiv& K, float a, float b, float& P) {
   const float tot = g + P;
   P = b * tot;
-  return g / K + a * tot;
+  return kdiv(g, K) + a * tot;
 }
-// (D+)^T and (D-)^T along a stride st (1: z, nz: x), for
+// The transposed differences are written through dfw: for a stride st,
+// (D+)^T B at j = (c1 (B[j-st] - B[j]) - c2 (B[j-2st] - B[j+st])) / h and
+// (D-)^T B at j = (c1 (B[j] - B[j+st]) - c2 (B[j-st] - B[j+2st])) / h, for
 // D+ f = (c1 (f[k+1] - f[k]) - c2 (f[k+2] - f[k-1])) / h and
-// D- f = (c1 (f[k] - f[k-1]) - c2 (f[k+1] - f[k-2])) / h
-__device__ __forceinline__ float dpt(const float* B, long long j, long long st, float h) {
-  return (kC1 * (B[j - st] - B[j]) - kC2 * (B[j - 2 * st] - B[j + st])) / h;
-}
-__device__ __forceinline__ float dmt(const float* B, long long j, long long st, float h) {
-  return (kC1 * (B[j] - B[j + st]) - kC2 * (B[j - st] - B[j + 2 * st])) / h;
-}
+// D- f = (c1 (f[k] - f[k-1]) - c2 (f[k+1] - f[k-2])) / h.
 
-// Transposed stress update of step it+1 (unless `first`), then the velocity
-// operands of step it: B = C-PML-adjoint(dt b v_adj) per derivative, and the
-// density sums acc_a += vz_adj (dszz/dz + dsxz/dx), acc_b += vx_adj (dsxz/dz +
-// dsxx/dx) from the forward stresses sigma^{it+1} held in w.
-__global__ __launch_bounds__(256) void k_exact_a(Part P, int first) {
+__global__ __launch_bounds__(256) void k_exact_a(Part P, int it, int first, int rev) {
   const Geo& g = P.g;
   const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int x = wave_col((int)(blockIdx.y * blockDim.y));
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
-  const long long N = g.N, i = (long long)x * nz + z;
+  const long long i = (long long)x * nz + z;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
+  const int slot = rev ? ring_slot(z, x, g) : -1;
+  const bool re = rev && (in || slot >= 0);  // the cell takes part in the reversal
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
-  const bool pxv = x < g.nPml || x > g.nx - g.nPml;  // el_velocity's gate
+  const bool px = x < g.nPml || x > g.nx - g.nPml;  // el_velocity's gate
+  const Cell C = cell_of((unsigned)i * 4u, nz);
+  const int Nb = (int)(g.N * 4);
+  const Arr W = arr(P.w, Nb), A = arr(P.a, Nb), M = arr(P.p, Nb), X = arr(P.xt, Nb);
   const float ba = P.m.ba[i], bb = P.m.bb[i];
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  Kdiv kzh{}, kz{}, kx{}, kxh{};
+  float azh = 0.f, bzh = 0.f, az = 0.f, bz = 0.f, ax = 0.f, bx = 0.f, axh = 0.f, bxh = 0.f;
+  if (pz) {
+    kzh = kdiv_of(P.c.Kzh[z]), azh = P.c.azh[z], bzh = P.c.bzh[z];
+    kz = kdiv_of(P.c.Kz[z]), az = P.c.az[z], bz = P.c.bz[z];
+  }
+  if (px) {
+    kx = kdiv_of(P.c.Kx[x]), ax = P.c.ax[x], bx = P.c.bx[x];
+    kxh = kdiv_of(P.c.Kxh[x]), axh = P.c.axh[x], bxh = P.c.bxh[x];
+  }
   float ga = 0.f, gb = 0.f;
   for (int s = 0; s < g.ns; ++s) {
-    float* a = P.a + (size_t)s * kWaveComps * N;
-    float* m = P.p + (size_t)s * kMemComps * N;
-    float* xs = P.xt + (size_t)s * 8 * N;  // stress operands Azz, Axx, Axz, Azx
-    float* xv = xs + 4 * N;                // velocity operands Bz, Bzx, Bxz, Bx
-    float vz = a[kVz * N + i], vx = a[kVx * N + i];
+    const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb, sx = s * 8 * Nb;
+    // forward stresses sigma^{it+1} around the cell (interior cells)
+    float fz = 0.f, fx = 0.f;
+    if (in) {
+      const f4 szc = lz4(W, sw, kSzz, C, -1), sxc = lz4(W, sw, kSxz, C, -2);
+      const float sxzmx = lx(W, sw, kSxz, C, -1), sxzpx = lx(W, sw, kSxz, C, 1),
+                  sxzm2x = lx(W, sw, kSxz, C, -2);
+      const float sxxpx = lx(W, sw, kSxx, C, 1), sxx0 = lz(W, sw, kSxx, C, 0),
+                  sxxp2x = lx(W, sw, kSxx, C, 2), sxxmx = lx(W, sw, kSxx, C, -1);
+      const float dszz_dz = dfw(szc[2], szc[1], szc[3], szc[0], dz, rdz);
+      const float dsxz_dx = dfw(sxc[2], sxzmx, sxzpx, sxzm2x, dx, rdx);
+      const float dsxz_dz = dfw(sxc[2], sxc[1], sxc[3], sxc[0], dz, rdz);
+      const float dsxx_dx = dfw(sxxpx, sxx0, sxxp2x, sxxmx, dx, rdx);
+      fz = dszz_dz + dsxz_dx;
+      fx = dsxz_dz + dsxx_dx;
+    }
+    float wvz = 0.f, wvx = 0.f;
+    if (re) {
+      if (slot < 0) {
+        wvz = lz(W, sw, kVz, C, 0), wvx = lz(W, sw, kVx, C, 0);
+      } else {
+        wvz = *ring_ptr(P.ring, g, it, 3, s, slot);
+        wvx = *ring_ptr(P.ring, g, it, 4, s, slot);
+      }
+    }
+    float vz = lz(A, sw, kVz, C, 0), vx = lz(A, sw, kVx, C, 0);
     if (!first) {
-      vz = vz + (dmt(xs, i, 1, dz) + dpt(xs + 3 * N, i, nz, dx));
-      vx = vx + (dmt(xs + N, i, nz, dx) + dpt(xs + 2 * N, i, 1, dz));
-      a[kVz * N + i] = vz;
-      a[kVx * N + i] = vx;
+      const f4 azz = lz4(X, sx, 0, C, -1);  // Azz at z-1 .. z+2
+      const f4 axz = lz4(X, sx, 2, C, -2);  // Axz at z-2 .. z+1
+      const float azxm = lx(X, sx, 3, C, -1), azx0 = lz(X, sx, 3, C, 0),
+                  azxm2 = lx(X, sx, 3, C, -2), azxp = lx(X, sx, 3, C, 1);
+      const float axx0 = lz(X, sx, 1, C, 0), axxp = lx(X, sx, 1, C, 1),
+                  axxm = lx(X, sx, 1, C, -1), axxp2 = lx(X, sx, 1, C, 2);
+      // (D-)^T Azz along z + (D+)^T Azx along x; (D-)^T Axx along x + (D+)^T Axz along z
+      vz = vz + (dfw(azz[1], azz[2], azz[0], azz[3], dz, rdz) + dfw(azxm, azx0, azxm2, azxp, dx, rdx));
+      vx = vx + (dfw(axx0, axxp, axxm, axxp2, dx, rdx) + dfw(axz[1], axz[2], axz[0], axz[3], dz, rdz));
+      st(A, sw, kVz, C, vz);
+      st(A, sw, kVx, C, vx);
     }
     const float gz = dt * ba * vz, gx = dt * bb * vx;
+    float oz = gz, ozx = gz, oxz = gx, ox = gx;
     if (pz) {
-      float q = m[kMszz * N + i];
-      xv[i] = pml_t(true, gz, P.c.Kzh[z], P.c.azh[z], P.c.bzh[z], q);
-      m[kMszz * N + i] = q;
-      q = m[kMsxzz * N + i];
-      xv[2 * N + i] = pml_t(true, gx, P.c.Kz[z], P.c.az[z], P.c.bz[z], q);
-      m[kMsxzz * N + i] = q;
-    } else {
-      xv[i] = gz;
-      xv[2 * N + i] = gx;
+      float q = lz(M, sm, kMszz, C, 0);
+      oz = pml_t(gz, kzh, azh, bzh, q);
+      st(M, sm, kMszz, C, q);
+      q = lz(M, sm, kMsxzz, C, 0);
+      oxz = pml_t(gx, kz, az, bz, q);
+      st(M, sm, kMsxzz, C, q);
     }
-    if (pxv) {
-      float q = m[kMsxzx * N + i];
-      xv[N + i] = pml_t(true, gz, P.c.Kx[x], P.c.ax[x], P.c.bx[x], q);
-      m[kMsxzx * N + i] = q;
-      q = m[kMsxx * N + i];
-      xv[3 * N + i] = pml_t(true, gx, P.c.Kxh[x], P.c.axh[x], P.c.bxh[x], q);
-      m[kMsxx * N + i] = q;
-    } else {
-      xv[N + i] = gz;
-      xv[3 * N + i] = gx;
+    if (px) {
+      float q = lz(M, sm, kMsxzx, C, 0);
+      ozx = pml_t(gz, kx, ax, bx, q);
+      st(M, sm, kMsxzx, C, q);
+      q = lz(M, sm, kMsxx, C, 0);
+      ox = pml_t(gx, kxh, axh, bxh, q);
+      st(M, sm, kMsxx, C, q);
     }
+    st(X, sx, 4, C, oz);
+    st(X, sx, 5, C, ozx);
+    st(X, sx, 6, C, oxz);
+    st(X, sx, 7, C, ox);
     if (in) {
-      const float* w = P.w + (size_t)s * kWaveComps * N;
-      const float* szz = w + kSzz * N;
-      const float* sxx = w + kSxx * N;
-      const float* sxz = w + kSxz * N;
-      const float dszz_dz = dfw(szz[i + 1], szz[i], szz[i + 2], szz[i - 1], dz, rdz);
-      const float dsxz_dx = dfw(sxz[i], sxz[i - nz], sxz[i + nz], sxz[i - 2 * nz], dx, rdx);
-      const float dsxz_dz = dfw(sxz[i], sxz[i - 1], sxz[i + 1], sxz[i - 2], dz, rdz);
-      const float dsxx_dx = dfw(sxx[i + nz], sxx[i], sxx[i + 2 * nz], sxx[i - nz], dx, rdx);
-      ga += vz * (dszz_dz + dsxz_dx);
-      gb += vx * (dsxz_dz + dsxx_dx);
+      ga += vz * fz;
+      gb += vx * fx;
+    }
+    if (re) {
+      if (slot < 0) {  // el_velocity.cu:84-117, isFor = false
+        wvz = wvz - fz * ba * dt;
+        wvx = wvx - fx * bb * dt;
+      }
+      st(W, sw, kVz, C, wvz);
+      st(W, sw, kVx, C, wvx);
     }
   }
   if (in) {
-    P.acc[3 * N + i] += ga;
-    P.acc[4 * N + i] += gb;
+    P.acc[3 * g.N + i] += ga;
+    P.acc[4 * g.N + i] += gb;
   }
 }
 
-// Residual of sample it+1 (surveys with the folded injection) and the
-// transposed velocity update of step it into sigma_adj; the stf gradient of
-// step it; the stress operands of step it: A = C-PML-adjoint(dt C sigma_adj)
-// per derivative; and the sums acc_l += (szz_adj + sxx_adj)(dvz/dz + dvx/dx),
-// acc_m += szz_adj dvz/dz + sxx_adj dvx/dx, acc_s += sxz_adj (dvx/dz + dvz/dx)
-// from the forward velocities v^it held in w.
-__global__ __launch_bounds__(256) void k_exact_b(Part P, int it) {
+__global__ __launch_bounds__(256) void k_exact_b(Part P, int it, int rev) {
   const Geo& g = P.g;
   const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int x = wave_col((int)(blockIdx.y * blockDim.y));
   if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
   const int nz = g.nz;
-  const long long N = g.N, i = (long long)x * nz + z;
+  const long long i = (long long)x * nz + z;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
+  const int slot = rev ? ring_slot(z, x, g) : -1;
+  const bool re = rev && (in || slot >= 0);
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
-  const bool pxs = x < g.nPml || x > g.nx - g.nPml - 1;
+  const bool px = x < g.nPml || x > g.nx - g.nPml - 1;  // el_stress's gate
+  const Cell C = cell_of((unsigned)i * 4u, nz);
+  const int Nb = (int)(g.N * 4);
+  const Arr W = arr(P.w, Nb), A = arr(P.a, Nb), M = arr(P.p, Nb), X = arr(P.xt, Nb);
   const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i];
-  const float l2m = (float)((double)lam + 2.0 * (double)mu);
+  const double l2m64 = (double)lam + 2.0 * (double)mu;
+  const float l2m = (float)l2m64;
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
+  const double dt64 = (double)dt;
+  Kdiv kz{}, kzh{}, kx{}, kxh{};
+  float az = 0.f, bz = 0.f, azh = 0.f, bzh = 0.f, ax = 0.f, bx = 0.f, axh = 0.f, bxh = 0.f;
+  if (pz) {
+    kz = kdiv_of(P.c.Kz[z]), az = P.c.az[z], bz = P.c.bz[z];
+    kzh = kdiv_of(P.c.Kzh[z]), azh = P.c.azh[z], bzh = P.c.bzh[z];
+  }
+  if (px) {
+    kx = kdiv_of(P.c.Kx[x]), ax = P.c.ax[x], bx = P.c.bx[x];
+    kxh = kdiv_of(P.c.Kxh[x]), axh = P.c.axh[x], bxh = P.c.bxh[x];
+  }
   const bool inj_row = P.inj.res && z == P.inj.z;
   const float* res_row = P.inj.res ? P.inj.res + (size_t)(it + 1) * g.R : nullptr;
   float gl = 0.f, gm = 0.f, gs = 0.f;
   for (int s = 0; s < g.ns; ++s) {
-    float* a = P.a + (size_t)s * kWaveComps * N;
-    float* m = P.p + (size_t)s * kMemComps * N;
-    float* xs = P.xt + (size_t)s * 8 * N;
-    const float* xv = xs + 4 * N;
-    float szz = a[kSzz * N + i], sxx = a[kSxx * N + i], sxz = a[kSxz * N + i];
+    const int sw = s * kWaveComps * Nb, sm = s * kMemComps * Nb, sx = s * 8 * Nb;
+    const int zs = P.src.zx[2 * s], xs = P.src.zx[2 * s + 1];
+    // forward velocities v^it around the cell (interior cells)
+    float dvz_dz = 0.f, dvx_dx = 0.f, dvx_dz = 0.f, dvz_dx = 0.f;
+    if (in) {
+      const f4 vzc = lz4(W, sw, kVz, C, -2), vxc = lz4(W, sw, kVx, C, -1);
+      const float vzpx = lx(W, sw, kVz, C, 1), vzp2x = lx(W, sw, kVz, C, 2),
+                  vzmx = lx(W, sw, kVz, C, -1);
+      const float vxmx = lx(W, sw, kVx, C, -1), vxpx = lx(W, sw, kVx, C, 1),
+                  vxm2x = lx(W, sw, kVx, C, -2);
+      dvz_dz = dfw(vzc[2], vzc[1], vzc[3], vzc[0], dz, rdz);
+      dvx_dx = dfw(vxc[1], vxmx, vxpx, vxm2x, dx, rdx);
+      dvx_dz = dfw(vxc[2], vxc[1], vxc[3], vxc[0], dz, rdz);
+      dvz_dx = dfw(vzpx, vzc[2], vzp2x, vzmx, dx, rdx);
+    }
+    float wszz = 0.f, wsxx = 0.f, wsxz = 0.f;
+    if (re) {
+      if (slot < 0) {
+        wszz = lz(W, sw, kSzz, C, 0), wsxx = lz(W, sw, kSxx, C, 0), wsxz = lz(W, sw, kSxz, C, 0);
+      } else {
+        wszz = *ring_ptr(P.ring, g, it, 0, s, slot);
+        wsxz = *ring_ptr(P.ring, g, it, 1, s, slot);
+        wsxx = *ring_ptr(P.ring, g, it, 2, s, slot);
+      }
+    }
+    float szz = lz(A, sw, kSzz, C, 0), sxx = lz(A, sw, kSxx, C, 0), sxz = lz(A, sw, kSxz, C, 0);
     if (inj_row) {
       const int k = P.inj.at[(size_t)(g.s0 + s) * g.nx + x];
       if (k >= 0) {
@@ -1145,58 +1208,75 @@ __global__ __launch_bounds__(256) void k_exact_b(Part P, int it) {
         sxx = (float)((double)sxx + kRS * (double)v);
       }
     }
-    szz = szz + dpt(xv, i, 1, dz);
-    sxz = sxz + (dmt(xv + N, i, nz, dx) + dmt(xv + 2 * N, i, 1, dz));
-    sxx = sxx + dpt(xv + 3 * N, i, nz, dx);
-    a[kSzz * N + i] = szz;
-    a[kSxx * N + i] = sxx;
-    a[kSxz * N + i] = sxz;
-    if (P.stf_grad && z == P.src.zx[2 * s] && x == P.src.zx[2 * s + 1])
+    {
+      const f4 bzc = lz4(X, sx, 4, C, -2);   // Bz at z-2 .. z+1
+      const f4 bxzc = lz4(X, sx, 6, C, -1);  // Bxz at z-1 .. z+2
+      const float bzx0 = lz(X, sx, 5, C, 0), bzxp = lx(X, sx, 5, C, 1),
+                  bzxm = lx(X, sx, 5, C, -1), bzxp2 = lx(X, sx, 5, C, 2);
+      const float bxm = lx(X, sx, 7, C, -1), bx0 = lz(X, sx, 7, C, 0),
+                  bxm2 = lx(X, sx, 7, C, -2), bxp = lx(X, sx, 7, C, 1);
+      // (D+)^T Bz along z; (D-)^T Bzx along x + (D-)^T Bxz along z; (D+)^T Bx along x
+      szz = szz + dfw(bzc[1], bzc[2], bzc[0], bzc[3], dz, rdz);
+      sxz = sxz + (dfw(bzx0, bzxp, bzxm, bzxp2, dx, rdx) + dfw(bxzc[1], bxzc[2], bxzc[0], bxzc[3], dz, rdz));
+      sxx = sxx + dfw(bxm, bx0, bxm2, bxp, dx, rdx);
+    }
+    st(A, sw, kSzz, C, szz);
+    st(A, sw, kSxx, C, sxx);
+    st(A, sw, kSxz, C, sxz);
+    // d(szz, sxx)/d(stf) at the source: 1500^2 dt and the window (add_source)
+    if (P.stf_grad && z == zs && x == xs)
       P.stf_grad[(size_t)s * g.nSteps + it] =
-          (float)(-(double)kSrcScale * (double)dt * ((double)szz + kRS * (double)sxx) *
+          (float)(-(double)kSrcScale * dt64 * ((double)szz + kRS * (double)sxx) *
                   (double)P.srcwin[it]);
     const float Gzz = dt * (l2m * szz + lam * sxx);
     const float Gxx = dt * (lam * szz + l2m * sxx);
     const float Gs = dt * amu * sxz;
+    float ozz = Gzz, oxx = Gxx, oxz = Gs, ozx = Gs;
     if (pz) {
-      float q = m[kMvzz * N + i];
-      xs[i] = pml_t(true, Gzz, P.c.Kz[z], P.c.az[z], P.c.bz[z], q);
-      m[kMvzz * N + i] = q;
-      q = m[kMvxz * N + i];
-      xs[2 * N + i] = pml_t(true, Gs, P.c.Kzh[z], P.c.azh[z], P.c.bzh[z], q);
-      m[kMvxz * N + i] = q;
-    } else {
-      xs[i] = Gzz;
-      xs[2 * N + i] = Gs;
+      float q = lz(M, sm, kMvzz, C, 0);
+      ozz = pml_t(Gzz, kz, az, bz, q);
+      st(M, sm, kMvzz, C, q);
+      q = lz(M, sm, kMvxz, C, 0);
+      oxz = pml_t(Gs, kzh, azh, bzh, q);
+      st(M, sm, kMvxz, C, q);
     }
-    if (pxs) {
-      float q = m[kMvxx * N + i];
-      xs[N + i] = pml_t(true, Gxx, P.c.Kx[x], P.c.ax[x], P.c.bx[x], q);
-      m[kMvxx * N + i] = q;
-      q = m[kMvzx * N + i];
-      xs[3 * N + i] = pml_t(true, Gs, P.c.Kxh[x], P.c.axh[x], P.c.bxh[x], q);
-      m[kMvzx * N + i] = q;
-    } else {
-      xs[N + i] = Gxx;
-      xs[3 * N + i] = Gs;
+    if (px) {
+      float q = lz(M, sm, kMvxx, C, 0);
+      oxx = pml_t(Gxx, kx, ax, bx, q);
+      st(M, sm, kMvxx, C, q);
+      q = lz(M, sm, kMvzx, C, 0);
+      ozx = pml_t(Gs, kxh, axh, bxh, q);
+      st(M, sm, kMvzx, C, q);
     }
+    st(X, sx, 0, C, ozz);
+    st(X, sx, 1, C, oxx);
+    st(X, sx, 2, C, oxz);
+    st(X, sx, 3, C, ozx);
     if (in) {
-      const float* w = P.w + (size_t)s * kWaveComps * N;
-      const float* vzw = w + kVz * N;
-      const float* vxw = w + kVx * N;
-      const float dvz_dz = dfw(vzw[i], vzw[i - 1], vzw[i + 1], vzw[i - 2], dz, rdz);
-      const float dvx_dx = dfw(vxw[i], vxw[i - nz], vxw[i + nz], vxw[i - 2 * nz], dx, rdx);
-      const float dvx_dz = dfw(vxw[i + 1], vxw[i], vxw[i + 2], vxw[i - 1], dz, rdz);
-      const float dvz_dx = dfw(vzw[i + nz], vzw[i], vzw[i + 2 * nz], vzw[i - nz], dx, rdx);
       gl += (szz + sxx) * (dvz_dz + dvx_dx);
       gm += szz * dvz_dz + sxx * dvx_dx;
       gs += sxz * (dvx_dz + dvz_dx);
     }
+    if (re) {
+      if (slot < 0) {  // add_source(-) then el_stress.cu:90-131, isFor = false
+        if (z == zs && x == xs) {
+          const float amp = P.src.amp[(size_t)s * g.nSteps + it];
+          wszz = wszz - kSrcScale * amp * dt;
+          wsxx = (float)((double)wsxx - kRS * (double)kSrcScale * (double)amp * dt64);
+        }
+        wszz = (float)((double)wszz - (l2m64 * (double)dvz_dz + (double)(lam * dvx_dx)) * dt64);
+        wsxx = (float)((double)wsxx - ((double)(lam * dvz_dz) + l2m64 * (double)dvx_dx) * dt64);
+        wsxz = wsxz - amu * (dvx_dz + dvz_dx) * dt;
+      }
+      st(W, sw, kSzz, C, wszz);
+      st(W, sw, kSxx, C, wsxx);
+      st(W, sw, kSxz, C, wsxz);
+    }
   }
   if (in) {
     P.acc[i] += gl;
-    P.acc[N + i] += gm;
-    P.acc[2 * N + i] += gs;
+    P.acc[g.N + i] += gm;
+    P.acc[2 * g.N + i] += gs;
   }
 }
 
@@ -2415,7 +2495,7 @@ class Runner {
   void fwd_vel(const StepLaunch& s, int it, int U);
   void bwd_a(const StepLaunch& s, int it, int U);
   void bwd_b(const StepLaunch& s, int it, int U);
-  void exact_a(const Part& P, int first);
+  void exact_a(const Part& P, int it, int first);
   void exact_b(const Part& P, int it);
   Status snapshot(const GroupParts& gp, int sg0, int j, bool save);
   // run `body` (launches only, no memsets) as a graph captured on first use
@@ -2515,17 +2595,19 @@ void Runner::bwd_b(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_b); }
 #undef STEP_LAUNCH_ONE
 
 // exact-adjoint launches: 64 x 4 workgroups over the update region
-#define EXACT_LAUNCH(K, arg)                                                                \
+// (with the reversal of the step: the plan's ring is that of a gradient run)
+#define EXACT_LAUNCH(K, ...)                                                                \
   do {                                                                                      \
     const dim3 grid((unsigned)((P.g.nzA + kBZ - 1) / kBZ), (unsigned)((P.g.nx + 3) / 4));    \
     const dim3 b4(kBZ, 4);                                                                  \
     if (d0->kernel_start)                                                                   \
-      hipExtLaunchKernelGGL(K, grid, b4, 0, st, d0->kernel_start, d0->kernel_end, 0, P, arg); \
+      hipExtLaunchKernelGGL(K, grid, b4, 0, st, d0->kernel_start, d0->kernel_end, 0, P,     \
+                            __VA_ARGS__);                                                   \
     else                                                                                    \
-      hipLaunchKernelGGL(K, grid, b4, 0, st, P, arg);                                       \
+      hipLaunchKernelGGL(K, grid, b4, 0, st, P, __VA_ARGS__);                               \
   } while (0)
-void Runner::exact_a(const Part& P, int first) { EXACT_LAUNCH(k_exact_a, first); }
-void Runner::exact_b(const Part& P, int it) { EXACT_LAUNCH(k_exact_b, it); }
+void Runner::exact_a(const Part& P, int it, int first) { EXACT_LAUNCH(k_exact_a, it, first, 1); }
+void Runner::exact_b(const Part& P, int it) { EXACT_LAUNCH(k_exact_b, it, 1); }
 #undef EXACT_LAUNCH
 
 GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
@@ -2560,7 +2642,6 @@ GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
     P.nrec = d.roff[s1] - d.roff[s0];
     P.gzU = d.gzU();
     P.gzR = d.gzR();
-    P.exact = d.adjoint;
     P.xt = d.xt;
     P.srcwin = d.srcwin;
     gp.parts.push_back(P);
@@ -2951,19 +3032,17 @@ Status Runner::run(int calc_id) {
         }
       };
       // exact adjoint, reverse steps t1-1 .. t0 of a segment: the forward
-      // state enters as (v, sigma) after step t1-1; at reverse step it it
-      // holds (v^it, sigma^{it+1}).  No initial adjoint step: sample 0 is
-      // not recorded, and sample it+1 enters at reverse step it.
+      // state enters as (v, sigma) after step t1-1 and each reverse step it
+      // re-materialises (v^it, sigma^it) in its two launches.  No initial
+      // adjoint step: sample 0 is not recorded, and sample it+1 enters at
+      // reverse step it.
       const Part* xp = gp.parts.empty() ? nullptr : &gp.parts[0];
       auto xsegment = [&](int t0, int t1) {
-        bwd_b(sRV, t1, Ub);  // rev_vel(t1-1)
         for (int it = t1 - 1; it >= t0; it--) {
           const bool smp = it % sample_every == 0;
-          d0->launch("exact_a", smp, [&] { exact_a(*xp, it == nSteps - 2 ? 1 : 0); });
+          d0->launch("exact_a", smp, [&] { exact_a(*xp, it, it == nSteps - 2 ? 1 : 0); });
           inject(it + 1);
           d0->launch("exact_b", smp, [&] { exact_b(*xp, it); });
-          d0->launch("rev_stress", smp, [&] { bwd_a(sR, it, Ub); });
-          if (it > t0) d0->launch("rev_vel", smp, [&] { bwd_b(sRV, it, Ub); });
         }
       };
       if (d0->adjoint) {
@@ -2974,7 +3053,7 @@ Status Runner::run(int calc_id) {
       }
       if (d0->adjoint && !ckpt) {
         if (nSteps >= 2)
-          TRY(replay(graph_key("X", gp, {&mR, &mRV}), [&] { xsegment(0, steps); }));
+          TRY(replay(graph_key("X", gp, {}), [&] { xsegment(0, steps); }));
         HIP_TRY(hipGetLastError());
         continue;
       }

@@ -53,10 +53,13 @@ class PlanObjective:
     written by ``compute_observation``'s files, either a list of [nrec][nSteps]
     float32 arrays per shot or None to load ``Shot{id}.bin`` from the
     workspace.  shot_ids are 1-based as in the reference's Julia API.
+    ``adjoint``: "reference" (the operator's gradient as cufd computes it) or
+    "exact" (the transpose of the forward step: the derivative of the
+    discrete misfit).
     """
 
     def __init__(self, fwi, cs, rho, stf_array, cp_ref, cs_ref=None, rho_ref=None,
-                 shot_ids=None, obs=None, gpu_id=0, illumination=True):
+                 shot_ids=None, obs=None, gpu_id=0, illumination=True, adjoint="reference"):
         self.fwi = fwi
         if shot_ids is None:
             shot_ids = np.arange(1, len(fwi.ind_src_x) + 1)
@@ -70,6 +73,7 @@ class PlanObjective:
         _on_grid(fwi, self.cs, self.rho, self.cp_ref, self.cs_ref, self.rho_ref)
         self.mask = fwi.mask
         self.plan = Plan(fwi.para_path, self.ids, gpu_id)
+        self.plan.set_adjoint(adjoint)
         self.plan.set_stf(stf)
         if obs is None:
             self.plan.load_obs()

@@ -164,6 +164,35 @@ def test_config3_gradient_against_central_differences(workspace):
 
 
 @pytest.mark.timeout(600)
+def test_config3_exact_adjoint_equals_central_differences(workspace):
+    """The exact adjoint (Plan.set_adjoint("exact"), the transpose of the
+    forward step) on the same grid, shots and directions: its directional
+    derivatives equal the oracle's FP64 central differences to 1e-3, where
+    the reference's adjoint is 2%-125% off."""
+    fx = fixture("c3_fd")
+    c = models.case3(nSteps=C3_NT, nshots=64)
+    para, stf = models.write_case(c, workspace)
+    fwiflow.fwi_obs_op(*models.moduli(c, "true"), stf, 0, C3_IDS, para)
+    m0 = models.moduli(c, "start")
+    margin = (c.nPml + 12, c.nPml + c.nPad)
+    dirs = [[smooth_direction((c.NZ, c.NX), 1000 + 10 * k + q, 8.0, (margin, c.nPml))
+             for q in range(3)] for k in range(len(fx["fd64"]))]
+    with fwiflow.Plan(para, C3_IDS) as P:
+        P.set_adjoint("exact")
+        P.set_stf(stf)
+        P.load_obs()
+        P.set_model(*m0)
+        P.run(1)
+        g = P.gradients()[:3]
+    gdm = np.array([sum(np.sum(gq * mq * dq) for gq, mq, dq in zip(g, m0, d)) for d in dirs])
+    fd64 = fx["fd64"]
+    gap = np.abs(gdm - fd64) / np.abs(fd64)
+    say("exact gdm", gdm, "FP64 FD", fd64, "gap", gap,
+        "reference-adjoint gap", np.abs(fx["gdm64"] - fd64) / np.abs(fd64))
+    assert np.all(gap <= 1e-3), gap
+
+
+@pytest.mark.timeout(600)
 @pytest.mark.parametrize("grid", C4_GRIDS, ids=[f"{z}x{x}" for z, x in C4_GRIDS])
 def test_config4_time_lapse_parity(workspace, grid):
     """Time-lapse surveys on the 12 m grids (32-cell C-PML, dt 1.5 ms,

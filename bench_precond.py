@@ -26,6 +26,11 @@ not a descent direction, so no optimiser can use it.
    with a line search (``precond.learned_descent``).  The same L-BFGS with the
    illumination-scaled H0 and no network runs as a control.
 
+``--adjoint exact`` gives every objective the exact gradient (the transpose
+of the forward step) instead of the reference's.  ``--init-net`` continues
+training from saved parameters and ``--eval-first`` / ``--time-cap`` split
+and bound the evaluation, so a full-size run fits several shorter sessions.
+
 Prints ONE JSON line; progress goes to stderr.  Defaults are sized to finish
 in about 15 minutes on one MI355X; --train-models 20 --eval-models 5
 --maxiter 100 is the full-size experiment.  The recorded run used
@@ -70,6 +75,12 @@ def parse():
                    help="start training from these parameters (.npy) instead of a fresh "
                         "network: training in stages over several runs")
     p.add_argument("--train-only", action="store_true", help="stop after training")
+    p.add_argument("--time-cap", type=float, default=None,
+                   help="stop the learned and control runs at this multiple of the plain "
+                        "method's wall time (their ratio is then censored at the cap)")
+    p.add_argument("--eval-first", type=int, default=0,
+                   help="index of the first held-out model (seed seed0 + 1000 + index): "
+                        "evaluation split over several runs")
     p.add_argument("--adjoint", choices=["reference", "exact"], default="reference",
                    help="gradient of every objective: the reference's adjoint or the exact "
                         "one (the transpose of the forward step)")
@@ -151,7 +162,7 @@ def main():
         net.close()
         return
     evals = []
-    for k in range(a.eval_models):
+    for k in range(a.eval_first, a.eval_first + a.eval_models):
         seed = a.seed0 + 1000 + k
         ws = tempfile.mkdtemp(prefix=f"fwiflow_pc_{seed}_")
         fwi, obj, x0 = setup(seed, a.nt, ws, a.adjoint)
@@ -163,16 +174,19 @@ def main():
         def progress(k, J, used, alpha, nev):
             if k % 5 == 0:
                 log(f"  iteration {k}: misfit {J:.4g} ({used}, alpha {alpha:.3g}, {nev} evaluations)")
+        cap = a.time_cap * rp["seconds"] if a.time_cap else None
         if a.driver == "lbfgs":
             rl = precond.learned_lbfgs(obj, x0, fwi.mask, net=net, maxiter=2 * a.maxiter,
-                                       target_misfit=rp["misfit"], on_iter=progress)
+                                       target_misfit=rp["misfit"], on_iter=progress,
+                                       max_seconds=cap)
         else:
             rl = precond.learned_descent(obj, x0, net, fwi.mask, maxiter=2 * a.maxiter,
                                          target_misfit=rp["misfit"], step0=a.step0)
         log(f"eval {seed}: plain {rp['misfit']:.4g} in {rp['seconds']:.1f} s; "
             f"learned {rl['misfit']:.4g} in {rl['seconds']:.1f} s")
         rc = precond.learned_lbfgs(obj, x0, fwi.mask, net=None, maxiter=2 * a.maxiter,
-                                   target_misfit=rp["misfit"], on_iter=progress)
+                                   target_misfit=rp["misfit"], on_iter=progress,
+                                   max_seconds=cap)
         log(f"eval {seed}: control {rc['misfit']:.4g} in {rc['seconds']:.1f} s")
         obj.close()
         reached = rl["misfit"] <= rp["misfit"]
@@ -205,7 +219,8 @@ def main():
                    "grid": [int(H), int(W)],
                    "nt": a.nt, "shots": 30,
                    "train_models":
-                   a.train_models, "eval_models": a.eval_models, "maxiter": a.maxiter,
+                   a.train_models, "eval_models": a.eval_models, "eval_first": a.eval_first,
+                   "maxiter": a.maxiter, "time_cap": a.time_cap,
                    "pairs": len(samples), "epochs": a.epochs, "lr": a.lr},
         "training": {"seconds": round(t_train, 1), "loss_first_epoch": hist[0],
                      "loss_last_epoch": hist[-1]},

@@ -226,7 +226,7 @@ def learned_descent(objective, x0, net, mask, bounds=CP_BOUNDS, maxiter=100,
 
 def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, memory=5,
                   target_misfit=None, first_step_rms=10.0, c_armijo=1e-4, max_backtracks=4,
-                  illum_eps=1e-3, on_iter=None, on_pair=None):
+                  illum_eps=1e-3, on_iter=None, on_pair=None, max_seconds=None):
     """L-BFGS on cp with the network as the initial inverse Hessian.
 
     The two-loop recursion (Nocedal & Wright, Alg. 7.4) applies H0 to the
@@ -245,8 +245,9 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
     at a tenth of the step; only then does the run stop.  ``on_iter(k, J,
     used, alpha, evaluations)`` is called after each accepted step and
     ``on_pair(gradient, extras, update, misfit)`` with the training pair of
-    that step, as ``inversion.lbfgs`` logs them.  Returns the dict of
-    ``learned_descent``.
+    that step, as ``inversion.lbfgs`` logs them.  ``max_seconds`` stops the
+    run after the first step that ends past that wall time.  Returns the dict
+    of ``learned_descent``.
     """
     shape = np.asarray(x0).shape
     m2 = np.asarray(mask)
@@ -348,6 +349,8 @@ def learned_lbfgs(objective, x0, mask, net=None, bounds=CP_BOUNDS, maxiter=100, 
         if on_iter is not None:
             on_iter(len(hist), J, used, alpha, nev)
         if target_misfit is not None and J <= target_misfit:
+            break
+        if max_seconds is not None and time.perf_counter() - t0 > max_seconds:
             break
     return {"x": x.reshape(shape), "misfit": J, "iterations": len(hist), "evaluations": nev,
             "seconds": time.perf_counter() - t0, "history": hist}

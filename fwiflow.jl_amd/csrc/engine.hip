@@ -207,9 +207,6 @@ struct Part {
   int nrec;         // receivers of the part's shots
   int gzU;          // tiles along z of the update-region tiling
   int gzR;          // tiles along z of the ring-rectangle tiling
-  // exact adjoint (the transpose of the forward step; no reference counterpart)
-  float* xt;             // transposed-stencil operands [group shot][8][cell]
-  const float* srcwin;   // d(source amplitude)/d(stf) of each step
 };
 
 // Kernel argument: one part by value, or a table of parts in device memory.
@@ -1031,11 +1028,26 @@ __device__ __forceinline__ float pml_t(float g, const Kdiv& K, float a, float b,
 // D+ f = (c1 (f[k+1] - f[k]) - c2 (f[k+2] - f[k-1])) / h and
 // D- f = (c1 (f[k] - f[k-1]) - c2 (f[k+1] - f[k-2])) / h.
 
-__global__ __launch_bounds__(256) void k_exact_a(Part P, int it, int first, int rev) {
+// The cell of the calling thread: 64 x 4 tiles over the update region,
+// z fastest, in XCD bands (workgroup b runs on XCD b % 8, and XCD k takes
+// the k-th eighth of the tiles, so the columns a tile shares with its
+// neighbours stay in that XCD's L2).  The grid is 8 * ceil(tiles / 8)
+// workgroups of 64 x 4; false for threads outside the update region.
+__device__ __forceinline__ bool exact_cell(const Geo& g, int& z, int& x) {
+  const int gz = (g.nzA + kBZ - 1) / kBZ, gx = (g.nx + 3) / 4;
+  const int tiles = gz * gx, per = (tiles + 7) / 8;
+  const int b = (int)blockIdx.x, t = (b & 7) * per + (b >> 3);
+  if (t >= tiles) return false;
+  z = (t % gz) * kBZ + (int)threadIdx.x;
+  x = wave_col((t / gz) * 4);
+  return z >= 2 && z <= g.nzA - 3 && x >= 2 && x <= g.nx - 3;
+}
+
+__global__ __launch_bounds__(256) void k_exact_a(Part P, float* __restrict__ xt, int it, int first,
+                                                 int rev) {
   const Geo& g = P.g;
-  const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int x = wave_col((int)(blockIdx.y * blockDim.y));
-  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  int z, x;
+  if (!exact_cell(g, z, x)) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
@@ -1045,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_exact_a(Part P, int it, int first, int 
   const bool px = x < g.nPml || x > g.nx - g.nPml;  // el_velocity's gate
   const Cell C = cell_of((unsigned)i * 4u, nz);
   const int Nb = (int)(g.N * 4);
-  const Arr W = arr(P.w, Nb), A = arr(P.a, Nb), M = arr(P.p, Nb), X = arr(P.xt, Nb);
+  const Arr W = arr(P.w, Nb), A = arr(P.a, Nb), M = arr(P.p, Nb), X = arr(xt, Nb);
   const float ba = P.m.ba[i], bb = P.m.bb[i];
   const float dt = g.dt, dz = g.dz, dx = g.dx, rdz = g.rdz, rdx = g.rdx;
   Kdiv kzh{}, kz{}, kx{}, kxh{};
@@ -1140,11 +1152,11 @@ __global__ __launch_bounds__(256) void k_exact_a(Part P, int it, int first, int 
   }
 }
 
-__global__ __launch_bounds__(256) void k_exact_b(Part P, int it, int rev) {
+__global__ __launch_bounds__(256) void k_exact_b(Part P, float* __restrict__ xt,
+                                                 const float* __restrict__ srcwin, int it, int rev) {
   const Geo& g = P.g;
-  const int z = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int x = wave_col((int)(blockIdx.y * blockDim.y));
-  if (z < 2 || z > g.nzA - 3 || x < 2 || x > g.nx - 3) return;
+  int z, x;
+  if (!exact_cell(g, z, x)) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
@@ -1154,7 +1166,7 @@ __global__ __launch_bounds__(256) void k_exact_b(Part P, int it, int rev) {
   const bool px = x < g.nPml || x > g.nx - g.nPml - 1;  // el_stress's gate
   const Cell C = cell_of((unsigned)i * 4u, nz);
   const int Nb = (int)(g.N * 4);
-  const Arr W = arr(P.w, Nb), A = arr(P.a, Nb), M = arr(P.p, Nb), X = arr(P.xt, Nb);
+  const Arr W = arr(P.w, Nb), A = arr(P.a, Nb), M = arr(P.p, Nb), X = arr(xt, Nb);
   const float lam = P.m.lam[i], mu = P.m.mu[i], amu = P.m.amu[i];
   const double l2m64 = (double)lam + 2.0 * (double)mu;
   const float l2m = (float)l2m64;
@@ -1227,7 +1239,7 @@ __global__ __launch_bounds__(256) void k_exact_b(Part P, int it, int rev) {
     if (P.stf_grad && z == zs && x == xs)
       P.stf_grad[(size_t)s * g.nSteps + it] =
           (float)(-(double)kSrcScale * dt64 * ((double)szz + kRS * (double)sxx) *
-                  (double)P.srcwin[it]);
+                  (double)srcwin[it]);
     const float Gzz = dt * (l2m * szz + lam * sxx);
     const float Gxx = dt * (lam * szz + l2m * sxx);
     const float Gs = dt * amu * sxz;
@@ -2497,6 +2509,8 @@ class Runner {
   void bwd_b(const StepLaunch& s, int it, int U);
   void exact_a(const Part& P, int it, int first);
   void exact_b(const Part& P, int it);
+  float* xt_ = nullptr;            // exact adjoint operands of the running group
+  const float* srcwin_ = nullptr;  // and the source window
   Status snapshot(const GroupParts& gp, int sg0, int j, bool save);
   // run `body` (launches only, no memsets) as a graph captured on first use
   Status replay(const std::string& key, const std::function<void()>& body);
@@ -2598,7 +2612,8 @@ void Runner::bwd_b(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_b); }
 // (with the reversal of the step: the plan's ring is that of a gradient run)
 #define EXACT_LAUNCH(K, ...)                                                                \
   do {                                                                                      \
-    const dim3 grid((unsigned)((P.g.nzA + kBZ - 1) / kBZ), (unsigned)((P.g.nx + 3) / 4));    \
+    const int tiles_ = ((P.g.nzA + kBZ - 1) / kBZ) * ((P.g.nx + 3) / 4);                   \
+    const dim3 grid((unsigned)(8 * ((tiles_ + 7) / 8)));                                    \
     const dim3 b4(kBZ, 4);                                                                  \
     if (d0->kernel_start)                                                                   \
       hipExtLaunchKernelGGL(K, grid, b4, 0, st, d0->kernel_start, d0->kernel_end, 0, P,     \
@@ -2606,8 +2621,10 @@ void Runner::bwd_b(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_b); }
     else                                                                                    \
       hipLaunchKernelGGL(K, grid, b4, 0, st, P, __VA_ARGS__);                               \
   } while (0)
-void Runner::exact_a(const Part& P, int it, int first) { EXACT_LAUNCH(k_exact_a, it, first, 1); }
-void Runner::exact_b(const Part& P, int it) { EXACT_LAUNCH(k_exact_b, it, 1); }
+void Runner::exact_a(const Part& P, int it, int first) {
+  EXACT_LAUNCH(k_exact_a, xt_, it, first, 1);
+}
+void Runner::exact_b(const Part& P, int it) { EXACT_LAUNCH(k_exact_b, xt_, srcwin_, it, 1); }
 #undef EXACT_LAUNCH
 
 GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
@@ -2642,8 +2659,6 @@ GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {
     P.nrec = d.roff[s1] - d.roff[s0];
     P.gzU = d.gzU();
     P.gzR = d.gzR();
-    P.xt = d.xt;
-    P.srcwin = d.srcwin;
     gp.parts.push_back(P);
     gp.member.push_back((int)mi);
     gp.key += "|" + std::to_string(mi) + ":" + std::to_string(s0) + "-" + std::to_string(s1);
@@ -3050,10 +3065,13 @@ Status Runner::run(int calc_id) {
           DeviceState& d = *M[gp.member[p]].d;
           HIP_TRY(hipMemsetAsync(d.xt, 0, (size_t)gp.parts[p].g.ns * 8 * d.g.N * sizeof(float), st));
         }
+        xt_ = d0->xt;
+        srcwin_ = d0->srcwin;
       }
       if (d0->adjoint && !ckpt) {
         if (nSteps >= 2)
-          TRY(replay(graph_key("X", gp, {}), [&] { xsegment(0, steps); }));
+          TRY(replay(graph_key("X", gp, {}) + "/" + std::to_string((uintptr_t)xt_),
+                     [&] { xsegment(0, steps); }));
         HIP_TRY(hipGetLastError());
         continue;
       }

@@ -24,6 +24,7 @@ CASES = {
     "zero_shear_layer": {"water": True},
     "groups_of_1": {"settings": {"FWIFLOW_SHOT_GROUP": "1", "FWIFLOW_SHOT_GROUP_BWD": "1"}},
     "irregular_survey": {"irregular": True},
+    "graph_replay": {"settings": {"FWIFLOW_GRAPHS": "1"}},
 }
 
 
@@ -57,7 +58,14 @@ def _exact_run(para, ids, m0, stf, settings=None, checkpoint=None, mode="exact")
         if checkpoint is not None:
             P.set_checkpoint(checkpoint)
         P.run(1)
-        return P.gradients(), P.misfit()
+        g, J = P.gradients(), P.misfit()
+        if (settings or {}).get("FWIFLOW_GRAPHS") == "1":
+            # the captured backward replayed, with eager work in between
+            for _ in range(2):
+                P.run(2)
+                P.run(1)
+                assert all(np.array_equal(a, b) for a, b in zip(P.gradients(), g))
+        return g, J
 
 
 @pytest.mark.parametrize("case", list(CASES))

@@ -1938,11 +1938,13 @@ struct DeviceState {
   int gzR() const { return (g.ringH + kBZ - 1) / kBZ; }
   int gxR() const { return (g.ringW + tile_x - 1) / tile_x; }
   // ~40 B per cell and shot of forward state stay cache-resident from step to
-  // step; the backward holds forward and adjoint state (~64 B).  224 MB of
-  // the 256 MB Infinity Cache for them (measured at 608x1780: 5 shots per
-  // forward launch, 3 per backward launch).
+  // step; the backward holds forward and adjoint state (~64 B), and the exact
+  // adjoint its two operand sets besides (+32 B).  224 MB of the 256 MB
+  // Infinity Cache for them (measured at 608x1780: 5 shots per forward
+  // launch, 3 per backward launch; 2 for the exact adjoint, 4.87
+  // shot-gradients/s against 4.37 with 3).
   double fwd_bytes_per_shot() const { return 40.0 * (double)g.N; }
-  double bwd_bytes_per_shot() const { return 64.0 * (double)g.N; }
+  double bwd_bytes_per_shot() const { return (adjoint ? 96.0 : 64.0) * (double)g.N; }
   size_t ring_step_floats() const { return (size_t)5 * g.ringlen; }
   size_t snap_floats_per_shot() const { return (size_t)13 * g.N; }
   // zero shots [s0, s0+n) of a [shot][comps][cell] array

@@ -301,3 +301,53 @@ def test_config2_parity(workspace, water):
     fx = fixture(f"c2_{'water' if water else 'elastic'}")
     gpu, _, _ = gpu_case(c, C2_IDS, workspace, gradient=not water)
     compare(gpu, fx, gradient=not water)
+
+
+@pytest.mark.timeout(600)
+def test_config2_water_layer_exact_adjoint_equals_central_differences(workspace):
+    """Config 2 with its 10-row water layer (Vs = 0), 2000 steps, shots 0, 14
+    and 29: the reference's adjoint grows without bound there, the exact
+    adjoint (the transpose of the forward step) stays the derivative of the
+    misfit.  Its directional derivatives in 3 smooth random directions
+    (lambda, mu, rho perturbed together by m (1 +- h r), h = 1e-3; the water's
+    mu stays 0) equal central differences of the GPU's misfit to 2e-3, and
+    the reference-mode gradient's do not (printed with the exact ones)."""
+    h = 1e-3
+    c = models.case2(water=True)
+    para, stf = models.write_case(c, workspace)
+    fwiflow.fwi_obs_op(*models.moduli(c, "true"), stf, 0, C2_IDS, para)
+    m0 = models.moduli(c, "start")
+    hard = np.zeros((c.NZ, c.NX))
+    hard[c.nPml + 2:c.NZ - c.nPad - c.nPml - 2, c.nPml + 2:c.NX - c.nPml - 2] = 1.0
+    margin = (c.nPml + 2, c.nPml + c.nPad)
+    dirs = [[smooth_direction((c.NZ, c.NX), 3000 + 10 * k + q, 6.0, (margin, c.nPml)) * hard
+             for q in range(3)] for k in range(3)]
+    with fwiflow.Plan(para, C2_IDS) as P:
+        P.set_stf(stf)
+        P.load_obs()
+        P.set_model(*m0)
+        P.run(1)
+        gref = P.gradients()[:3]
+        P.set_adjoint("exact")
+        P.run(1)
+        g = P.gradients()[:3]
+        fd = []
+        for d in dirs:
+            J = []
+            for sgn in (1.0, -1.0):
+                P.set_model(*[mq * (1.0 + sgn * h * dq) for mq, dq in zip(m0, d)])
+                P.run(0)
+                J.append(P.misfit())
+            fd.append((J[0] - J[1]) / (2 * h))
+    fd = np.array(fd)
+
+    def dd(gr):
+        return np.array([sum(np.sum(gq * mq * dq) for gq, mq, dq in zip(gr, m0, d)) for d in dirs])
+
+    gap = np.abs(dd(g) - fd) / np.abs(fd)
+    gap_ref = np.abs(dd(gref) - fd) / np.abs(fd)
+    say("water layer: exact gdm", dd(g), "FD", fd, "gap", gap, "reference-adjoint gdm",
+        dd(gref), "gap", gap_ref)
+    assert all(np.all(np.isfinite(a)) for a in g)
+    assert np.all(gap <= 2e-3), gap
+    assert np.all(~(gap_ref <= 10 * gap)), (gap_ref, gap)  # NaN counts as off

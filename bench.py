@@ -39,12 +39,17 @@ KERNEL_BYTES = {
     "rev_stress": 44.0,   # read 2 velocity, 3 stress, 3 adjoint stress, write 3 stress
     "adj_vel": 28.0,      # read 3 adjoint stress, 2 adjoint velocity, write 2
     "adj_stress": 32.0,   # read 2 adjoint velocity, 3 adjoint stress, write 3
-    # exact adjoint (--adjoint exact), each with the reverse step of its field
-    "exact_a": 76.0,      # read 2 adjoint velocity, 4 stress operands, 3 stress, 2 velocity;
-                          # write 2 adjoint velocity, 4 velocity operands, 2 velocity
-    "exact_b": 88.0,      # read 3 adjoint stress, 4 velocity operands, 2 velocity, 3 stress;
-                          # write 3 adjoint stress, 4 stress operands, 3 stress
+    # exact adjoint (--adjoint exact), each with the reverse step of its field;
+    # outside the C-PML frame 3 of the 4 stress operands and 2 of the 4
+    # velocity operands are distinct (KERNEL_FRAME adds the others)
+    "exact_a": 64.0,      # read 2 adjoint velocity, 3 stress operands, 3 stress, 2 velocity;
+                          # write 2 adjoint velocity, 2 velocity operands, 2 velocity
+    "exact_b": 76.0,      # read 3 adjoint stress, 2 velocity operands, 2 velocity, 3 stress;
+                          # write 3 adjoint stress, 3 stress operands, 3 stress
 }
+# exact adjoint: the operands stored only in the C-PML frame, per frame cell
+# (k_exact_a reads 1 and writes 2, k_exact_b reads 2 and writes 1)
+KERNEL_FRAME = {"exact_a": 12.0, "exact_b": 12.0}
 # kernels that run two of the above as one grid (disjoint workgroup ranges)
 KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_stress", "rev_vel")}
 # Measured HBM-side bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, each
@@ -53,8 +58,7 @@ KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_str
 # Reported as roofline.traffic when the dominant kernel and its mean shots
 # per launch match the profiled run.
 MEASURED_TRAFFIC = {("bwd_pair_a", 2.91): 231.5e6, ("bwd_pair_b", 2.91): 226.9e6,
-                    ("stress_fwd", 4.92): 204.4e6, ("vel_fwd", 4.92): 180.9e6,
-                    ("exact_a", 2.0): 182.6e6, ("exact_b", 2.0): 218.3e6}
+                    ("stress_fwd", 4.92): 204.4e6, ("vel_fwd", 4.92): 180.9e6}
 # kernels of the backward sweep (their launches cover the backward shot groups)
 BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "bwd_pair_a", "bwd_pair_b",
             "exact_a", "exact_b")
@@ -126,8 +130,9 @@ def geometry_counts(c):
     pml_x = 2 * c.nPml * nzA  # x-direction C-PML cells (all active rows; corners have both)
     upd = (nzA - 4) * (NX - 4)  # stencil update region [2, n-3]^2
     ring_rect = (nzA - 2 * c.nPml + 4) * (NX - 2 * c.nPml + 4)
+    frame = upd - (nzA - 2 * c.nPml) * (NX - 2 * c.nPml)  # update-region cells in the C-PML
     return dict(active=active, pml_dirs=pml_z + pml_x, update=upd, ring_rect=ring_rect,
-                pml_z=pml_z, pml_x=pml_x)
+                pml_z=pml_z, pml_x=pml_x, frame=frame)
 
 
 def algorithmic_bytes_per_shot_gradient(c, nt):
@@ -144,6 +149,7 @@ def kernel_launch_bytes(name, c, ns):
     cells = g["ring_rect"] if name.startswith("rev_") else g["update"]
     b = KERNEL_BYTES[name] * cells * ns
     b += KERNEL_PML.get(name, 0.0) * g["pml_dirs"] * ns
+    b += KERNEL_FRAME.get(name, 0.0) * g["frame"] * ns
     return b
 
 

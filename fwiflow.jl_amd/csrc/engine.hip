@@ -1016,7 +1016,10 @@ __device__ __forceinline__ void adj_stress(const Part& P, int tile, int it) {
 // Each reversal writes only its cell's own forward field, which the adjoint
 // half of its launch does not read.  xt holds two operand sets per shot
 // (slots 0-3: Azz, Axx, Axz, Azx of the stress update; 4-7: Bz, Bzx, Bxz, Bx
-// of the velocity update).
+// of the velocity update).  Outside the C-PML frame (cells whose row or
+// column is gated) Azx = Axz, Bzx = Bz and Bx = Bxz, so slots 3, 5 and 7 are
+// written and read only inside the frame, and a reader takes slots 2, 4 and 6
+// for neighbours outside it.
 __device__ __forceinline__ float pml_t(float g, const Kdiv& K, float a, float b, float& P) {
   const float tot = g + P;
   P = b * tot;
@@ -1027,6 +1030,15 @@ __device__ __forceinline__ float pml_t(float g, const Kdiv& K, float a, float b,
 // (D-)^T B at j = (c1 (B[j] - B[j+st]) - c2 (B[j-st] - B[j+2st])) / h, for
 // D+ f = (c1 (f[k+1] - f[k]) - c2 (f[k+2] - f[k-1])) / h and
 // D- f = (c1 (f[k] - f[k-1]) - c2 (f[k+1] - f[k-2])) / h.
+
+// x neighbour k (-2..2) of the cell from operand slot `own` when the
+// neighbour lies in the C-PML frame (`f`), else from slot `base`; the slot
+// goes into the per-lane offset, so `f` may differ between lanes.
+__device__ __forceinline__ float lxf(const Arr& X, int sx, int own, int base, const Cell& C, int k,
+                                     bool f) {
+  const int v = k == 0 ? C.zm2 + 8 : k == -2 ? C.xm2 : k == -1 ? C.xm1 : k == 1 ? C.xp1 : C.xp2;
+  return ldb(X.r, sx, v + (f ? own : base) * X.nb);
+}
 
 // The cell of the calling thread: 64 x 4 tiles over the update region,
 // z fastest, in XCD bands (workgroup b runs on XCD b % 8, and XCD k takes
@@ -1101,8 +1113,10 @@ __global__ __launch_bounds__(256) void k_exact_a(Part P, float* __restrict__ xt,
     if (!first) {
       const f4 azz = lz4(X, sx, 0, C, -1);  // Azz at z-1 .. z+2
       const f4 axz = lz4(X, sx, 2, C, -2);  // Axz at z-2 .. z+1
-      const float azxm = lx(X, sx, 3, C, -1), azx0 = lz(X, sx, 3, C, 0),
-                  azxm2 = lx(X, sx, 3, C, -2), azxp = lx(X, sx, 3, C, 1);
+      // Azx (slot 3 in the stress update's frame, else Axz)
+      auto fs = [&](int k) { return pz || x + k < g.nPml || x + k > g.nx - g.nPml - 1; };
+      const float azxm = lxf(X, sx, 3, 2, C, -1, fs(-1)), azx0 = lxf(X, sx, 3, 2, C, 0, fs(0)),
+                  azxm2 = lxf(X, sx, 3, 2, C, -2, fs(-2)), azxp = lxf(X, sx, 3, 2, C, 1, fs(1));
       const float axx0 = lz(X, sx, 1, C, 0), axxp = lx(X, sx, 1, C, 1),
                   axxm = lx(X, sx, 1, C, -1), axxp2 = lx(X, sx, 1, C, 2);
       // (D-)^T Azz along z + (D+)^T Azx along x; (D-)^T Axx along x + (D+)^T Axz along z
@@ -1130,9 +1144,11 @@ __global__ __launch_bounds__(256) void k_exact_a(Part P, float* __restrict__ xt,
       st(M, sm, kMsxx, C, q);
     }
     st(X, sx, 4, C, oz);
-    st(X, sx, 5, C, ozx);
     st(X, sx, 6, C, oxz);
-    st(X, sx, 7, C, ox);
+    if (pz || px) {  // the velocity update's frame
+      st(X, sx, 5, C, ozx);
+      st(X, sx, 7, C, ox);
+    }
     if (in) {
       ga += vz * fz;
       gb += vx * fx;
@@ -1223,10 +1239,12 @@ __global__ __launch_bounds__(256) void k_exact_b(Part P, float* __restrict__ xt,
     {
       const f4 bzc = lz4(X, sx, 4, C, -2);   // Bz at z-2 .. z+1
       const f4 bxzc = lz4(X, sx, 6, C, -1);  // Bxz at z-1 .. z+2
-      const float bzx0 = lz(X, sx, 5, C, 0), bzxp = lx(X, sx, 5, C, 1),
-                  bzxm = lx(X, sx, 5, C, -1), bzxp2 = lx(X, sx, 5, C, 2);
-      const float bxm = lx(X, sx, 7, C, -1), bx0 = lz(X, sx, 7, C, 0),
-                  bxm2 = lx(X, sx, 7, C, -2), bxp = lx(X, sx, 7, C, 1);
+      // Bzx and Bx (slots 5 and 7 in the velocity update's frame, else Bz and Bxz)
+      auto fv = [&](int k) { return pz || x + k < g.nPml || x + k > g.nx - g.nPml; };
+      const float bzx0 = lxf(X, sx, 5, 4, C, 0, fv(0)), bzxp = lxf(X, sx, 5, 4, C, 1, fv(1)),
+                  bzxm = lxf(X, sx, 5, 4, C, -1, fv(-1)), bzxp2 = lxf(X, sx, 5, 4, C, 2, fv(2));
+      const float bxm = lxf(X, sx, 7, 6, C, -1, fv(-1)), bx0 = lxf(X, sx, 7, 6, C, 0, fv(0)),
+                  bxm2 = lxf(X, sx, 7, 6, C, -2, fv(-2)), bxp = lxf(X, sx, 7, 6, C, 1, fv(1));
       // (D+)^T Bz along z; (D-)^T Bzx along x + (D-)^T Bxz along z; (D+)^T Bx along x
       szz = szz + dfw(bzc[1], bzc[2], bzc[0], bzc[3], dz, rdz);
       sxz = sxz + (dfw(bzx0, bzxp, bzxm, bzxp2, dx, rdx) + dfw(bxzc[1], bxzc[2], bxzc[0], bxzc[3], dz, rdz));
@@ -1263,7 +1281,7 @@ __global__ __launch_bounds__(256) void k_exact_b(Part P, float* __restrict__ xt,
     st(X, sx, 0, C, ozz);
     st(X, sx, 1, C, oxx);
     st(X, sx, 2, C, oxz);
-    st(X, sx, 3, C, ozx);
+    if (pz || px) st(X, sx, 3, C, ozx);  // the stress update's frame
     if (in) {
       gl += (szz + sxx) * (dvz_dz + dvx_dx);
       gm += szz * dvz_dz + sxx * dvx_dx;

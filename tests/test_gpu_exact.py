@@ -141,3 +141,39 @@ def test_exact_adjoint_in_a_job_runs_plans_one_after_another(workspace):
     finally:
         for P in plans:
             P.close()
+
+
+def test_exact_adjoint_with_conditioning_matches_oracle(oracle_mod, workspace):
+    """Per-trace windows and weights plus the band-pass filter
+    (libCUFD.cu:254-309): the exact adjoint injects the conditioned
+    residual, as the reference's does; gradients against the oracle's scheme
+    3 with the same stages, FP32 and FP64."""
+    from test_gpu import _cond_setup
+    c, para, stf, cond = _cond_setup(workspace, if_win=True, filter_para=[2.0, 5.0, 25.0, 40.0])
+    ids = list(range(len(c.src_x)))
+    g = oracle_mod.geom(c.NZ, c.NX, c.nPml, c.nPad, c.nSteps, c.dz, c.dx, c.dt, c.f0)
+    src = [(z + c.nPml, x + c.nPml) for z, x in zip(c.src_z, c.src_x)]
+    recs = [(c.rec_z + c.nPml, c.rec_x + c.nPml)] * len(src)
+    obs = oracle_mod.run(g, 2, *models.moduli(c, "true"), stf, src, recs, precision=32)["syn"]
+    m0 = models.moduli(c, "start")
+    o32 = oracle_mod.run(g, 1, *m0, stf, src, recs, obs, precision=32, cond=cond, scheme=3,
+                         ckpt_every=50)
+    o64 = oracle_mod.run(g, 1, *m0, stf, src, recs, obs, precision=64, cond=cond, scheme=3,
+                         ckpt_every=50)
+    with fwiflow.Plan(para, ids) as P:
+        P.set_adjoint("exact")
+        P.set_model(*m0)
+        P.set_stf(stf)
+        P.set_obs(obs)
+        P.run(1)
+        gL, gM, gD, gS = P.gradients()
+    out = {}
+    n = len(ids)
+    for name, a, key in (("lam", gL, "grad_lambda"), ("mu", gM, "grad_mu"), ("den", gD, "grad_den"),
+                         ("stf", gS[:n], "grad_stf")):
+        b, d = np.asarray(o32[key]), np.asarray(o64[key])
+        if name == "stf":  # rows 0..n-1 hold the group's shots
+            b, d = b[:n], d[:n]
+        out[name] = (rel(a, b), rel(a, d), rel(b, d))
+    say("exact + conditioning", {k: tuple(f"{x:.2e}" for x in v) for k, v in out.items()})
+    _assert(out)

@@ -1055,16 +1055,15 @@ __device__ __forceinline__ bool exact_cell(const Geo& g, int& z, int& x) {
   return z >= 2 && z <= g.nzA - 3 && x >= 2 && x <= g.nx - 3;
 }
 
-__global__ __launch_bounds__(256) void k_exact_a(Part P, float* __restrict__ xt, int it, int first,
-                                                 int rev) {
+__global__ __launch_bounds__(256) void k_exact_a(Part P, float* __restrict__ xt, int it, int first) {
   const Geo& g = P.g;
   int z, x;
   if (!exact_cell(g, z, x)) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
-  const int slot = rev ? ring_slot(z, x, g) : -1;
-  const bool re = rev && (in || slot >= 0);  // the cell takes part in the reversal
+  const int slot = ring_slot(z, x, g);
+  const bool re = in || slot >= 0;  // the cell takes part in the reversal
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
   const bool px = x < g.nPml || x > g.nx - g.nPml;  // el_velocity's gate
   const Cell C = cell_of((unsigned)i * 4u, nz);
@@ -1169,15 +1168,15 @@ __global__ __launch_bounds__(256) void k_exact_a(Part P, float* __restrict__ xt,
 }
 
 __global__ __launch_bounds__(256) void k_exact_b(Part P, float* __restrict__ xt,
-                                                 const float* __restrict__ srcwin, int it, int rev) {
+                                                 const float* __restrict__ srcwin, int it) {
   const Geo& g = P.g;
   int z, x;
   if (!exact_cell(g, z, x)) return;
   const int nz = g.nz;
   const long long i = (long long)x * nz + z;
   const bool in = z >= g.nPml && z <= g.nzA - 1 - g.nPml && x >= g.nPml && x <= g.nx - 1 - g.nPml;
-  const int slot = rev ? ring_slot(z, x, g) : -1;
-  const bool re = rev && (in || slot >= 0);
+  const int slot = ring_slot(z, x, g);
+  const bool re = in || slot >= 0;  // the cell takes part in the reversal
   const bool pz = z < g.nPml || z > g.nzA - g.nPml - 1;
   const bool px = x < g.nPml || x > g.nx - g.nPml - 1;  // el_stress's gate
   const Cell C = cell_of((unsigned)i * 4u, nz);
@@ -2641,10 +2640,8 @@ void Runner::bwd_b(const StepLaunch& s, int it, int U) { STEP_LAUNCH(k_bwd_b); }
     else                                                                                    \
       hipLaunchKernelGGL(K, grid, b4, 0, st, P, __VA_ARGS__);                               \
   } while (0)
-void Runner::exact_a(const Part& P, int it, int first) {
-  EXACT_LAUNCH(k_exact_a, xt_, it, first, 1);
-}
-void Runner::exact_b(const Part& P, int it) { EXACT_LAUNCH(k_exact_b, xt_, srcwin_, it, 1); }
+void Runner::exact_a(const Part& P, int it, int first) { EXACT_LAUNCH(k_exact_a, xt_, it, first); }
+void Runner::exact_b(const Part& P, int it) { EXACT_LAUNCH(k_exact_b, xt_, srcwin_, it); }
 #undef EXACT_LAUNCH
 
 GroupParts Runner::parts_of(RowRange rr, int sg0, Phase ph) {

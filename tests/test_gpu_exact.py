@@ -177,3 +177,19 @@ def test_exact_adjoint_with_conditioning_matches_oracle(oracle_mod, workspace):
         out[name] = (rel(a, b), rel(a, d), rel(b, d))
     say("exact + conditioning", {k: tuple(f"{x:.2e}" for x in v) for k, v in out.items()})
     _assert(out)
+
+
+def test_exact_adjoint_through_cufd_and_noalloc(workspace):
+    """FWIFLOW_EXACT_ADJOINT=1 selects the exact adjoint for the stateless
+    entry points too: fwiflow_cufd and fwiflow_cufd_noalloc give the plan's
+    exact gradients (to the summation order of the shots of a launch)."""
+    c, para, stf, ids, m0 = _case(workspace)
+    g, J = _exact_run(para, ids, m0, stf)
+    for noalloc in (False, True):
+        with env(FWIFLOW_EXACT_ADJOINT="1"):
+            _, gL, gM, gD, gS = fwiflow.ops.cufd(*m0, stf, 1, 0, ids, para, noalloc=noalloc)
+        d = [rel(a, b) for a, b in zip((gL, gM, gD, gS), g)]
+        say("cufd exact", "noalloc" if noalloc else "", [f"{x:.1e}" for x in d])
+        assert max(d) <= 1e-6, d
+    _, gL, _, _, _ = fwiflow.ops.cufd(*m0, stf, 1, 0, ids, para)
+    assert rel(gL, g[0]) > 1e-3  # without the variable: the reference's adjoint

@@ -23,8 +23,10 @@ not a descent direction, so no optimiser can use it.
    lower is better, the target is <= 0.5.  ``--driver lbfgs`` (default) is
    L-BFGS with the network as the initial inverse Hessian
    (``precond.learned_lbfgs``); ``--driver descent`` the network's direction
-   with a line search (``precond.learned_descent``).  The same L-BFGS with the
-   illumination-scaled H0 and no network runs as a control.
+   with a line search (``precond.learned_descent``); ``--driver diag`` the
+   plain method's L-BFGS-B on variables scaled by the diagonal the network
+   implies at the starting model (``precond.diagonal_scale``).  The same
+   driver with the illumination scaling and no network runs as a control.
 
 ``--adjoint exact`` gives every objective the exact gradient (the transpose
 of the forward step) instead of the reference's.  ``--init-net`` continues
@@ -63,7 +65,7 @@ def parse():
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--step0", type=float, default=20.0)
     p.add_argument("--seed0", type=int, default=100)
-    p.add_argument("--driver", choices=["lbfgs", "descent"], default="lbfgs")
+    p.add_argument("--driver", choices=["lbfgs", "descent", "diag"], default="lbfgs")
     p.add_argument("--teacher", choices=["plain", "control"], default="control",
                    help="optimiser whose accepted updates are the training pairs")
     p.add_argument("--teacher-maxiter", type=int, default=None,
@@ -175,7 +177,22 @@ def main():
             if k % 5 == 0:
                 log(f"  iteration {k}: misfit {J:.4g} ({used}, alpha {alpha:.3g}, {nev} evaluations)")
         cap = a.time_cap * rp["seconds"] if a.time_cap else None
-        if a.driver == "lbfgs":
+
+        def diag(net_):
+            # one evaluation at the start gives the scaling, then the plain
+            # method's L-BFGS-B on the scaled variables (both timed)
+            t0 = time.perf_counter()
+            _, g0, ex0 = obj(x0)
+            s = precond.diagonal_scale(g0, ex0, fwi.mask, net_)
+            r = inversion.lbfgs(obj, x0, maxiter=2 * a.maxiter, target_misfit=rp["misfit"],
+                                scale=s)
+            r["seconds"] = time.perf_counter() - t0
+            r["evaluations"] += 1
+            r["history"] = [(t, J, "diag") for t, J in r["history"]]
+            return r
+        if a.driver == "diag":
+            rl = diag(net)
+        elif a.driver == "lbfgs":
             rl = precond.learned_lbfgs(obj, x0, fwi.mask, net=net, maxiter=2 * a.maxiter,
                                        target_misfit=rp["misfit"], on_iter=progress,
                                        max_seconds=cap)
@@ -184,9 +201,12 @@ def main():
                                          target_misfit=rp["misfit"], step0=a.step0)
         log(f"eval {seed}: plain {rp['misfit']:.4g} in {rp['seconds']:.1f} s; "
             f"learned {rl['misfit']:.4g} in {rl['seconds']:.1f} s")
-        rc = precond.learned_lbfgs(obj, x0, fwi.mask, net=None, maxiter=2 * a.maxiter,
-                                   target_misfit=rp["misfit"], on_iter=progress,
-                                   max_seconds=cap)
+        if a.driver == "diag":
+            rc = diag(None)
+        else:
+            rc = precond.learned_lbfgs(obj, x0, fwi.mask, net=None, maxiter=2 * a.maxiter,
+                                       target_misfit=rp["misfit"], on_iter=progress,
+                                       max_seconds=cap)
         log(f"eval {seed}: control {rc['misfit']:.4g} in {rc['seconds']:.1f} s")
         obj.close()
         reached = rl["misfit"] <= rp["misfit"]

@@ -158,7 +158,8 @@ class _Log:
 
 
 def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL,
-          log_dir=None, shape=None, target_misfit=None, on_pair=None, first_step_rms=10.0):
+          log_dir=None, shape=None, target_misfit=None, on_pair=None, first_step_rms=10.0,
+          scale=None):
     """Minimise ``objective`` with L-BFGS-B from ``x0``.
 
     ``objective(x) -> (misfit, gradient, extras)`` with ``extras`` optionally
@@ -177,7 +178,9 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
     then fails.  The variables are therefore scaled so that a unit step is
     `first_step_rms` m/s RMS per cell (gtol then applies to the scaled
     gradient); None runs on cp itself as the reference's
-    ScipyOptimizerInterface does.
+    ScipyOptimizerInterface does.  ``scale``: optional positive factors, one
+    per variable (a diagonal preconditioner: a unit step moves variable i by
+    scale[i] times as much).
 
     Returns a dict: x, misfit, iterations, evaluations, seconds, history
     (per-iteration (seconds, misfit)), scipy's message.
@@ -186,6 +189,8 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
     shape = x0.shape if shape is None else shape
     x0 = x0.ravel()
     k = 1.0 if first_step_rms is None else 1.0 / (float(first_step_rms) * np.sqrt(x0.size))
+    if scale is not None:
+        k = k / np.asarray(scale, dtype=np.float64).ravel()
     lo, hi = bounds
     y0 = x0 * k
     to_x = lambda y: np.clip(np.asarray(y) / k, lo, hi)  # noqa: E731
@@ -239,7 +244,9 @@ def lbfgs(objective, x0, bounds=CP_BOUNDS, maxiter=MAXITER, ftol=FTOL, gtol=GTOL
     res_x, msg = None, ""
     try:
         r = minimize(fun_y, y0, jac=True, method="L-BFGS-B",
-                     bounds=[(lo * k, hi * k)] * x0.size, callback=callback_y,
+                     bounds=list(zip(np.broadcast_to(lo * k, x0.shape),
+                                     np.broadcast_to(hi * k, x0.shape))),
+                     callback=callback_y,
                      options={"maxiter": maxiter, "ftol": ftol, "gtol": gtol})
         res_x, msg = to_x(r.x), str(r.message)
     except _Stop:

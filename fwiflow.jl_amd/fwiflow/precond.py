@@ -130,6 +130,39 @@ def target_from_update(update, mask):
     return t
 
 
+def diagonal_scale(gradient, extras, mask, net=None, smooth=3.0, clip=(0.2, 5.0),
+                   illum_eps=1e-3):
+    """A diagonal preconditioner for ``inversion.lbfgs(scale=...)`` from one
+    evaluation at the starting model: with a network, the ratio of the
+    magnitudes of its predicted update and of the gradient (both over their
+    RMS in the mask) cell by cell, the diagonal inverse Hessian it has
+    learned; without one (the control), 1 / sqrt(illumination / max + eps),
+    the pseudo-Hessian scaling.  Gaussian-smoothed (``smooth`` cells),
+    normalised to mean 1 in the mask and clipped to ``clip``; 1 outside it."""
+    from scipy.ndimage import gaussian_filter
+
+    m2 = np.asarray(mask)
+    il = np.asarray(extras.get("illumination", np.ones(m2.shape)),
+                    dtype=np.float64).reshape(m2.shape)
+    if net is None:
+        s = 1.0 / np.sqrt(np.maximum(il, 0.0) / max(il.max(), 1e-30) + illum_eps)
+    else:
+        grads = extras.get("gradients")
+        g = np.asarray(gradient, dtype=np.float64).reshape(m2.shape) * m2
+        gq = (g,) + tuple(grads[1:] if grads is not None else (np.zeros(m2.shape),) * 2)
+        d = net.forward(features(gq, il, m2)).astype(np.float64)[0] * m2
+        d = np.abs(d) / _rms(d, m2)
+        q = np.abs(g) / _rms(g, m2)
+        s = d / (q + 0.1)
+    if smooth:
+        s = gaussian_filter(s, smooth)
+    inside = m2 != 0
+    s = s / max(float(np.mean(s[inside])), 1e-30)
+    s = np.clip(s, *clip)
+    s[~inside] = 1.0
+    return s
+
+
 def train(net, samples, mask, epochs=10, lr=1e-3, seed=0, log=None):
     """Adam on (features, target) pairs in random order; returns the mean loss
     of each epoch."""

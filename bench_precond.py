@@ -80,6 +80,13 @@ def parse():
     p.add_argument("--time-cap", type=float, default=None,
                    help="stop the learned and control runs at this multiple of the plain "
                         "method's wall time (their ratio is then censored at the cap)")
+    p.add_argument("--eval-seed0", type=int, default=None,
+                   help="seed of held-out model 0 (default seed0 + 1000); the training seeds "
+                        "serve as a validation set for the diagonal's settings")
+    p.add_argument("--diag-clip", type=float, nargs=2, default=(0.2, 5.0))
+    p.add_argument("--diag-smooth", type=float, default=3.0)
+    p.add_argument("--diag-illum-power", type=float, default=0.0)
+    p.add_argument("--no-control", action="store_true", help="skip the control runs")
     p.add_argument("--eval-first", type=int, default=0,
                    help="index of the first held-out model (seed seed0 + 1000 + index): "
                         "evaluation split over several runs")
@@ -165,7 +172,7 @@ def main():
         return
     evals = []
     for k in range(a.eval_first, a.eval_first + a.eval_models):
-        seed = a.seed0 + 1000 + k
+        seed = (a.seed0 + 1000 if a.eval_seed0 is None else a.eval_seed0) + k
         ws = tempfile.mkdtemp(prefix=f"fwiflow_pc_{seed}_")
         fwi, obj, x0 = setup(seed, a.nt, ws, a.adjoint)
         J0 = obj(x0)[0]
@@ -183,7 +190,9 @@ def main():
             # method's L-BFGS-B on the scaled variables (both timed)
             t0 = time.perf_counter()
             _, g0, ex0 = obj(x0)
-            s = precond.diagonal_scale(g0, ex0, fwi.mask, net_)
+            s = precond.diagonal_scale(g0, ex0, fwi.mask, net_, smooth=a.diag_smooth,
+                                       clip=tuple(a.diag_clip),
+                                       illum_power=a.diag_illum_power if net_ else 0.0)
             r = inversion.lbfgs(obj, x0, maxiter=2 * a.maxiter, target_misfit=rp["misfit"],
                                 scale=s)
             r["seconds"] = time.perf_counter() - t0
@@ -201,7 +210,9 @@ def main():
                                          target_misfit=rp["misfit"], step0=a.step0)
         log(f"eval {seed}: plain {rp['misfit']:.4g} in {rp['seconds']:.1f} s; "
             f"learned {rl['misfit']:.4g} in {rl['seconds']:.1f} s")
-        if a.driver == "diag":
+        if a.no_control:
+            rc = {"misfit": float("inf"), "seconds": float("nan")}
+        elif a.driver == "diag":
             rc = diag(None)
         else:
             rc = precond.learned_lbfgs(obj, x0, fwi.mask, net=None, maxiter=2 * a.maxiter,
@@ -241,6 +252,8 @@ def main():
                    "train_models":
                    a.train_models, "eval_models": a.eval_models, "eval_first": a.eval_first,
                    "maxiter": a.maxiter, "time_cap": a.time_cap,
+                   "diag": {"clip": list(a.diag_clip), "smooth": a.diag_smooth,
+                            "illum_power": a.diag_illum_power},
                    "pairs": len(samples), "epochs": a.epochs, "lr": a.lr},
         "training": {"seconds": round(t_train, 1), "loss_first_epoch": hist[0],
                      "loss_last_epoch": hist[-1]},

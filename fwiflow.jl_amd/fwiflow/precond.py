@@ -131,21 +131,24 @@ def target_from_update(update, mask):
 
 
 def diagonal_scale(gradient, extras, mask, net=None, smooth=3.0, clip=(0.2, 5.0),
-                   illum_eps=1e-3):
+                   illum_eps=1e-3, illum_power=0.0):
     """A diagonal preconditioner for ``inversion.lbfgs(scale=...)`` from one
     evaluation at the starting model: with a network, the ratio of the
     magnitudes of its predicted update and of the gradient (both over their
     RMS in the mask) cell by cell, the diagonal inverse Hessian it has
     learned; without one (the control), 1 / sqrt(illumination / max + eps),
-    the pseudo-Hessian scaling.  Gaussian-smoothed (``smooth`` cells),
-    normalised to mean 1 in the mask and clipped to ``clip``; 1 outside it."""
+    the pseudo-Hessian scaling.  ``illum_power`` > 0 multiplies the network's
+    diagonal by the illumination one to that power.  Gaussian-smoothed
+    (``smooth`` cells), normalised to mean 1 in the mask and clipped to
+    ``clip``; 1 outside it."""
     from scipy.ndimage import gaussian_filter
 
     m2 = np.asarray(mask)
     il = np.asarray(extras.get("illumination", np.ones(m2.shape)),
                     dtype=np.float64).reshape(m2.shape)
+    s_il = 1.0 / np.sqrt(np.maximum(il, 0.0) / max(il.max(), 1e-30) + illum_eps)
     if net is None:
-        s = 1.0 / np.sqrt(np.maximum(il, 0.0) / max(il.max(), 1e-30) + illum_eps)
+        s = s_il
     else:
         grads = extras.get("gradients")
         g = np.asarray(gradient, dtype=np.float64).reshape(m2.shape) * m2
@@ -154,6 +157,8 @@ def diagonal_scale(gradient, extras, mask, net=None, smooth=3.0, clip=(0.2, 5.0)
         d = np.abs(d) / _rms(d, m2)
         q = np.abs(g) / _rms(g, m2)
         s = d / (q + 0.1)
+        if illum_power:
+            s = s * s_il ** illum_power
     if smooth:
         s = gaussian_filter(s, smooth)
     inside = m2 != 0

@@ -187,3 +187,48 @@ def test_config5_learned_lbfgs_on_a_held_out_model(workspace):
         assert res["x"].min() >= 1500.0 and res["x"].max() <= 5500.0
     assert rz["history"][0][2] == "fallback"
     assert all(h[2] in ("lbfgs", "h0", "fallback", "restart") for h in rl["history"])
+
+
+@pytest.mark.timeout(600)
+def test_config5_learned_diagonal_preconditioner_on_a_held_out_model(workspace):
+    """The learned diagonal preconditioner with the exact adjoint at its own
+    scale (224x448, nt 2000, 30 shots): a network trained on pairs logged on
+    two training seeds gives, with the illumination diagonal, the variable
+    scaling of L-BFGS-B on a held-out seed (bench_precond.py --driver diag);
+    the scale is finite, within its clip range and mean 1 in the mask, and
+    every accepted iterate lowers the operator's misfit within the bounds."""
+    import bench_precond as B
+    from fwiflow import inversion, precond
+
+    samples, mask = [], None
+    for seed in (100, 101):
+        fwi, obj, x0 = B.setup(seed, 2000, os.path.join(workspace, f"train{seed}"), "exact")
+        mask = fwi.mask
+
+        def on_pair(g, extras, update, J):
+            samples.append((precond.features(extras["gradients"], extras["illumination"], mask),
+                            precond.target_from_update(update, mask)))
+        try:
+            precond.learned_lbfgs(obj, x0, mask, net=None, maxiter=4, on_pair=on_pair)
+        finally:
+            obj.close()
+    fwi, obj, x0 = B.setup(1100, 2000, os.path.join(workspace, "heldout"), "exact")
+    try:
+        J0, g0, ex0 = obj(x0)
+        with CNN(224, 448, seed=0) as net:
+            train(net, samples, mask.astype(np.float32), epochs=3, lr=1e-3, seed=0)
+            s = precond.diagonal_scale(g0, ex0, fwi.mask, net, clip=(0.1, 10.0),
+                                       illum_power=1.0)
+        inside = fwi.mask != 0
+        assert np.all(np.isfinite(s)) and s.min() >= 0.1 and s.max() <= 10.0
+        assert abs(float(s[inside].mean()) - 1.0) < 0.3 and np.all(s[~inside] == 1.0)
+        r = inversion.lbfgs(obj, x0, maxiter=6, scale=s)
+        rp = inversion.lbfgs(obj, x0, maxiter=6)
+    finally:
+        obj.close()
+    ms = [J0] + [h[1] for h in r["history"]]
+    say("held-out diagonal", J0, "->", r["misfit"], "evaluations", r["evaluations"],
+        "plain after as many iterations", rp["misfit"])
+    assert r["iterations"] >= 1 and r["misfit"] < J0
+    assert all(b < a for a, b in zip(ms, ms[1:]))
+    assert r["x"].min() >= 1500.0 and r["x"].max() <= 5500.0

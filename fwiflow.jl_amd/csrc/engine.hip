@@ -12,6 +12,11 @@
 //   boundary ring, 5 layers x 4 sides, 5 components   [shot][step][component][ring slot]
 //   seismograms syn/obs/res, time-major               [nSteps][all receivers]
 //   gradient accumulators (5, summed over shots)      [NX][NZ]
+//   exact adjoint only: transposed-update operands    [launch shot][slot][NX][NZ]
+//
+// Two adjoints: the reference's (el_velocity_adj / el_stress_adj, the default)
+// and the exact one, the transpose of the forward step (a plan option, no
+// reference counterpart; k_exact_a / k_exact_b).
 //
 // Arithmetic.  Every wavefield update evaluates the reference's expression in
 // the reference's order and types, so that the fields follow its FP32

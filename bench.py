@@ -58,7 +58,8 @@ KERNEL_PARTS = {"bwd_pair_a": ("rev_stress", "adj_vel"), "bwd_pair_b": ("adj_str
 # Reported as roofline.traffic when the dominant kernel and its mean shots
 # per launch match the profiled run.
 MEASURED_TRAFFIC = {("bwd_pair_a", 2.91): 231.5e6, ("bwd_pair_b", 2.91): 226.9e6,
-                    ("stress_fwd", 4.92): 204.4e6, ("vel_fwd", 4.92): 180.9e6}
+                    ("stress_fwd", 4.92): 204.4e6, ("vel_fwd", 4.92): 180.9e6,
+                    ("exact_a", 2.0): 161.8e6, ("exact_b", 2.0): 199.1e6}
 # kernels of the backward sweep (their launches cover the backward shot groups)
 BACKWARD = ("rev_vel", "rev_stress", "adj_vel", "adj_stress", "bwd_pair_a", "bwd_pair_b",
             "exact_a", "exact_b")

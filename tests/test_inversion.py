@@ -169,3 +169,40 @@ def test_diagonal_scale_preconditions_lbfgs():
     d = precond.diagonal_scale(np.ones((8, 8)), {"illumination": il}, mask, smooth=0)
     assert abs(d[mask != 0].mean() - 1.0) < 0.2 and np.all(d[mask == 0] == 1.0)
     assert np.all(np.diff(d[2:6, 3]) <= 0)  # less illumination, larger steps
+
+
+class _ScaledNet:
+    """Stands in for the network: predicts the cp update as -w * gradient
+    plane, w a fixed positive field (a known diagonal)."""
+
+    def __init__(self, w):
+        self.w = w
+
+    def forward(self, x):
+        return np.stack([-self.w * x[0], np.zeros_like(x[0]), np.zeros_like(x[0])])
+
+
+def test_diagonal_scale_recovers_a_known_network_diagonal():
+    """A network whose predicted update is -w * gradient implies the diagonal
+    w (up to the mean-1 normalisation) wherever the gradient is well above
+    the 0.1 RMS floor; with illum_power 1 it is multiplied by the
+    illumination diagonal; clip bounds hold."""
+    from fwiflow import precond
+    n = 16
+    mask = np.ones((n, n))
+    w = np.outer(np.linspace(1.0, 3.0, n), np.ones(n))
+    g = np.full((n, n), 5.0)  # gradient RMS 5: |q| = 1 >> 0.1
+    ex = {"illumination": np.ones((n, n)), "gradients": (g, np.zeros((n, n)), np.zeros((n, n)))}
+    s = precond.diagonal_scale(g, ex, mask, _ScaledNet(w), smooth=0, clip=(0.01, 100.0))
+    want = w / w.mean()
+    assert np.allclose(s, want, rtol=1e-6)
+    il = np.outer(np.ones(n), np.linspace(1.0, 4.0, n))
+    ex["illumination"] = il
+    s2 = precond.diagonal_scale(g, ex, mask, _ScaledNet(w), smooth=0, clip=(0.01, 100.0),
+                                illum_power=1.0)
+    s_il = 1.0 / np.sqrt(il / il.max() + 1e-3)
+    want2 = (w * s_il) / (w * s_il).mean()
+    assert np.allclose(s2, want2, rtol=1e-6)
+    s3 = precond.diagonal_scale(g, ex, mask, _ScaledNet(w), smooth=0, clip=(0.9, 1.1),
+                                illum_power=1.0)
+    assert s3.min() >= 0.9 and s3.max() <= 1.1
